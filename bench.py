@@ -1,0 +1,180 @@
+"""Lloyd-iteration throughput on one MI355X.
+
+Workload (default): N=2^27 points, d=4 float32 features (2 GiB, feature-major),
+k=10, Gaussian blobs sigma=1 with centres uniform in [-50,50]^4, seed 0,
+init = first 10 points; one step = one Lloyd iteration of the single_stage
+pipeline (zero accumulators, fused label + mass + feature-sum pass, FP64
+merge, FP64 divide, swap).  Points are resident in HBM before timing starts.
+
+Prints one JSON line: iter/s, the fused kernel's achieved HBM bandwidth
+against the 8 TB/s peak (algorithmic bytes N*(4d+4) per launch over its
+HIP-event-timed average duration), and the multi-threaded CPU restatement
+timed on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "cl-kmeans_amd"))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Lloyd iter/s+%HBM (4f/10c 2GiB); served/batched vs plain solves/s @tol; p99"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--log2n", type=int, default=27)
+    ap.add_argument("--d", type=int, default=4)
+    ap.add_argument("--k", type=int, default=10)
+    ap.add_argument("--pipeline", default="single_stage", choices=["single_stage", "three_stage"])
+    ap.add_argument("--strategy", default="cluster_merge")
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import clkmeans
+    from clkmeans import hip
+
+    n, d, k = 1 << args.log2n, args.d, args.k
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.time()
+    pts, _ = clkmeans.generate_blobs(n, d, k, sigma=1.0, box=50.0, seed=0, threads=threads)
+    log("generated %d x %d points in %.1fs" % (n, d, time.time() - t0))
+    init = np.ascontiguousarray(pts[:, :k])  # first_x
+
+    ctx = clkmeans.Context(0)
+    stream = hip.Stream()
+    dp = hip.DeviceArray.from_host(pts)
+    cen = [hip.DeviceArray.from_host(init), hip.DeviceArray((d, k), np.float32)]
+    sums = hip.DeviceArray((d, k), np.float64)
+    masses = hip.DeviceArray((k,), np.uint32)
+    labels = hip.DeviceArray((n,), np.uint32)
+    fcfg = ctx.config(clkmeans.STAGE_FUSED, args.strategy)
+    lcfg = ctx.config(clkmeans.STAGE_LABELING, "unroll_vector")
+    mcfg = ctx.config(clkmeans.STAGE_MASS_UPDATE, "part_private")
+    ccfg = ctx.config(clkmeans.STAGE_CENTROID_UPDATE, "cluster_merge")
+    s = stream.handle
+
+    def iteration():
+        old, new = cen[0], cen[1]
+        ctx.fill_zero(masses, stream=s)
+        ctx.fill_zero(sums, stream=s)
+        if args.pipeline == "single_stage":
+            ctx.fused(fcfg, d, n, k, dp, old, sums, labels, masses, stream=s)
+        else:
+            ctx.labeling(lcfg, d, n, k, dp, old, labels, stream=s)
+            ctx.mass_update(mcfg, n, k, labels, masses, stream=s)
+            ctx.centroid_update(ccfg, d, n, k, dp, sums, labels, masses, stream=s)
+        ctx.divide(d, k, sums, masses, new, stream=s)
+        cen[0], cen[1] = new, old
+
+    for _ in range(args.warmup):
+        iteration()
+    stream.synchronize()
+    hip.synchronize()
+
+    # Timed region: K iterations, every kernel bracketed by HIP events on the
+    # stream it runs on.
+    ctx.clear_records()
+    ctx.profiling(True)
+    e0, e1 = hip.Event(), hip.Event()
+    w0 = time.perf_counter()
+    e0.record(stream)
+    for it in range(args.steps):
+        ctx.set_iteration(it)
+        iteration()
+    e1.record(stream)
+    e1.synchronize()
+    hip.synchronize()
+    wall = time.perf_counter() - w0
+    ctx.profiling(False)
+    total_ms = e0.elapsed_ms(e1)
+    recs = ctx.records()
+    ms_per_step = total_ms / args.steps
+    by_name = {}
+    for name, _, ms in recs:
+        by_name.setdefault(name, []).append(ms)
+    hot = "FusedClusterMerge" if args.pipeline == "single_stage" else "LabelingUnrollVector"
+    if args.pipeline == "single_stage" and args.strategy == "feature_sum":
+        hot = "FusedFeatureSum"
+    hot_ms = float(np.mean(by_name[hot])) if hot in by_name else float("nan")
+    bytes_per_launch = n * (4 * d + 4) if args.pipeline == "single_stage" else n * (4 * d + 4)
+    achieved = bytes_per_launch / (hot_ms * 1e-3) / 1e9
+    log("kernels (avg ms): " + ", ".join("%s=%.4f x%d" % (kk, np.mean(v), len(v)) for kk, v in by_name.items()))
+
+    # Parity spot check of one more step: labels of a sample vs the oracle.
+    from oracle import oracle as O
+    c_in = cen[0].to_host()
+    iteration()
+    stream.synchronize()
+    lab = labels.to_host()
+    rng = np.random.default_rng(0)
+    idx = np.sort(rng.choice(n, size=min(n, 1 << 16), replace=False))
+    ref = O.label(np.ascontiguousarray(pts[:, idx]), c_in)
+    parity = bool(np.array_equal(lab[idx], ref))
+    mass_ok = bool(int(masses.to_host().astype(np.int64).sum()) == n)
+
+    cpu = None
+    if not args.no_cpu:
+        iters = max(1, args.cpu_iters)
+        t = time.perf_counter()
+        O.lloyd_mt(pts, init, iters, threads)
+        dt = time.perf_counter() - t
+        cpu = {"value": iters / dt, "unit": "iter/s", "cores": threads, "kind": "port",
+               "sample": "%d full iterations of the same %d-point workload, fused cluster_merge "
+                         "restatement (oracle/kmeans_oracle.c ora_lloyd_mt_f32), %d threads" % (iters, n, threads)}
+
+    traffic = None
+    pmc = os.path.join(REPO, "profiles", "fused_cluster_merge_n2p27_d4_k10_pmc.json")
+    if os.path.exists(pmc) and args.pipeline == "single_stage" and n == (1 << 27) and d == 4 and k == 10:
+        with open(pmc) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": 1000.0 / ms_per_step,
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain fused Lloyd, N=2^%d d=%d k=%d float32 feature-major, sigma=1 blobs "
+                               "in [-50,50]^%d, seed 0, first-x init" % (args.log2n, d, k, d),
+                   "pipeline": args.pipeline, "strategy": args.strategy if args.pipeline == "single_stage" else
+                   "unroll_vector/part_private/cluster_merge", "n": n, "d": d, "k": k},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": hot,
+                     "kernel_ms": hot_ms, "bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+        "wall_s": wall,
+        "parity": {"labels_sample": int(idx.size), "labels_equal": parity, "mass_sum_equal_n": mass_ok},
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,240 @@
+"""ctypes binding of libclkmeans.so (the C ABI declared in include/clkmeans.h)."""
+import ctypes
+import os
+
+import numpy as np
+
+from . import hip
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libclkmeans.so")
+
+CLK_OK = 0
+CLK_ERR_INVALID_ARGUMENT = -1
+CLK_ERR_SIZE = -2
+CLK_ERR_HIP = -3
+CLK_ERR_UNSUPPORTED = -4
+CLK_ERR_NOMEM = -5
+
+STAGE_FUSED, STAGE_LABELING, STAGE_MASS_UPDATE, STAGE_CENTROID_UPDATE = 0, 1, 2, 3
+
+
+class ClkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("clk error %d: %s" % (code, msg))
+        self.code = code
+
+
+class InvalidArgument(ClkError, ValueError):
+    """The reference throws std::invalid_argument in these cases."""
+
+
+class SizeMismatch(ClkError, AssertionError):
+    """The reference asserts in these cases."""
+
+
+class KernelConfig(ctypes.Structure):
+    _fields_ = [
+        ("stage", ctypes.c_int32),
+        ("strategy", ctypes.c_int32),
+        ("global_size", ctypes.c_uint64 * 3),
+        ("local_size", ctypes.c_uint64 * 3),
+        ("vector_length", ctypes.c_uint64),
+        ("local_features", ctypes.c_uint64),
+        ("thread_features", ctypes.c_uint64),
+    ]
+
+
+_lib = None
+
+EXPORTS = [
+    "clk_last_error", "clk_context_create", "clk_context_destroy", "clk_strategy_id",
+    "clk_default_config", "clk_profiling_enable", "clk_profiling_set_iteration",
+    "clk_profiling_clear", "clk_profiling_count", "clk_profiling_get", "clk_fused",
+    "clk_labeling", "clk_mass_update", "clk_centroid_update", "clk_matrix_divide_row",
+    "clk_matrix_add_u32", "clk_matrix_add_f64", "clk_reduce_vector_parcol_u32",
+    "clk_reduce_vector_parcol_f64", "clk_fill_zero", "clk_init_first_x", "clk_generate_blobs",
+]
+
+
+def load(path=None):
+    """Load libclkmeans.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise ImportError("%s is not built; run __graft_entry__.build() (make -C cl-kmeans_amd)" % path)
+    hip.lib()  # the HIP runtime first, so both resolve to one instance
+    L = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    vp, u64, i32, dp = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_double
+    cfgp = ctypes.POINTER(KernelConfig)
+    sig = {
+        "clk_last_error": (ctypes.c_char_p, []),
+        "clk_context_create": (i32, [i32, ctypes.POINTER(vp)]),
+        "clk_context_destroy": (i32, [vp]),
+        "clk_strategy_id": (i32, [i32, ctypes.c_char_p]),
+        "clk_default_config": (i32, [vp, i32, i32, cfgp]),
+        "clk_profiling_enable": (i32, [vp, i32]),
+        "clk_profiling_set_iteration": (i32, [vp, i32]),
+        "clk_profiling_clear": (i32, [vp]),
+        "clk_profiling_count": (ctypes.c_int64, [vp]),
+        "clk_profiling_get": (i32, [vp, ctypes.c_int64, ctypes.c_char_p, ctypes.c_size_t,
+                                    ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float)]),
+        "clk_fused": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp, vp, vp]),
+        "clk_labeling": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp]),
+        "clk_mass_update": (i32, [vp, cfgp, u64, u64, vp, vp, vp]),
+        "clk_centroid_update": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp, vp]),
+        "clk_matrix_divide_row": (i32, [vp, u64, u64, vp, vp, vp, vp, vp, vp]),
+        "clk_matrix_add_u32": (i32, [vp, u64, vp, vp, vp]),
+        "clk_matrix_add_f64": (i32, [vp, u64, vp, vp, vp]),
+        "clk_reduce_vector_parcol_u32": (i32, [vp, u64, u64, vp, vp]),
+        "clk_reduce_vector_parcol_f64": (i32, [vp, u64, u64, vp, vp]),
+        "clk_fill_zero": (i32, [vp, vp, u64, vp]),
+        "clk_init_first_x": (i32, [vp, u64, u64, u64, vp, vp, vp]),
+        "clk_generate_blobs": (i32, [u64, u64, u64, dp, dp, u64, vp, vp, i32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype, fn.argtypes = res, args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc == CLK_OK:
+        return
+    msg = load().clk_last_error().decode(errors="replace")
+    if rc == CLK_ERR_INVALID_ARGUMENT:
+        raise InvalidArgument(rc, msg)
+    if rc == CLK_ERR_SIZE:
+        raise SizeMismatch(rc, msg)
+    raise ClkError(rc, msg)
+
+
+def strategy_id(stage, name):
+    """Factory strategy string -> id; InvalidArgument for unknown names."""
+    rc = load().clk_strategy_id(stage, name.encode())
+    if rc < 0:
+        check(rc)
+    return rc
+
+
+def generate_blobs(n, d, k, sigma=1.0, box=50.0, seed=0, threads=None):
+    """Feature-major float32 points [d][n] and blob centres [d][k] (host)."""
+    threads = threads or min(16, os.cpu_count() or 1)
+    pts = np.empty((d, n), np.float32)
+    cen = np.empty((d, k), np.float32)
+    check(load().clk_generate_blobs(n, d, k, float(sigma), float(box), int(seed),
+                                    pts.ctypes.data, cen.ctypes.data, int(threads)))
+    return pts, cen
+
+
+def _p(x):
+    """Device address of a DeviceArray / int / c_void_p / None."""
+    if x is None:
+        return None
+    if isinstance(x, hip.DeviceArray):
+        return x.ptr
+    if isinstance(x, ctypes.c_void_p):
+        return x
+    return ctypes.c_void_p(int(x))
+
+
+def _s(stream):
+    if stream is None:
+        return None
+    return stream.handle if isinstance(stream, hip.Stream) else stream
+
+
+class Context:
+    """One clk_context (owns scratch; one host thread at a time)."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        self.handle = ctypes.c_void_p()
+        check(self.L.clk_context_create(device, ctypes.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            self.L.clk_context_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def config(self, stage, strategy, **kw):
+        """A KernelConfig from a strategy name (or id) plus overrides."""
+        sid = strategy_id(stage, strategy) if isinstance(strategy, str) else int(strategy)
+        cfg = KernelConfig()
+        check(self.L.clk_default_config(self.handle, stage, sid, ctypes.byref(cfg)))
+        for key, val in kw.items():
+            if key in ("global_size", "local_size"):
+                vals = list(val) if isinstance(val, (list, tuple)) else [val]
+                arr = getattr(cfg, key)
+                for j in range(3):
+                    arr[j] = int(vals[j]) if j < len(vals) else (1 if vals and vals[0] else (0 if j == 0 else 1))
+            else:
+                setattr(cfg, key, int(val))
+        return cfg
+
+    # -- operators (device buffers, enqueue on `stream`) ---------------------
+    def fused(self, cfg, d, n, k, points, old_centroids, sums, labels, masses, stream=None):
+        check(self.L.clk_fused(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),
+                               _p(old_centroids), _p(sums), _p(labels), _p(masses), _s(stream)))
+
+    def labeling(self, cfg, d, n, k, points, centroids, labels, stream=None):
+        check(self.L.clk_labeling(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),
+                                  _p(centroids), _p(labels), _s(stream)))
+
+    def mass_update(self, cfg, n, k, labels, masses, stream=None):
+        check(self.L.clk_mass_update(self.handle, ctypes.byref(cfg) if cfg else None, n, k, _p(labels),
+                                     _p(masses), _s(stream)))
+
+    def centroid_update(self, cfg, d, n, k, points, sums, labels, masses=None, stream=None):
+        check(self.L.clk_centroid_update(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k,
+                                         _p(points), _p(sums), _p(labels), _p(masses), _s(stream)))
+
+    def divide(self, d, k, sums, masses, centroids, old_centroids=None, max_shift=None, stream=None):
+        check(self.L.clk_matrix_divide_row(self.handle, d, k, _p(sums), _p(masses), _p(centroids),
+                                           _p(old_centroids), _p(max_shift), _s(stream)))
+
+    def matrix_add(self, count, fst, snd, stream=None):
+        fn = self.L.clk_matrix_add_f64 if fst.dtype == np.float64 else self.L.clk_matrix_add_u32
+        check(fn(self.handle, count, _p(fst), _p(snd), _s(stream)))
+
+    def reduce_vector_parcol(self, num_cols, rows, data, stream=None):
+        fn = (self.L.clk_reduce_vector_parcol_f64 if data.dtype == np.float64
+              else self.L.clk_reduce_vector_parcol_u32)
+        check(fn(self.handle, num_cols, rows, _p(data), _s(stream)))
+
+    def fill_zero(self, buf, nbytes=None, stream=None):
+        nbytes = buf.nbytes if nbytes is None else nbytes
+        check(self.L.clk_fill_zero(self.handle, _p(buf), nbytes, _s(stream)))
+
+    def first_x(self, d, n, k, points, centroids, stream=None):
+        check(self.L.clk_init_first_x(self.handle, d, n, k, _p(points), _p(centroids), _s(stream)))
+
+    # -- timing records ------------------------------------------------------
+    def profiling(self, enable=True):
+        check(self.L.clk_profiling_enable(self.handle, 1 if enable else 0))
+
+    def set_iteration(self, it):
+        check(self.L.clk_profiling_set_iteration(self.handle, int(it)))
+
+    def records(self):
+        n = self.L.clk_profiling_count(self.handle)
+        out = []
+        name = ctypes.create_string_buffer(128)
+        it = ctypes.c_int32()
+        ms = ctypes.c_float()
+        for i in range(n):
+            check(self.L.clk_profiling_get(self.handle, i, name, 128, ctypes.byref(it), ctypes.byref(ms)))
+            out.append((name.value.decode(), it.value, ms.value))
+        return out
+
+    def clear_records(self):
+        check(self.L.clk_profiling_clear(self.handle))

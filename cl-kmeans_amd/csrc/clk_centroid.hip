@@ -1,0 +1,259 @@
+// Centroid update: sums[f,c] += sum_{p : labels[p] == c} x_f, the three
+// strategies of CentroidUpdateFactory built for gfx950:
+//   cluster_merge       one LDS tile column [k*d] per work-item (no
+//                       collisions), all features of the work-item's points
+//                       (lloyd_cluster_merge.cl's private k x d tile)
+//   feature_sum_pardim  2-D grid: blockIdx.y selects a slice of
+//                       local_features*thread_features features, threadIdx.y
+//                       a thread_features sub-slice; private LDS columns
+//                       (lloyd_feature_sum_pardim.cl's points x feature tiles)
+//   feature_sum         one LDS tile [d][k] shared by the work-group, work-
+//                       items stride over the flattened feature-major matrix
+//                       (coalesced rows), ds_add_f32 resolves collisions; the
+//                       smallest LDS footprint, for large k*d
+// Tiles are summed per work-group in FP64 and merged by a fixed-order column
+// reduction, except feature_sum whose in-tile FP32 addition order is the
+// hardware's.  When no tile fits in LDS the FP64 global-atomic pass is used.
+// `masses` is not read, as in the reference kernels.
+#include "clk_common.hpp"
+#include "clk_kernels.hpp"
+
+#include <algorithm>
+
+namespace clk {
+namespace {
+
+template <int DMAX, int VEC, int BLOCK>
+__global__ void __launch_bounds__(BLOCK > 0 ? BLOCK : 1024)
+cu_cluster_merge_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
+                        const uint32_t *__restrict__ labels, double *__restrict__ part)
+{
+    extern __shared__ __align__(16) float tile[];  // [k*d][B]
+    const uint32_t B = BLOCK > 0 ? BLOCK : blockDim.x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t rows = k * d;
+    for (uint32_t i = tid; i < rows * B; i += B)
+        tile[i] = 0.0f;
+    __syncthreads();
+    float *mine = tile + tid;
+    const uint32_t row_stride = d * B;
+    const uint64_t groups = n / VEC;
+    const uint64_t stride = (uint64_t)gridDim.x * B;
+    for (uint64_t g = (uint64_t)blockIdx.x * B + tid; g < groups; g += stride) {
+        const uint64_t p = g * VEC;
+        float x[DMAX][VEC];
+        load_points<DMAX, VEC>(pts, n, d, p, x);
+        uint32_t lab[VEC];
+        load_labels<VEC>(labels, p, lab);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            if (lab[v] >= k)
+                continue;
+            float *t = mine + __umul24(lab[v], row_stride);
+#pragma unroll
+            for (int f = 0; f < DMAX; ++f)
+                if (f < (int)d)
+                    lds_acc(t + f * B, x[f][v]);
+        }
+    }
+    __syncthreads();
+    for (uint32_t r = tid; r < rows; r += B) {
+        const uint32_t c = r / d, f = r % d;
+        part[(uint64_t)blockIdx.x * rows + (uint64_t)f * k + c] =
+            lds_row_sum_f32(tile + (uint64_t)r * B, B, r);
+    }
+}
+
+// blockDim = (Bx, lf); block (x, y) owns features [y*FS, y*FS+FS), FS = lf*tf;
+// thread (tx, ty) owns features y*FS + ty*tf + [0, tf) of points
+// x*Bx + tx + i*gridDim.x*Bx.  LDS rows c*FS + (ty*tf + j), columns tx.
+__global__ void __launch_bounds__(1024)
+cu_pardim_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k, uint32_t tf,
+                 const uint32_t *__restrict__ labels, double *__restrict__ part)
+{
+    extern __shared__ __align__(16) float tile[];
+    const uint32_t Bx = blockDim.x, lf = blockDim.y;
+    const uint32_t FS = lf * tf;
+    const uint32_t f0 = blockIdx.y * FS;
+    const uint32_t tlin = threadIdx.y * Bx + threadIdx.x;
+    const uint32_t nthreads = Bx * lf;
+    const uint32_t rows = k * FS;
+    for (uint32_t i = tlin; i < rows * Bx; i += nthreads)
+        tile[i] = 0.0f;
+    __syncthreads();
+    const uint32_t fsub = threadIdx.y * tf;  // first feature of this thread in the slice
+    const uint64_t stride = (uint64_t)gridDim.x * Bx;
+    for (uint64_t p = (uint64_t)blockIdx.x * Bx + threadIdx.x; p < n; p += stride) {
+        const uint32_t c = labels[p];
+        if (c >= k)
+            continue;
+        float *t = tile + ((uint64_t)c * FS + fsub) * Bx + threadIdx.x;
+        for (uint32_t j = 0; j < tf; ++j) {
+            const uint32_t f = f0 + fsub + j;
+            if (f < d)
+                lds_acc(t + j * Bx, pts[(uint64_t)f * n + p]);
+        }
+    }
+    __syncthreads();
+    for (uint32_t r = tlin; r < rows; r += nthreads) {
+        const uint32_t c = r / FS, f = f0 + r % FS;
+        if (f < d)
+            part[(uint64_t)blockIdx.x * ((uint64_t)d * k) + (uint64_t)f * k + c] =
+                lds_row_sum_f32(tile + (uint64_t)r * Bx, Bx, r);
+    }
+}
+
+// Shared [d][k] tile; work-items stride over elements e = f*n + p.
+__global__ void __launch_bounds__(1024)
+cu_feature_sum_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
+                      const uint32_t *__restrict__ labels, double *__restrict__ part)
+{
+    extern __shared__ __align__(16) float tile[];
+    const uint32_t rows = d * k;
+    for (uint32_t i = threadIdx.x; i < rows; i += blockDim.x)
+        tile[i] = 0.0f;
+    __syncthreads();
+    const uint64_t total = n * d;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        const uint64_t f = e / n, p = e - f * n;
+        const uint32_t c = labels[p];
+        if (c < k)
+            lds_add(tile + f * k + c, pts[e]);
+    }
+    __syncthreads();
+    for (uint32_t r = threadIdx.x; r < rows; r += blockDim.x)
+        part[(uint64_t)blockIdx.x * rows + r] = (double)tile[r];
+}
+
+using CmKernel = void (*)(const float *, uint64_t, uint32_t, uint32_t, const uint32_t *, double *);
+
+template <int DMAX>
+CmKernel pick_cm_vec(int vec, bool b256)
+{
+    if (b256)
+        return vec == 4 ? cu_cluster_merge_kernel<DMAX, 4, 256>
+             : vec == 2 ? cu_cluster_merge_kernel<DMAX, 2, 256>
+                        : cu_cluster_merge_kernel<DMAX, 1, 256>;
+    return vec == 4 ? cu_cluster_merge_kernel<DMAX, 4, 0>
+         : vec == 2 ? cu_cluster_merge_kernel<DMAX, 2, 0>
+                    : cu_cluster_merge_kernel<DMAX, 1, 0>;
+}
+
+CmKernel pick_cm(int dmax, int vec, bool b256)
+{
+    switch (dmax) {
+    case 1: return pick_cm_vec<1>(vec, b256);
+    case 2: return pick_cm_vec<2>(vec, b256);
+    case 4: return pick_cm_vec<4>(vec, b256);
+    case 8: return pick_cm_vec<8>(vec, b256);
+    case 16: return pick_cm_vec<16>(vec, b256);
+    case 32: return pick_cm_vec<32>(vec, b256);
+    case 64: return pick_cm_vec<64>(vec, b256);
+    default: return nullptr;
+    }
+}
+
+}  // namespace
+
+int run_centroid_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64_t n,
+                        uint64_t k, const float *points, double *sums, const uint32_t *labels,
+                        hipStream_t stream)
+{
+    CLK_REQUIRE(d >= 1 && k >= 1 && k <= 0xFFFFFFFFull && d <= 0xFFFFull, CLK_ERR_SIZE,
+                "centroid_update: bad d or k");
+    CLK_REQUIRE(points && sums && labels, CLK_ERR_INVALID_ARGUMENT, "centroid_update: null buffer");
+    if (n == 0)
+        return CLK_OK;
+    const int strategy = cfg ? cfg->strategy : CLK_CENTROID_CLUSTER_MERGE;
+    CLK_REQUIRE(strategy >= CLK_CENTROID_FEATURE_SUM && strategy <= CLK_CENTROID_CLUSTER_MERGE,
+                CLK_ERR_INVALID_ARGUMENT, "centroid_update: unknown strategy id");
+    const uint64_t B = (cfg && cfg->local_size[0]) ? cfg->local_size[0] : 256;
+    CLK_REQUIRE(B <= (uint64_t)ctx->max_threads_per_block, CLK_ERR_INVALID_ARGUMENT,
+                "centroid_update: local_size exceeds the device limit");
+    const uint64_t user_grid = (cfg && cfg->global_size[0]) ? div_up(cfg->global_size[0], B) : 0;
+    const uint64_t dk = d * k;
+    const int dmax = dmax_bucket(d);
+
+    const void *kern = nullptr;
+    const char *name = nullptr;
+    size_t lds = 0;
+    dim3 grid, block;
+    uint32_t tf = 1;
+    uint64_t gx = 0;
+
+    if (strategy == CLK_CENTROID_CLUSTER_MERGE && dmax && dk * B * 4 <= ctx->lds_per_block) {
+        const int vec = pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels);
+        CmKernel kc = pick_cm(dmax, vec, B == 256);
+        lds = dk * B * 4;
+        gx = user_grid ? user_grid : auto_grid(ctx, (const void *)kc, (int)B, lds, n / vec);
+        gx = std::max<uint64_t>(1, std::min<uint64_t>(gx, 1u << 20));
+        CLK_TRY(ensure_scratch(ctx->part_sums, gx * dk * sizeof(double)));
+        double *ps = static_cast<double *>(ctx->part_sums.ptr);
+        {
+            KernelTimer t(ctx, "CentroidUpdateClusterMerge", stream);
+            hipLaunchKernelGGL(kc, dim3((unsigned)gx), dim3((unsigned)B), lds, stream, points, n,
+                               (uint32_t)d, (uint32_t)k, labels, ps);
+            CLK_HIP_CHECK(hipGetLastError());
+        }
+        return launch_reduce_f64(ctx, ps, gx, dk, sums, true, stream);
+    }
+    if (strategy == CLK_CENTROID_FEATURE_SUM_PARDIM) {
+        uint64_t lf = (cfg && cfg->local_features) ? cfg->local_features : 1;
+        tf = (uint32_t)((cfg && cfg->thread_features) ? cfg->thread_features : 1);
+        lf = std::min<uint64_t>(lf, d);
+        uint64_t bx = std::max<uint64_t>(1, B / lf);
+        uint64_t fs = lf * tf;
+        lds = k * fs * bx * 4;
+        if (lds <= ctx->lds_per_block && bx * lf <= (uint64_t)ctx->max_threads_per_block) {
+            kern = (const void *)cu_pardim_kernel;
+            block = dim3((unsigned)bx, (unsigned)lf);
+            uint64_t gy = div_up(d, fs);
+            gx = user_grid ? user_grid : auto_grid(ctx, kern, (int)(bx * lf), lds, n);
+            gx = std::max<uint64_t>(1, std::min<uint64_t>(div_up(gx, gy), 1u << 20));
+            grid = dim3((unsigned)gx, (unsigned)gy);
+            name = "CentroidUpdateFeatureSumPardim";
+        }
+    }
+    if (!kern && (strategy == CLK_CENTROID_FEATURE_SUM || strategy == CLK_CENTROID_CLUSTER_MERGE ||
+                  strategy == CLK_CENTROID_FEATURE_SUM_PARDIM) &&
+        dk * 4 <= ctx->lds_per_block) {
+        kern = (const void *)cu_feature_sum_kernel;
+        lds = dk * 4;
+        block = dim3((unsigned)B);
+        gx = user_grid ? user_grid : auto_grid(ctx, kern, (int)B, lds, n * d);
+        gx = std::max<uint64_t>(1, std::min<uint64_t>(gx, 1u << 20));
+        grid = dim3((unsigned)gx);
+        name = "CentroidUpdateFeatureSum";
+    }
+    if (!kern)
+        return launch_accumulate_global(ctx, d, n, k, points, labels, sums, nullptr, stream);
+
+    CLK_TRY(ensure_scratch(ctx->part_sums, gx * dk * sizeof(double)));
+    double *ps = static_cast<double *>(ctx->part_sums.ptr);
+    {
+        KernelTimer t(ctx, name, stream);
+        if (kern == (const void *)cu_pardim_kernel)
+            hipLaunchKernelGGL(cu_pardim_kernel, grid, block, lds, stream, points, n, (uint32_t)d,
+                               (uint32_t)k, tf, labels, ps);
+        else
+            hipLaunchKernelGGL(cu_feature_sum_kernel, grid, block, lds, stream, points, n,
+                               (uint32_t)d, (uint32_t)k, labels, ps);
+        CLK_HIP_CHECK(hipGetLastError());
+    }
+    return launch_reduce_f64(ctx, ps, gx, dk, sums, true, stream);
+}
+
+}  // namespace clk
+
+extern "C" int clk_centroid_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
+                                   uint64_t n, uint64_t k, const float *points, double *sums,
+                                   const uint32_t *labels, const uint32_t *masses, void *stream)
+{
+    (void)masses;
+    CLK_TRY(clk::enter(ctx));
+    CLK_REQUIRE(!cfg || cfg->stage == CLK_STAGE_CENTROID_UPDATE, CLK_ERR_INVALID_ARGUMENT,
+                "clk_centroid_update: config is not a centroid-update-stage config");
+    return clk::run_centroid_update(ctx, cfg, d, n, k, points, sums, labels,
+                                    clk::as_stream(stream));
+}

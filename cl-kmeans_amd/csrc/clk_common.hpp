@@ -1,0 +1,277 @@
+// Shared host/device machinery of the gfx950 Lloyd kernels: error reporting,
+// the context (scratch buffers, device limits, per-kernel timing records),
+// launch helpers and the device-side distance / argmin / reduction helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "clkmeans.h"
+
+namespace clk {
+
+// ---------------------------------------------------------------- errors --
+
+void set_error(const std::string &msg);
+
+#define CLK_HIP_CHECK(call)                                                      \
+    do {                                                                         \
+        hipError_t clk_e_ = (call);                                              \
+        if (clk_e_ != hipSuccess) {                                              \
+            ::clk::set_error(std::string(#call) + ": " + hipGetErrorString(clk_e_)); \
+            return CLK_ERR_HIP;                                                  \
+        }                                                                        \
+    } while (0)
+
+#define CLK_REQUIRE(cond, code, msg)                                             \
+    do {                                                                         \
+        if (!(cond)) {                                                           \
+            ::clk::set_error(msg);                                               \
+            return code;                                                         \
+        }                                                                        \
+    } while (0)
+
+#define CLK_TRY(expr)                                                            \
+    do {                                                                         \
+        int clk_rc_ = (expr);                                                    \
+        if (clk_rc_ != CLK_OK)                                                   \
+            return clk_rc_;                                                      \
+    } while (0)
+
+// --------------------------------------------------------------- context --
+
+struct Scratch {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct TimingRecord {
+    std::string name;
+    int32_t iteration;
+    hipEvent_t start;
+    hipEvent_t stop;
+};
+
+}  // namespace clk
+
+struct clk_context {
+    int device = 0;
+    int num_cus = 0;
+    size_t lds_per_block = 0;     // bytes a work-group may allocate
+    int max_threads_per_block = 0;
+    clk::Scratch part_sums;       // per-block FP64 partial sums
+    clk::Scratch part_masses;     // per-block uint32 partial masses
+    bool profiling = false;
+    int32_t iteration = -1;
+    std::vector<clk::TimingRecord> records;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace clk {
+
+// Make the context's device current; checks arguments common to all ops.
+int enter(clk_context *ctx);
+
+// Grow a scratch buffer to at least `bytes` (hipMalloc; never shrinks).
+int ensure_scratch(Scratch &s, size_t bytes);
+
+// Bracket one kernel launch with timing events when profiling is on.
+struct KernelTimer {
+    KernelTimer(clk_context *ctx, const char *name, hipStream_t stream);
+    ~KernelTimer();
+    clk_context *ctx;
+    hipStream_t stream;
+    int64_t rec = -1;
+};
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+__host__ __device__ inline uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Largest vector width <= requested (1,2,4,8) such that every feature row of
+// `n` floats starting at `p` stays aligned to width*4 bytes, and `labels`
+// too.  Rows start at f*n, so n must be a multiple of the width.
+int pick_vector_width(uint64_t requested, uint64_t n, const void *points,
+                      const void *labels);
+
+// Persistent-style grid: blocks_per_cu resident blocks on every CU, capped by
+// the work available.
+uint64_t auto_grid(clk_context *ctx, const void *kernel, int block,
+                   size_t dyn_lds, uint64_t work_items);
+
+// ----------------------------------------------------------- device side --
+
+constexpr int kWave = 64;
+
+// Point x (VEC consecutive points of DMAX features; features >= d read as 0)
+// against every centroid staged in LDS as cen[c * DMAX + f] (padded with 0).
+// Distances are the reference's sequential fma over f = 0..d-1 of (x_f-c_f)^2
+// from 0 (lloyd_fused_feature_sum.cl:213-239); padded features add
+// fma(0, 0, dist) == dist exactly.  Strict '<' against a running minimum
+// starting at FLT_MAX; label 0 when nothing is smaller (kmeans_naive.cpp:49-63).
+template <int DMAX, int VEC>
+__device__ __forceinline__ void argmin_lds(const float (&x)[DMAX][VEC],
+                                           const float *__restrict__ cen,
+                                           uint32_t k, uint32_t (&lab)[VEC])
+{
+    float best[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+        best[v] = FLT_MAX;
+        lab[v] = 0;
+    }
+    for (uint32_t c = 0; c < k; ++c) {
+        float cc[DMAX];
+#pragma unroll
+        for (int f = 0; f < DMAX; ++f)
+            cc[f] = cen[c * DMAX + f];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            float t = x[0][v] - cc[0];
+            float dist = t * t;
+#pragma unroll
+            for (int f = 1; f < DMAX; ++f) {
+                t = x[f][v] - cc[f];
+                dist = __builtin_fmaf(t, t, dist);
+            }
+            bool lt = dist < best[v];
+            best[v] = lt ? dist : best[v];
+            lab[v] = lt ? c : lab[v];
+        }
+    }
+}
+
+// Load VEC consecutive points (p % VEC == 0, rows aligned) of DMAX features;
+// features >= d are 0.  EXACT: d == DMAX (no per-feature guards).
+template <int DMAX, int VEC, bool EXACT = false>
+__device__ __forceinline__ void load_points(const float *__restrict__ pts,
+                                            uint64_t n, uint32_t d, uint64_t p,
+                                            float (&x)[DMAX][VEC])
+{
+#pragma unroll
+    for (int f = 0; f < DMAX; ++f) {
+        if (EXACT || f < (int)d) {
+            const float *row = pts + (uint64_t)f * n + p;
+            if constexpr (VEC == 1) {
+                x[f][0] = row[0];
+            } else if constexpr (VEC == 2) {
+                float2 a = *reinterpret_cast<const float2 *>(row);
+                x[f][0] = a.x; x[f][1] = a.y;
+            } else if constexpr (VEC == 4) {
+                float4 a = *reinterpret_cast<const float4 *>(row);
+                x[f][0] = a.x; x[f][1] = a.y; x[f][2] = a.z; x[f][3] = a.w;
+            } else {
+                float4 a = *reinterpret_cast<const float4 *>(row);
+                float4 b = *reinterpret_cast<const float4 *>(row + 4);
+                x[f][0] = a.x; x[f][1] = a.y; x[f][2] = a.z; x[f][3] = a.w;
+                x[f][4] = b.x; x[f][5] = b.y; x[f][6] = b.z; x[f][7] = b.w;
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < VEC; ++v)
+                x[f][v] = 0.0f;
+        }
+    }
+}
+
+template <int VEC>
+__device__ __forceinline__ void store_labels(uint32_t *__restrict__ labels,
+                                             uint64_t p, const uint32_t (&lab)[VEC])
+{
+    uint32_t *dst = labels + p;
+    if constexpr (VEC == 1) {
+        dst[0] = lab[0];
+    } else if constexpr (VEC == 2) {
+        *reinterpret_cast<uint2 *>(dst) = make_uint2(lab[0], lab[1]);
+    } else if constexpr (VEC == 4) {
+        *reinterpret_cast<uint4 *>(dst) = make_uint4(lab[0], lab[1], lab[2], lab[3]);
+    } else {
+        *reinterpret_cast<uint4 *>(dst) = make_uint4(lab[0], lab[1], lab[2], lab[3]);
+        *reinterpret_cast<uint4 *>(dst + 4) = make_uint4(lab[4], lab[5], lab[6], lab[7]);
+    }
+}
+
+template <int VEC>
+__device__ __forceinline__ void load_labels(const uint32_t *__restrict__ labels,
+                                            uint64_t p, uint32_t (&lab)[VEC])
+{
+    const uint32_t *src = labels + p;
+    if constexpr (VEC == 1) {
+        lab[0] = src[0];
+    } else if constexpr (VEC == 2) {
+        uint2 a = *reinterpret_cast<const uint2 *>(src);
+        lab[0] = a.x; lab[1] = a.y;
+    } else if constexpr (VEC == 4) {
+        uint4 a = *reinterpret_cast<const uint4 *>(src);
+        lab[0] = a.x; lab[1] = a.y; lab[2] = a.z; lab[3] = a.w;
+    } else {
+        uint4 a = *reinterpret_cast<const uint4 *>(src);
+        uint4 b = *reinterpret_cast<const uint4 *>(src + 4);
+        lab[0] = a.x; lab[1] = a.y; lab[2] = a.z; lab[3] = a.w;
+        lab[4] = b.x; lab[5] = b.y; lab[6] = b.z; lab[7] = b.w;
+    }
+}
+
+// Stage centroids [d][k] (global) into LDS as [c][DMAX], zero-padded.
+template <int DMAX>
+__device__ __forceinline__ void stage_centroids(const float *__restrict__ cen,
+                                                uint32_t d, uint32_t k,
+                                                float *__restrict__ lds)
+{
+    for (uint32_t i = threadIdx.x; i < k * DMAX; i += blockDim.x) {
+        uint32_t c = i / DMAX, f = i % DMAX;
+        lds[i] = (f < d) ? cen[(uint64_t)f * k + c] : 0.0f;
+    }
+}
+
+// Add into a work-item-private LDS cell: plain read-add-write (no other
+// work-item touches the address).  On gfx950 this is ~5x faster than
+// ds_add_f32 for the fused pass.
+__device__ __forceinline__ void lds_acc(float *p, float v) { *p = *p + v; }
+__device__ __forceinline__ void lds_acc(uint32_t *p, uint32_t v) { *p = *p + v; }
+
+// LDS float / uint atomics without return value (ds_add_f32 / ds_add_u32),
+// for cells shared between work-items.
+__device__ __forceinline__ void lds_add(float *p, float v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_add(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// End-of-block merge of work-item-private LDS tiles laid out [value][cols]:
+// one thread sums one value's `cols` lane entries in FP64 in a fixed order.
+// The start column is rotated by `rot` (the thread's value index) so the
+// threads of a wave read different LDS banks at each step.
+__device__ __forceinline__ double lds_row_sum_f32(const float *__restrict__ row,
+                                                  uint32_t cols, uint32_t rot)
+{
+    double s0 = 0.0, s1 = 0.0;
+    uint32_t j = rot % cols;
+    for (uint32_t i = 0; i < cols; ++i) {
+        double v = (double)row[j];
+        if (i & 1) s1 += v; else s0 += v;
+        j = (j + 1 == cols) ? 0 : j + 1;
+    }
+    return s0 + s1;
+}
+
+__device__ __forceinline__ uint32_t lds_row_sum_u32(const uint32_t *__restrict__ row,
+                                                    uint32_t cols, uint32_t rot)
+{
+    uint32_t s = 0;
+    uint32_t j = rot % cols;
+    for (uint32_t i = 0; i < cols; ++i) {
+        s += row[j];
+        j = (j + 1 == cols) ? 0 : j + 1;
+    }
+    return s;
+}
+
+}  // namespace clk

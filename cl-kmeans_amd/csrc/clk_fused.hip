@@ -1,0 +1,238 @@
+// Fused single pass (labeling + mass histogram + centroid feature sums in one
+// read of the points): the work of lloyd_fused_cluster_merge.cl and
+// lloyd_fused_feature_sum.cl, designed for gfx950.
+//
+// Layout per work-group (LDS):
+//   cen   [k][DMAX]            centroids, zero-padded to the register bucket
+//   tile  [k*(d+1)][COLS]      row c*(d+1)+0 = mass of c, rows c*(d+1)+1+f =
+//                              sum of feature f of c; COLS = work-group size
+//                              (cluster_merge: one column per work-item, so
+//                              ds_add never collides) or 1 (feature_sum: one
+//                              tile shared by the work-group).
+// Each work-item labels VEC consecutive points per step with 128-bit loads
+// (one per feature row), writes the labels, and adds the points into its
+// column (cluster_merge: plain read-add-write, the column is the work-item's
+// own; feature_sum: ds_add_f32 / ds_add_u32 atomics on the shared tile).  At the end the
+// tile columns are summed in FP64 in a fixed order and written as per-group
+// partials; a fixed-order column reduction adds the partials into the
+// caller's FP64 sums and uint32 masses.
+#include "clk_common.hpp"
+#include "clk_kernels.hpp"
+
+#include <algorithm>
+
+namespace clk {
+namespace {
+
+template <int DMAX, int VEC, int BLOCK, bool PRIVATE, bool EXACT>
+__global__ void __launch_bounds__(BLOCK > 0 ? BLOCK : 1024)
+fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
+                  const float *__restrict__ cen, uint32_t *__restrict__ labels,
+                  double *__restrict__ part_sums, uint32_t *__restrict__ part_masses)
+{
+    extern __shared__ __align__(16) float smem[];
+    const uint32_t B = BLOCK > 0 ? BLOCK : blockDim.x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t cols = PRIVATE ? B : 1;
+    const uint32_t rows = k * (d + 1);
+    float *cen_lds = smem;
+    float *tile = smem + ((k * DMAX + 3u) & ~3u);
+
+    stage_centroids<DMAX>(cen, d, k, cen_lds);
+    for (uint32_t i = tid; i < rows * cols; i += B)
+        tile[i] = 0.0f;
+    __syncthreads();
+
+    const uint64_t groups = n / VEC;
+    const uint64_t stride = (uint64_t)gridDim.x * B;
+    float *mine = tile + (PRIVATE ? tid : 0);
+    const uint32_t row_stride = (d + 1) * cols;  // < 2^24: the tile fits in LDS
+    // Software pipeline, two register buffers: one step's points are in
+    // flight while the other step's are labelled and accumulated.
+    auto step = [&](const float (&x)[DMAX][VEC], uint64_t g) {
+        const uint64_t p = g * VEC;
+        uint32_t lab[VEC];
+        argmin_lds<DMAX, VEC>(x, cen_lds, k, lab);
+        store_labels<VEC>(labels, p, lab);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            float *t = mine + __umul24(lab[v], row_stride);
+            if (PRIVATE) {
+                lds_acc(reinterpret_cast<uint32_t *>(t), 1u);
+#pragma unroll
+                for (int f = 0; f < DMAX; ++f)
+                    if (EXACT || f < (int)d)
+                        lds_acc(t + (f + 1) * cols, x[f][v]);
+            } else {
+                lds_add(reinterpret_cast<uint32_t *>(t), 1u);
+#pragma unroll
+                for (int f = 0; f < DMAX; ++f)
+                    if (EXACT || f < (int)d)
+                        lds_add(t + (f + 1) * cols, x[f][v]);
+            }
+        }
+    };
+    // Loads are unconditional (index clamped to the last group, always in
+    // bounds) so the loop body is straight-line and the wait counters can
+    // keep the prefetch in flight across the labeling of the other buffer.
+    const uint64_t last = groups ? groups - 1 : 0;
+    uint64_t g = (uint64_t)blockIdx.x * B + tid;
+    float xa[DMAX][VEC], xb[DMAX][VEC];
+    if (groups)
+        load_points<DMAX, VEC, EXACT>(pts, n, d, min(g, last) * VEC, xa);
+    while (g < groups) {
+        const uint64_t g1 = g + stride;
+        load_points<DMAX, VEC, EXACT>(pts, n, d, min(g1, last) * VEC, xb);
+        step(xa, g);
+        if (g1 >= groups)
+            break;
+        const uint64_t g2 = g1 + stride;
+        load_points<DMAX, VEC, EXACT>(pts, n, d, min(g2, last) * VEC, xa);
+        step(xb, g1);
+        g = g2;
+    }
+    __syncthreads();
+
+    const uint64_t dk = (uint64_t)d * k;
+    for (uint32_t r = tid; r < rows; r += B) {
+        const uint32_t c = r / (d + 1), j = r % (d + 1);
+        const float *row = tile + (uint64_t)r * cols;
+        if (j == 0) {
+            part_masses[(uint64_t)blockIdx.x * k + c] =
+                PRIVATE ? lds_row_sum_u32(reinterpret_cast<const uint32_t *>(row), cols, r)
+                        : reinterpret_cast<const uint32_t *>(row)[0];
+        } else {
+            part_sums[(uint64_t)blockIdx.x * dk + (uint64_t)(j - 1) * k + c] =
+                PRIVATE ? lds_row_sum_f32(row, cols, r) : (double)row[0];
+        }
+    }
+}
+
+using FusedKernel = void (*)(const float *, uint64_t, uint32_t, uint32_t, const float *,
+                             uint32_t *, double *, uint32_t *);
+
+template <int DMAX, bool PRIVATE, bool EXACT>
+FusedKernel pick_vec(int vec, bool block256)
+{
+    if (PRIVATE && block256) {
+        switch (vec) {
+        case 4: return fused_tile_kernel<DMAX, 4, 256, PRIVATE, EXACT>;
+        case 2: return fused_tile_kernel<DMAX, 2, 256, PRIVATE, EXACT>;
+        default: return fused_tile_kernel<DMAX, 1, 256, PRIVATE, EXACT>;
+        }
+    }
+    switch (vec) {
+    case 4: return fused_tile_kernel<DMAX, 4, 0, PRIVATE, EXACT>;
+    case 2: return fused_tile_kernel<DMAX, 2, 0, PRIVATE, EXACT>;
+    default: return fused_tile_kernel<DMAX, 1, 0, PRIVATE, EXACT>;
+    }
+}
+
+template <int DMAX, bool PRIVATE>
+FusedKernel pick_exact(bool exact, int vec, bool block256)
+{
+    return exact ? pick_vec<DMAX, PRIVATE, true>(vec, block256)
+                 : pick_vec<DMAX, PRIVATE, false>(vec, block256);
+}
+
+template <bool PRIVATE>
+FusedKernel pick_fused(int dmax, bool exact, int vec, bool block256)
+{
+    switch (dmax) {
+    case 1: return pick_vec<1, PRIVATE, true>(vec, block256);
+    case 2: return pick_exact<2, PRIVATE>(exact, vec, block256);
+    case 4: return pick_exact<4, PRIVATE>(exact, vec, block256);
+    case 8: return pick_exact<8, PRIVATE>(exact, vec, block256);
+    case 16: return pick_exact<16, PRIVATE>(exact, vec, block256);
+    case 32: return pick_exact<32, PRIVATE>(exact, vec, block256);
+    case 64: return pick_exact<64, PRIVATE>(exact, vec, block256);
+    default: return nullptr;
+    }
+}
+
+}  // namespace
+
+// Common argument checks of the point-streaming ops.
+static int check_sizes(uint64_t d, uint64_t n, uint64_t k)
+{
+    CLK_REQUIRE(d >= 1 && k >= 1, CLK_ERR_SIZE, "d and k must be >= 1");
+    CLK_REQUIRE(n < (1ull << 40), CLK_ERR_SIZE, "n too large");
+    CLK_REQUIRE(k <= 0xFFFFFFFFull && d <= 0xFFFFull, CLK_ERR_SIZE, "k or d too large");
+    return CLK_OK;
+}
+
+static int block_size(clk_context *ctx, const clk_kernel_config *cfg)
+{
+    uint64_t b = cfg && cfg->local_size[0] ? cfg->local_size[0] : 256;
+    if (b > (uint64_t)ctx->max_threads_per_block)
+        return -1;
+    return (int)b;
+}
+
+int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64_t n, uint64_t k,
+              const float *points, const float *old_centroids, double *sums, uint32_t *labels,
+              uint32_t *masses, hipStream_t stream)
+{
+    CLK_TRY(check_sizes(d, n, k));
+    CLK_REQUIRE(points && old_centroids && sums && labels && masses, CLK_ERR_INVALID_ARGUMENT,
+                "fused: null buffer");
+    if (n == 0)
+        return CLK_OK;
+    const int strategy = cfg ? cfg->strategy : CLK_FUSED_CLUSTER_MERGE;
+    CLK_REQUIRE(strategy == CLK_FUSED_CLUSTER_MERGE || strategy == CLK_FUSED_FEATURE_SUM,
+                CLK_ERR_INVALID_ARGUMENT, "fused: unknown strategy id");
+    const int B = block_size(ctx, cfg);
+    CLK_REQUIRE(B > 0, CLK_ERR_INVALID_ARGUMENT, "fused: local_size exceeds the device limit");
+    const int dmax = dmax_bucket(d);
+    const int vec = pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels);
+
+    // Variant by LDS fit (the reference picks its GLOBAL_MEM variant the same
+    // way, fused_feature_sum.hpp:315-337): private tile, shared tile, or the
+    // global-accumulation pass for k*d beyond LDS.
+    const uint64_t cen_bytes = dmax ? ((k * dmax + 3) & ~3ull) * 4 : 0;
+    const uint64_t rows = k * (d + 1);
+    const uint64_t priv_bytes = cen_bytes + rows * (uint64_t)B * 4;
+    const uint64_t shared_bytes = cen_bytes + rows * 4;
+    bool use_private = dmax && strategy == CLK_FUSED_CLUSTER_MERGE && priv_bytes <= ctx->lds_per_block;
+    bool use_shared = dmax && !use_private && shared_bytes <= ctx->lds_per_block;
+
+    if (!use_private && !use_shared) {
+        CLK_TRY(launch_labeling(ctx, cfg, d, n, k, points, old_centroids, labels, stream));
+        return launch_accumulate_global(ctx, d, n, k, points, labels, sums, masses, stream);
+    }
+
+    const bool b256 = (B == 256);
+    const bool exact = (uint64_t)dmax == d;
+    FusedKernel kern = use_private ? pick_fused<true>(dmax, exact, vec, b256)
+                                   : pick_fused<false>(dmax, exact, vec, b256);
+    const size_t lds = use_private ? priv_bytes : shared_bytes;
+    const uint64_t groups = n / vec;
+    uint64_t grid = (cfg && cfg->global_size[0]) ? div_up(cfg->global_size[0], (uint64_t)B)
+                                                 : auto_grid(ctx, (const void *)kern, B, lds, groups);
+    grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, 1u << 20));
+
+    CLK_TRY(ensure_scratch(ctx->part_sums, grid * d * k * sizeof(double)));
+    CLK_TRY(ensure_scratch(ctx->part_masses, grid * k * sizeof(uint32_t)));
+    double *ps = static_cast<double *>(ctx->part_sums.ptr);
+    uint32_t *pm = static_cast<uint32_t *>(ctx->part_masses.ptr);
+    {
+        KernelTimer t(ctx, use_private ? "FusedClusterMerge" : "FusedFeatureSum", stream);
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(B), lds, stream, points, n,
+                           (uint32_t)d, (uint32_t)k, old_centroids, labels, ps, pm);
+        CLK_HIP_CHECK(hipGetLastError());
+    }
+    return launch_reduce_partials(ctx, ps, d * k, sums, pm, k, masses, grid, stream);
+}
+
+}  // namespace clk
+
+extern "C" int clk_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64_t n,
+                         uint64_t k, const float *points, const float *old_centroids,
+                         double *sums, uint32_t *labels, uint32_t *masses, void *stream)
+{
+    CLK_TRY(clk::enter(ctx));
+    CLK_REQUIRE(!cfg || cfg->stage == CLK_STAGE_FUSED, CLK_ERR_INVALID_ARGUMENT,
+                "clk_fused: config is not a fused-stage config");
+    return clk::run_fused(ctx, cfg, d, n, k, points, old_centroids, sums, labels, masses,
+                          clk::as_stream(stream));
+}

@@ -1,0 +1,41 @@
+// Internal launchers shared between the op translation units.
+#pragma once
+
+#include "clk_common.hpp"
+
+namespace clk {
+
+// out[row] (+)= sum over col of in[col * rows + row], fixed order.
+int launch_reduce_f64(clk_context *ctx, const double *in, uint64_t cols, uint64_t rows,
+                      double *out, bool accumulate, hipStream_t stream);
+int launch_reduce_u32(clk_context *ctx, const uint32_t *in, uint64_t cols, uint64_t rows,
+                      uint32_t *out, bool accumulate, hipStream_t stream);
+
+// sums[r] += sum over col of ps[col*rows_s + r] (FP64) and
+// masses[r] += sum over col of pm[col*rows_m + r] (uint32), one launch.
+int launch_reduce_partials(clk_context *ctx, const double *ps, uint64_t rows_s, double *sums,
+                           const uint32_t *pm, uint64_t rows_m, uint32_t *masses, uint64_t cols,
+                           hipStream_t stream);
+
+// Smallest register bucket >= d among 1,2,4,8,16,32,64; 0 when d > 64.
+inline int dmax_bucket(uint64_t d)
+{
+    int b = 1;
+    while (b < 64 && (uint64_t)b < d)
+        b *= 2;
+    return ((uint64_t)b >= d) ? b : 0;
+}
+
+// Labels every point with the reference arithmetic; used by the fused ops
+// for the global-accumulation variant and by clk_labeling.
+int launch_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64_t n,
+                    uint64_t k, const float *points, const float *centroids, uint32_t *labels,
+                    hipStream_t stream);
+
+// sums[f,c] += x_f and masses[c] += 1 for every point, with FP64 / u32
+// global atomics (any k*d; summation order not fixed).  masses may be null.
+int launch_accumulate_global(clk_context *ctx, uint64_t d, uint64_t n, uint64_t k,
+                             const float *points, const uint32_t *labels, double *sums,
+                             uint32_t *masses, hipStream_t stream);
+
+}  // namespace clk

@@ -1,0 +1,203 @@
+// Labeling pass (LabelingUnrollVector / lloyd_labeling_vp_clcp.cl) and the
+// global-accumulation pass used when k*d exceeds the LDS tiles.
+#include "clk_common.hpp"
+#include "clk_kernels.hpp"
+
+#include <algorithm>
+
+namespace clk {
+namespace {
+
+// Centroids staged in LDS ([k][DMAX]); each work-item labels VEC
+// consecutive points per step with 128-bit row loads.
+template <int DMAX, int VEC>
+__global__ void __launch_bounds__(1024)
+label_lds_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
+                 const float *__restrict__ cen, uint32_t *__restrict__ labels)
+{
+    extern __shared__ __align__(16) float cen_lds[];
+    stage_centroids<DMAX>(cen, d, k, cen_lds);
+    __syncthreads();
+    const uint64_t groups = n / VEC;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += stride) {
+        const uint64_t p = g * VEC;
+        float x[DMAX][VEC];
+        load_points<DMAX, VEC>(pts, n, d, p, x);
+        uint32_t lab[VEC];
+        argmin_lds<DMAX, VEC>(x, cen_lds, k, lab);
+        store_labels<VEC>(labels, p, lab);
+    }
+}
+
+// Centroids read from global memory with work-group-uniform addresses (any
+// k).  DMAX > 0 keeps the point in registers; DMAX == 0 re-reads it (d > 64).
+template <int DMAX>
+__global__ void __launch_bounds__(256)
+label_global_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
+                    const float *__restrict__ cen, uint32_t *__restrict__ labels)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        float best = FLT_MAX;
+        uint32_t lab = 0;
+        if constexpr (DMAX > 0) {
+            float x[DMAX];
+#pragma unroll
+            for (int f = 0; f < DMAX; ++f)
+                x[f] = (f < (int)d) ? pts[(uint64_t)f * n + p] : 0.0f;
+            for (uint32_t c = 0; c < k; ++c) {
+                float t = x[0] - cen[c];
+                float dist = t * t;
+#pragma unroll
+                for (int f = 1; f < DMAX; ++f) {
+                    if (f < (int)d) {
+                        t = x[f] - cen[(uint64_t)f * k + c];
+                        dist = __builtin_fmaf(t, t, dist);
+                    }
+                }
+                bool lt = dist < best;
+                best = lt ? dist : best;
+                lab = lt ? c : lab;
+            }
+        } else {
+            for (uint32_t c = 0; c < k; ++c) {
+                float t = pts[p] - cen[c];
+                float dist = t * t;
+                for (uint32_t f = 1; f < d; ++f) {
+                    t = pts[(uint64_t)f * n + p] - cen[(uint64_t)f * k + c];
+                    dist = __builtin_fmaf(t, t, dist);
+                }
+                bool lt = dist < best;
+                best = lt ? dist : best;
+                lab = lt ? c : lab;
+            }
+        }
+        labels[p] = lab;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+accumulate_global_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
+                         const uint32_t *__restrict__ labels, double *__restrict__ sums,
+                         uint32_t *__restrict__ masses)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        const uint32_t c = labels[p];
+        if (c >= k)
+            continue;
+        if (masses)
+            atomicAdd(&masses[c], 1u);
+        for (uint32_t f = 0; f < d; ++f)
+            unsafeAtomicAdd(&sums[(uint64_t)f * k + c], (double)pts[(uint64_t)f * n + p]);
+    }
+}
+
+using LabelKernel = void (*)(const float *, uint64_t, uint32_t, uint32_t, const float *,
+                             uint32_t *);
+
+template <int DMAX>
+LabelKernel pick_label_vec(int vec)
+{
+    switch (vec) {
+    case 4: return label_lds_kernel<DMAX, 4>;
+    case 2: return label_lds_kernel<DMAX, 2>;
+    default: return label_lds_kernel<DMAX, 1>;
+    }
+}
+
+LabelKernel pick_label_lds(int dmax, int vec)
+{
+    switch (dmax) {
+    case 1: return pick_label_vec<1>(vec);
+    case 2: return pick_label_vec<2>(vec);
+    case 4: return pick_label_vec<4>(vec);
+    case 8: return pick_label_vec<8>(vec);
+    case 16: return pick_label_vec<16>(vec);
+    case 32: return pick_label_vec<32>(vec);
+    case 64: return pick_label_vec<64>(vec);
+    default: return nullptr;
+    }
+}
+
+LabelKernel pick_label_global(int dmax)
+{
+    switch (dmax) {
+    case 1: return label_global_kernel<1>;
+    case 2: return label_global_kernel<2>;
+    case 4: return label_global_kernel<4>;
+    case 8: return label_global_kernel<8>;
+    case 16: return label_global_kernel<16>;
+    case 32: return label_global_kernel<32>;
+    case 64: return label_global_kernel<64>;
+    default: return label_global_kernel<0>;
+    }
+}
+
+}  // namespace
+
+int launch_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64_t n,
+                    uint64_t k, const float *points, const float *centroids, uint32_t *labels,
+                    hipStream_t stream)
+{
+    if (n == 0)
+        return CLK_OK;
+    const uint64_t B = (cfg && cfg->local_size[0]) ? cfg->local_size[0] : 256;
+    CLK_REQUIRE(B <= (uint64_t)ctx->max_threads_per_block, CLK_ERR_INVALID_ARGUMENT,
+                "labeling: local_size exceeds the device limit");
+    const int dmax = dmax_bucket(d);
+    const uint64_t cen_bytes = dmax ? k * dmax * 4 : 0;
+    if (dmax && cen_bytes <= ctx->lds_per_block) {
+        const int vec = pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels);
+        LabelKernel kern = pick_label_lds(dmax, vec);
+        uint64_t grid = (cfg && cfg->global_size[0])
+                            ? div_up(cfg->global_size[0], B)
+                            : auto_grid(ctx, (const void *)kern, (int)B, cen_bytes, n / vec);
+        grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, 1u << 20));
+        KernelTimer t(ctx, "LabelingUnrollVector", stream);
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3((unsigned)B), cen_bytes, stream,
+                           points, n, (uint32_t)d, (uint32_t)k, centroids, labels);
+        CLK_HIP_CHECK(hipGetLastError());
+        return CLK_OK;
+    }
+    LabelKernel kern = pick_label_global(dmax);
+    const uint64_t Bg = 256;
+    uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(div_up(n, Bg), (uint64_t)ctx->num_cus * 8));
+    KernelTimer t(ctx, "LabelingGlobal", stream);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3((unsigned)Bg), 0, stream, points, n,
+                       (uint32_t)d, (uint32_t)k, centroids, labels);
+    CLK_HIP_CHECK(hipGetLastError());
+    return CLK_OK;
+}
+
+int launch_accumulate_global(clk_context *ctx, uint64_t d, uint64_t n, uint64_t k,
+                             const float *points, const uint32_t *labels, double *sums,
+                             uint32_t *masses, hipStream_t stream)
+{
+    if (n == 0)
+        return CLK_OK;
+    uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(div_up(n, 256), (uint64_t)ctx->num_cus * 8));
+    KernelTimer t(ctx, "AccumulateGlobal", stream);
+    hipLaunchKernelGGL(accumulate_global_kernel, dim3((unsigned)grid), dim3(256), 0, stream,
+                       points, n, (uint32_t)d, (uint32_t)k, labels, sums, masses);
+    CLK_HIP_CHECK(hipGetLastError());
+    return CLK_OK;
+}
+
+}  // namespace clk
+
+extern "C" int clk_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
+                            uint64_t n, uint64_t k, const float *points,
+                            const float *centroids, uint32_t *labels, void *stream)
+{
+    CLK_TRY(clk::enter(ctx));
+    CLK_REQUIRE(!cfg || cfg->stage == CLK_STAGE_LABELING, CLK_ERR_INVALID_ARGUMENT,
+                "clk_labeling: config is not a labeling-stage config");
+    CLK_REQUIRE(!cfg || cfg->strategy == CLK_LABELING_UNROLL_VECTOR, CLK_ERR_INVALID_ARGUMENT,
+                "clk_labeling: unknown strategy id");
+    CLK_REQUIRE(d >= 1 && k >= 1 && k <= 0xFFFFFFFFull, CLK_ERR_SIZE, "labeling: bad d or k");
+    CLK_REQUIRE(points && centroids && labels, CLK_ERR_INVALID_ARGUMENT, "labeling: null buffer");
+    return clk::launch_labeling(ctx, cfg, d, n, k, points, centroids, labels,
+                                clk::as_stream(stream));
+}

@@ -1,0 +1,233 @@
+/*
+ * clkmeans.h -- C ABI of the MI355X (gfx950) Lloyd-iteration library.
+ *
+ * Every compute entry point replaces one operator of CL-kmeans (the
+ * std::function objects its factories return) and keeps that operator's
+ * argument meaning: sizes first, then device buffers, accumulate-vs-overwrite
+ * semantics unchanged.  Reference citations are file:line in the CL-kmeans
+ * source tree.
+ *
+ * Conventions
+ *  - All pointers named points/centroids/labels/masses/sums/data are DEVICE
+ *    pointers (hipMalloc memory) unless the name starts with host_.
+ *  - Layouts are the reference's column-major ones:
+ *      points     float[d][n]   points[f * n + p]    (binary_format.cpp:37-43)
+ *      centroids  float[d][k]   centroids[f * k + c] (lloyd_fused_feature_sum.cl:230-232)
+ *      sums       double[d][k]  same layout as centroids (FP64 accumulator)
+ *      labels     uint32[n], masses uint32[k]
+ *  - stream is a hipStream_t passed as void*; NULL is the default stream.
+ *    Every compute call only enqueues work on that stream.
+ *  - Return value: CLK_OK (0) or a negative CLK_ERR_*; clk_last_error()
+ *    returns the calling thread's last message.  Unknown strategy names give
+ *    CLK_ERR_INVALID_ARGUMENT (the reference throws std::invalid_argument,
+ *    fused_factory.hpp:84); size mismatches give CLK_ERR_SIZE (the reference
+ *    asserts, fused_feature_sum.hpp:211-220).
+ *  - A context owns scratch (per-block partials) and is used by one host
+ *    thread at a time, like one reference operator object.
+ *
+ * Accumulator type.  The reference accumulates centroid sums in PointT
+ * (float).  This library accumulates them in FP64 (double[d][k]) and divides
+ * in FP64, rounding once to float; masses stay uint32.
+ */
+#ifndef CLKMEANS_H
+#define CLKMEANS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLK_OK 0
+#define CLK_ERR_INVALID_ARGUMENT (-1)
+#define CLK_ERR_SIZE (-2)
+#define CLK_ERR_HIP (-3)
+#define CLK_ERR_UNSUPPORTED (-4)
+#define CLK_ERR_NOMEM (-5)
+
+typedef struct clk_context clk_context;
+
+/* Stage ids for clk_strategy_id / clk_kernel_config.stage. */
+#define CLK_STAGE_FUSED 0
+#define CLK_STAGE_LABELING 1
+#define CLK_STAGE_MASS_UPDATE 2
+#define CLK_STAGE_CENTROID_UPDATE 3
+
+/* Strategy ids (the factories' strategy strings). */
+#define CLK_FUSED_CLUSTER_MERGE 0   /* "cluster_merge"  fused_factory.hpp:73-77 */
+#define CLK_FUSED_FEATURE_SUM 1     /* "feature_sum"    fused_factory.hpp:78-82 */
+#define CLK_LABELING_UNROLL_VECTOR 0 /* "unroll_vector" labeling_factory.hpp:63-80 */
+#define CLK_MASS_GLOBAL_ATOMIC 0    /* "global_atomic"  mass_update_factory.hpp:71-75 */
+#define CLK_MASS_PART_GLOBAL 1      /* "part_global"    mass_update_factory.hpp:76-80 */
+#define CLK_MASS_PART_LOCAL 2       /* "part_local"     mass_update_factory.hpp:81-85 */
+#define CLK_MASS_PART_PRIVATE 3     /* "part_private"   mass_update_factory.hpp:86-90 */
+#define CLK_CENTROID_FEATURE_SUM 0        /* "feature_sum"        centroid_update_factory.hpp:83-92 */
+#define CLK_CENTROID_FEATURE_SUM_PARDIM 1 /* "feature_sum_pardim" centroid_update_factory.hpp:93-111 */
+#define CLK_CENTROID_CLUSTER_MERGE 2      /* "cluster_merge"      centroid_update_factory.hpp:112-120 */
+
+/*
+ * Launch geometry and strategy of one stage: the reference's
+ * {Fused,Labeling,MassUpdate,CentroidUpdate}Configuration structs
+ * (fused_configuration.hpp:18-25, labeling_configuration.hpp:18-27,
+ * mass_update_configuration.hpp:18-25, centroid_update_configuration.hpp:18-27)
+ * minus platform/device.  A size of 0 means "choose for this GPU".
+ * global_size counts work-items (threads), as in the reference.
+ */
+typedef struct clk_kernel_config {
+    int32_t stage;
+    int32_t strategy;
+    uint64_t global_size[3];
+    uint64_t local_size[3];
+    uint64_t vector_length;   /* points per work-item per load: 1, 2, 4 or 8 */
+    uint64_t local_features;  /* feature_sum_pardim: work-items per block along features */
+    uint64_t thread_features; /* feature_sum_pardim: features per work-item */
+} clk_kernel_config;
+
+/* ---- errors, context ------------------------------------------------- */
+
+const char *clk_last_error(void);
+
+/* Create a context on HIP device `device` (scratch, device properties). */
+int clk_context_create(int device, clk_context **out);
+int clk_context_destroy(clk_context *ctx);
+
+/* Map a factory strategy string to its id; CLK_ERR_INVALID_ARGUMENT when the
+ * reference factory would throw std::invalid_argument (fused_factory.hpp:84,
+ * labeling_factory.hpp:83, mass_update_factory.hpp:92,
+ * centroid_update_factory.hpp:122). */
+int clk_strategy_id(int stage, const char *name);
+
+/* Fill cfg with this library's defaults for the stage/strategy. */
+int clk_default_config(clk_context *ctx, int stage, int strategy, clk_kernel_config *cfg);
+
+/* ---- per-op timing (the reference's Measurement DataPoints) ------------ */
+
+/* When enabled, every kernel an op enqueues is bracketed by HIP events
+ * (measurement.cpp:37-60 sums OpenCL event durations the same way). */
+int clk_profiling_enable(clk_context *ctx, int enable);
+/* Tag subsequent records with an iteration number (-1 = none). */
+int clk_profiling_set_iteration(clk_context *ctx, int iteration);
+int clk_profiling_clear(clk_context *ctx);
+/* Number of kernel records so far. */
+int64_t clk_profiling_count(clk_context *ctx);
+/* Record i: kernel name, iteration, elapsed milliseconds (waits for it). */
+int clk_profiling_get(clk_context *ctx, int64_t i, char *name, size_t name_len,
+                      int32_t *iteration, float *ms);
+
+/* ---- the Lloyd operators ------------------------------------------------ */
+
+/*
+ * Fused single pass.  Replaces FusedFactory::FusedFunction
+ * (fused_factory.hpp:34-53) as implemented by FusedClusterMerge
+ * (fused_cluster_merge.hpp:193-414) and FusedFeatureSum
+ * (fused_feature_sum.hpp:191-444):
+ *   labels[p]  = argmin_c sum_f fma-sequence (x_f - c_f)^2   (overwrite)
+ *   masses[c] += #{p : labels[p] == c}                       (accumulate)
+ *   sums[f,c] += sum_{p : labels[p] == c} x_f                (accumulate)
+ * old_centroids is read only; sums / masses must not alias it.
+ */
+int clk_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
+              uint64_t n, uint64_t k, const float *points,
+              const float *old_centroids, double *sums, uint32_t *labels,
+              uint32_t *masses, void *stream);
+
+/*
+ * Labeling only.  Replaces LabelingFactory::LabelingFunction
+ * (labeling_factory.hpp:33-48) / LabelingUnrollVector
+ * (labeling_unroll_vector.hpp:168-272): same arithmetic as clk_fused.
+ */
+int clk_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
+                 uint64_t n, uint64_t k, const float *points,
+                 const float *centroids, uint32_t *labels, void *stream);
+
+/*
+ * masses[c] += #{p : labels[p] == c}.  Replaces
+ * MassUpdateFactory::MassUpdateFunction (mass_update_factory.hpp:36-48) and
+ * MassUpdate{GlobalAtomic,PartGlobal,PartLocal,PartPrivate}
+ * (mass_update_global_atomic.hpp:110-156, mass_update_part_global.hpp:104-189,
+ * mass_update_part_local.hpp:125-217, mass_update_part_private.hpp:138-249).
+ * Labels >= k are rejected by no kernel; callers pass labels from clk_labeling.
+ */
+int clk_mass_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t n,
+                    uint64_t k, const uint32_t *labels, uint32_t *masses,
+                    void *stream);
+
+/*
+ * sums[f,c] += sum_{p : labels[p] == c} x_f.  Replaces
+ * CentroidUpdateFactory::CentroidUpdateFunction
+ * (centroid_update_factory.hpp:37-54) and CentroidUpdate{FeatureSum,
+ * FeatureSumPardim,ClusterMerge} (centroid_update_feature_sum.hpp:116-206,
+ * centroid_update_feature_sum_pardim.hpp:149-306,
+ * centroid_update_cluster_merge.hpp:145-272).  masses is accepted and unused,
+ * as in the reference kernels.
+ */
+int clk_centroid_update(clk_context *ctx, const clk_kernel_config *cfg,
+                        uint64_t d, uint64_t n, uint64_t k, const float *points,
+                        double *sums, const uint32_t *labels,
+                        const uint32_t *masses, void *stream);
+
+/*
+ * centroids[f,c] = (float)(sums[f,c] / (double)masses[c]); an empty cluster
+ * gives NaN (0/0).  Replaces MatrixBinaryOp::row with Divide
+ * (matrix_binary_op.hpp:87-127, matrix_binary_op.cl:23-39), computed in FP64.
+ * When old_centroids and max_shift are both non-NULL, *max_shift (device
+ * double) receives max_c ||new_c - old_c||_2 over non-empty clusters.
+ */
+int clk_matrix_divide_row(clk_context *ctx, uint64_t d, uint64_t k,
+                          const double *sums, const uint32_t *masses,
+                          float *centroids, const float *old_centroids,
+                          double *max_shift, void *stream);
+
+/* fst[i] += snd[i], i < count.  MatrixBinaryOp::matrix with Add
+ * (matrix_binary_op.hpp:149-188, matrix_binary_op.cl:51-65). */
+int clk_matrix_add_u32(clk_context *ctx, uint64_t count, uint32_t *fst,
+                       const uint32_t *snd, void *stream);
+int clk_matrix_add_f64(clk_context *ctx, uint64_t count, double *fst,
+                       const double *snd, void *stream);
+
+/*
+ * In-place column-vector sum: data holds num_cols vectors of result_rows
+ * elements (data[col * result_rows + row]); afterwards
+ * data[row] = sum over col, for row < result_rows, in a fixed order
+ * (deterministic).  Replaces ReduceVectorParcol<T>
+ * (reduce_vector_parcol.hpp:30-167); any num_cols >= 1 is accepted.
+ */
+int clk_reduce_vector_parcol_u32(clk_context *ctx, uint64_t num_cols,
+                                 uint64_t result_rows, uint32_t *data,
+                                 void *stream);
+int clk_reduce_vector_parcol_f64(clk_context *ctx, uint64_t num_cols,
+                                 uint64_t result_rows, double *data,
+                                 void *stream);
+
+/* Zero `bytes` bytes at a device pointer with a kernel (usable under HIP graph
+ * capture).  Replaces boost::compute::fill_async(..., 0, queue) used at
+ * kmeans_single_stage.hpp:97-112. */
+int clk_fill_zero(clk_context *ctx, void *ptr, uint64_t bytes, void *stream);
+
+/* centroids[f,c] = points[f, c mod n] on the device: KmeansInitializer::first_x
+ * (kmeans_initializer.cpp:29-39). */
+int clk_init_first_x(clk_context *ctx, uint64_t d, uint64_t n, uint64_t k,
+                     const float *points, float *centroids, void *stream);
+
+/* ---- host-side synthetic data ----------------------------------------- */
+
+/*
+ * Gaussian blobs, feature-major float32 host_points[d][n]: point p belongs to
+ * blob p mod k (equal sizes, interleaved); blob centres uniform in
+ * [-box, box]^d; isotropic noise sigma.  Deterministic in (seed, n, d, k):
+ * a counter-based generator, independent of thread count.  Optional
+ * host_centres[d][k] receives the blob centres.  Takes the role of the
+ * reference's cluster_generator.cpp, which instead writes points grouped by
+ * blob (cluster_generator.cpp:178-195); interleaving makes first_x pick one
+ * seed from each blob.
+ */
+int clk_generate_blobs(uint64_t n, uint64_t d, uint64_t k, double sigma,
+                       double box, uint64_t seed, float *host_points,
+                       float *host_centres, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLKMEANS_H */

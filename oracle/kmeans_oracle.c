@@ -1,0 +1,385 @@
+/*
+ * kmeans_oracle.c -- CPU restatement of CL-kmeans' Lloyd iteration.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or as the timed CPU baseline).  The product (cl-kmeans_amd/) never calls it.
+ *
+ * Every function cites the reference file:line whose behaviour it restates
+ * (paths relative to the CL-kmeans source tree).  Layouts follow the
+ * reference: points are feature-major  pts[f * n + p]  (binary_format.cpp:37-43,
+ * matrix.hpp:117, ccoord2ind in the cl_kernels sources), centroids are feature-major
+ * cen[f * k + c]  (lloyd_fused_feature_sum.cl:230-232).
+ *
+ * Pinning status of each function (what checks it besides itself):
+ *   ora_label_*          exact-rational brute force on tiny inputs with
+ *                        adversarial near-ties (tests/test_oracle.py), hand
+ *                        ties / NaN / inf cases, sklearn on separated blobs.
+ *   ora_mass             integer invariants (sum == n, bincount).
+ *   ora_sum_f64          math.fsum per cell, k == 1 closed form.
+ *   ora_sum_kahan_f32    math.fsum within the Kahan bound.
+ *   ora_divide_*         IEEE division, 0/0 -> NaN case.
+ *   ora_lloyd_*          sklearn KMeans (lloyd, given init), the R stats
+ *                        kmeans_Lloyd built from the reference's
+ *                        baselines/r_stats_kmeans.c (oracle/_ref), k == 1 mean.
+ *   ora_reduce_u32       the reference's ReduceVectorParcol test cases
+ *                        (test/reduce_vector.cpp:25-38 verifier).
+ *   ora_first_x          closed form of kmeans_initializer.cpp:34-38.
+ *   ora_partition_matrix byte-exact chunk layout of buffer_helper.cpp:58-84,
+ *                        checked against a slicing restatement in numpy.
+ *   ora_lloyd_mt_f32     equality with ora_lloyd_f32 (mode 0).
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off): fmaf() is called
+ * explicitly where the reference's kernels call fma(); nothing else may be
+ * contracted.
+ */
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* Labeling                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Per point: dist_c = fma-sequence over features f = 0..d-1 of
+ * (x_f - c_f)^2 starting from 0; argmin with strict '<' against a running
+ * minimum initialised to FLT_MAX; label 0 when no distance is below FLT_MAX.
+ *   kmeans_naive.cpp:47-63 (with -ffp-contract=fast, CMakeLists.txt:31,
+ *   'distance += t*t' is an fma), lloyd_fused_feature_sum.cl:209-239,
+ *   lloyd_labeling_vp_clcp.cl:123-151.
+ */
+static inline uint32_t label_point_f32(const float *pts, uint64_t n, uint64_t d,
+                                       const float *cen, uint64_t k, uint64_t p)
+{
+    float best = FLT_MAX;
+    uint32_t lab = 0;
+    for (uint64_t c = 0; c < k; ++c) {
+        float dist = 0.0f;
+        for (uint64_t f = 0; f < d; ++f) {
+            float t = pts[f * n + p] - cen[f * k + c];
+            dist = fmaf(t, t, dist);
+        }
+        if (dist < best) {
+            best = dist;
+            lab = (uint32_t)c;
+        }
+    }
+    return lab;
+}
+
+void ora_label_f32(const float *pts, uint64_t n, uint64_t d, const float *cen,
+                   uint64_t k, uint32_t *labels)
+{
+    for (uint64_t p = 0; p < n; ++p)
+        labels[p] = label_point_f32(pts, n, d, cen, k, p);
+}
+
+/* Same as ora_label_f32 for the double / uint64 type triple
+ * (bench.cpp:469-481, CL_POINT_MAX = DBL_MAX in fused_feature_sum.hpp:90). */
+void ora_label_f64(const double *pts, uint64_t n, uint64_t d, const double *cen,
+                   uint64_t k, uint64_t *labels)
+{
+    for (uint64_t p = 0; p < n; ++p) {
+        double best = DBL_MAX;
+        uint64_t lab = 0;
+        for (uint64_t c = 0; c < k; ++c) {
+            double dist = 0.0;
+            for (uint64_t f = 0; f < d; ++f) {
+                double t = pts[f * n + p] - cen[f * k + c];
+                dist = fma(t, t, dist);
+            }
+            if (dist < best) {
+                best = dist;
+                lab = c;
+            }
+        }
+        labels[p] = lab;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mass update and centroid sums                                             */
+/* ------------------------------------------------------------------------ */
+
+/* masses[c] = #{p : labels[p] == c}, uint32 wrap-around arithmetic.
+ * kmeans_naive.cpp:76-78, histogram_*.cl.  Out-of-range labels are ignored
+ * (the reference would write out of bounds). */
+void ora_mass(const uint32_t *labels, uint64_t n, uint64_t k, uint32_t *masses)
+{
+    memset(masses, 0, k * sizeof(uint32_t));
+    for (uint64_t p = 0; p < n; ++p)
+        if (labels[p] < k)
+            masses[labels[p]] += 1u;
+}
+
+void ora_mass_u64(const uint64_t *labels, uint64_t n, uint64_t k, uint64_t *masses)
+{
+    memset(masses, 0, k * sizeof(uint64_t));
+    for (uint64_t p = 0; p < n; ++p)
+        if (labels[p] < k)
+            masses[labels[p]] += 1u;
+}
+
+/* sums[f*k + c] = sum over points with label c of x_f, in FP64, point order.
+ * The FP64 reference the GPU centroids are checked against. */
+void ora_sum_f64(const float *pts, uint64_t n, uint64_t d, const uint32_t *labels,
+                 uint64_t k, double *sums)
+{
+    memset(sums, 0, k * d * sizeof(double));
+    for (uint64_t p = 0; p < n; ++p) {
+        uint32_t c = labels[p];
+        if (c >= k)
+            continue;
+        for (uint64_t f = 0; f < d; ++f)
+            sums[f * k + c] += (double)pts[f * n + p];
+    }
+}
+
+void ora_sum_f64_dp(const double *pts, uint64_t n, uint64_t d, const uint64_t *labels,
+                    uint64_t k, double *sums)
+{
+    memset(sums, 0, k * d * sizeof(double));
+    for (uint64_t p = 0; p < n; ++p) {
+        uint64_t c = labels[p];
+        if (c >= k)
+            continue;
+        for (uint64_t f = 0; f < d; ++f)
+            sums[f * k + c] += pts[f * n + p];
+    }
+}
+
+/* Kahan-compensated FP32 sums in point order: kmeans_naive.cpp:73-94. */
+void ora_sum_kahan_f32(const float *pts, uint64_t n, uint64_t d,
+                       const uint32_t *labels, uint64_t k, float *sums)
+{
+    float *comp = (float *)calloc(k * d, sizeof(float));
+    memset(sums, 0, k * d * sizeof(float));
+    for (uint64_t p = 0; p < n; ++p) {
+        uint32_t c = labels[p];
+        if (c >= k)
+            continue;
+        for (uint64_t f = 0; f < d; ++f) {
+            float x = pts[f * n + p];
+            float *cm = &comp[f * k + c];
+            float *s = &sums[f * k + c];
+            float y = x - *cm;
+            float t = *s + y;
+            *cm = (t - *s) - y;
+            *s = t;
+        }
+    }
+    free(comp);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Division                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* cen = (float)(sum64 / (double)mass): FP64 divide, one final rounding.
+ * Empty cluster: 0/0 = NaN, as matrix_binary_op.cl:23-39 and
+ * kmeans_naive.cpp:96-100 produce. */
+void ora_divide_from_f64(const double *sums, const uint32_t *masses, uint64_t d,
+                         uint64_t k, float *cen)
+{
+    for (uint64_t f = 0; f < d; ++f)
+        for (uint64_t c = 0; c < k; ++c)
+            cen[f * k + c] = (float)(sums[f * k + c] / (double)masses[c]);
+}
+
+/* cen = sum32 / (float)mass: matrix_binary_op.cl:36-38 (BINARY_OP '/',
+ * CL_TYPE_2 cast to CL_TYPE_1), kmeans_naive.cpp:96-100. */
+void ora_divide_f32(const float *sums, const uint32_t *masses, uint64_t d,
+                    uint64_t k, float *cen)
+{
+    for (uint64_t f = 0; f < d; ++f)
+        for (uint64_t c = 0; c < k; ++c)
+            cen[f * k + c] = sums[f * k + c] / (float)masses[c];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Initialisation                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* centroid c = point (c mod n): kmeans_initializer.cpp:29-39. */
+void ora_first_x(const float *pts, uint64_t n, uint64_t d, uint64_t k, float *cen)
+{
+    for (uint64_t f = 0; f < d; ++f)
+        for (uint64_t c = 0; c < k; ++c)
+            cen[f * k + c] = pts[f * n + (c % n)];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Full Lloyd loop                                                           */
+/* ------------------------------------------------------------------------ */
+
+enum { ORA_MODE_FP64_MERGE = 0, ORA_MODE_NAIVE_KAHAN = 1 };
+
+/*
+ * Fixed-iteration Lloyd loop: kmeans_single_stage.hpp:92-151 /
+ * kmeans_three_stage.hpp:116-222 protocol (label, zero, mass, sum, divide,
+ * swap), no convergence test (kmeans.converge is never read,
+ * kmeans_three_stage.hpp:140).
+ *   mode 0: centroid = (float)(FP64 sum / mass)  -- the GPU product's form.
+ *   mode 1: Kahan FP32 sums / (float)mass        -- kmeans_naive.cpp:31-106.
+ * cen is in/out (k*d), labels (n) and masses (k) are out.
+ */
+void ora_lloyd_f32(const float *pts, uint64_t n, uint64_t d, uint64_t k,
+                   float *cen, uint32_t iterations, int mode,
+                   uint32_t *labels, uint32_t *masses)
+{
+    double *s64 = (double *)malloc(k * d * sizeof(double));
+    float *s32 = (float *)malloc(k * d * sizeof(float));
+    for (uint32_t it = 0; it < iterations; ++it) {
+        ora_label_f32(pts, n, d, cen, k, labels);
+        ora_mass(labels, n, k, masses);
+        if (mode == ORA_MODE_NAIVE_KAHAN) {
+            ora_sum_kahan_f32(pts, n, d, labels, k, s32);
+            ora_divide_f32(s32, masses, d, k, cen);
+        } else {
+            ora_sum_f64(pts, n, d, labels, k, s64);
+            ora_divide_from_f64(s64, masses, d, k, cen);
+        }
+    }
+    free(s64);
+    free(s32);
+}
+
+/* Double / uint64 type triple, FP64 throughout. */
+void ora_lloyd_f64(const double *pts, uint64_t n, uint64_t d, uint64_t k,
+                   double *cen, uint32_t iterations, uint64_t *labels,
+                   uint64_t *masses)
+{
+    double *s = (double *)malloc(k * d * sizeof(double));
+    for (uint32_t it = 0; it < iterations; ++it) {
+        ora_label_f64(pts, n, d, cen, k, labels);
+        ora_mass_u64(labels, n, k, masses);
+        ora_sum_f64_dp(pts, n, d, labels, k, s);
+        for (uint64_t f = 0; f < d; ++f)
+            for (uint64_t c = 0; c < k; ++c)
+                cen[f * k + c] = s[f * k + c] / (double)masses[c];
+    }
+    free(s);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multi-threaded Lloyd: the timed CPU baseline                              */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Restates the reference's CPU strategy geometry: each thread owns a
+ * contiguous block of points (LOCAL_STRIDE, lloyd_fused_cluster_merge.cl),
+ * labels it, and keeps a private k*d sum tile and k masses (the
+ * cluster_merge private tile, lloyd_fused_cluster_merge.cl:188-241); tiles
+ * are merged in thread order.  Sums are FP64, so results equal
+ * ora_lloyd_f32(mode 0) except for FP64 summation order.
+ */
+typedef struct {
+    const float *pts;
+    const float *cen;
+    uint64_t n, d, k, p0, p1;
+    uint32_t *labels;
+    double *sums;     /* k*d */
+    uint64_t *masses; /* k */
+} ora_mt_job;
+
+static void *ora_mt_worker(void *arg)
+{
+    ora_mt_job *j = (ora_mt_job *)arg;
+    memset(j->sums, 0, j->k * j->d * sizeof(double));
+    memset(j->masses, 0, j->k * sizeof(uint64_t));
+    for (uint64_t p = j->p0; p < j->p1; ++p) {
+        uint32_t c = label_point_f32(j->pts, j->n, j->d, j->cen, j->k, p);
+        j->labels[p] = c;
+        j->masses[c] += 1;
+        for (uint64_t f = 0; f < j->d; ++f)
+            j->sums[f * j->k + c] += (double)j->pts[f * j->n + p];
+    }
+    return NULL;
+}
+
+int ora_lloyd_mt_f32(const float *pts, uint64_t n, uint64_t d, uint64_t k,
+                     float *cen, uint32_t iterations, uint32_t *labels,
+                     uint32_t *masses, int threads)
+{
+    if (threads < 1)
+        threads = 1;
+    pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+    ora_mt_job *jobs = (ora_mt_job *)malloc(sizeof(ora_mt_job) * threads);
+    double *sums = (double *)malloc(sizeof(double) * k * d * threads);
+    uint64_t *ms = (uint64_t *)malloc(sizeof(uint64_t) * k * threads);
+    double *tot = (double *)malloc(sizeof(double) * k * d);
+    uint64_t per = (n + threads - 1) / threads;
+    for (uint32_t it = 0; it < iterations; ++it) {
+        for (int t = 0; t < threads; ++t) {
+            uint64_t p0 = per * t, p1 = p0 + per;
+            if (p0 > n) p0 = n;
+            if (p1 > n) p1 = n;
+            ora_mt_job jb = {pts, cen, n, d, k, p0, p1, labels,
+                             sums + (size_t)t * k * d, ms + (size_t)t * k};
+            jobs[t] = jb;
+            if (pthread_create(&tid[t], NULL, ora_mt_worker, &jobs[t]) != 0)
+                return -1;
+        }
+        for (int t = 0; t < threads; ++t)
+            pthread_join(tid[t], NULL);
+        memset(tot, 0, sizeof(double) * k * d);
+        for (uint64_t c = 0; c < k; ++c)
+            masses[c] = 0;
+        for (int t = 0; t < threads; ++t) {
+            for (uint64_t i = 0; i < k * d; ++i)
+                tot[i] += sums[(size_t)t * k * d + i];
+            for (uint64_t c = 0; c < k; ++c)
+                masses[c] += (uint32_t)ms[(size_t)t * k + c];
+        }
+        ora_divide_from_f64(tot, masses, d, k, cen);
+    }
+    free(tid);
+    free(jobs);
+    free(sums);
+    free(ms);
+    free(tot);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reductions and chunk layout                                               */
+/* ------------------------------------------------------------------------ */
+
+/* out[row] = sum over col of data[col * rows + row], uint32 wrap-around:
+ * the verifier of the reference's ReduceVectorParcol tests,
+ * test/reduce_vector.cpp:25-38. */
+void ora_reduce_u32(const uint32_t *data, uint64_t cols, uint64_t rows, uint32_t *out)
+{
+    memset(out, 0, rows * sizeof(uint32_t));
+    for (uint64_t c = 0; c < cols; ++c)
+        for (uint64_t r = 0; r < rows; ++r)
+            out[r] += data[c * rows + r];
+}
+
+/* Repack a feature-major byte matrix of num_dims rows into chunks of
+ * buffer_size bytes, each chunk feature-major over its own points:
+ * buffer_helper.cpp:16-87.  Returns the number of chunks or -1. */
+int64_t ora_partition_matrix(const void *src, void *dst, uint64_t size,
+                             uint64_t num_dims, uint64_t buffer_size)
+{
+    if (size % num_dims != 0 || buffer_size % num_dims != 0)
+        return -1;
+    uint64_t dim_size = size / num_dims;
+    uint64_t buf_dim_size = buffer_size / num_dims;
+    uint64_t num_bufs = (size + buffer_size - 1) / buffer_size;
+    const char *s = (const char *)src;
+    char *o = (char *)dst;
+    for (uint64_t b = 0; b < num_bufs; ++b) {
+        for (uint64_t v = 0; v < num_dims; ++v) {
+            uint64_t real = (buf_dim_size * (b + 1) > dim_size)
+                                ? dim_size - b * buf_dim_size
+                                : buf_dim_size;
+            memcpy(&o[b * buffer_size + v * real], &s[v * dim_size + b * buf_dim_size], real);
+        }
+    }
+    return (int64_t)num_bufs;
+}
