@@ -54,6 +54,7 @@ EXPORTS = [
     "clk_labeling", "clk_mass_update", "clk_centroid_update", "clk_matrix_divide_row",
     "clk_matrix_add_u32", "clk_matrix_add_f64", "clk_reduce_vector_parcol_u32",
     "clk_reduce_vector_parcol_f64", "clk_fill_zero", "clk_init_first_x", "clk_generate_blobs",
+    "clk_kmeans_run", "clk_conf_describe",
 ]
 
 
@@ -93,6 +94,9 @@ def load(path=None):
         "clk_fill_zero": (i32, [vp, vp, u64, vp]),
         "clk_init_first_x": (i32, [vp, u64, u64, u64, vp, vp, vp]),
         "clk_generate_blobs": (i32, [u64, u64, u64, dp, dp, u64, vp, vp, i32]),
+        "clk_kmeans_run": (i32, [ctypes.c_char_p, i32, u64, u64, u64, vp, vp, vp, vp, u64, dp,
+                                 ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+        "clk_conf_describe": (i32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -128,6 +132,35 @@ def generate_blobs(n, d, k, sigma=1.0, box=50.0, seed=0, threads=None):
     check(load().clk_generate_blobs(n, d, k, float(sigma), float(box), int(seed),
                                     pts.ctypes.data, cen.ctypes.data, int(threads)))
     return pts, cen
+
+
+def conf_describe(conf_text):
+    """Parsed .conf as a dict (ConfigurationParser rules; no GPU)."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    check(load().clk_conf_describe(conf_text.encode(), buf, len(buf)))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        k, _, v = line.partition("=")
+        out[k] = v
+    return out
+
+
+def kmeans_run(conf_text, points, centroids, device=0, buffer_size=0, tolerance=0.0):
+    """One clustering run of the strategy the .conf names (host arrays).
+    points [d][n] float32, centroids [d][k] float32 (initial).  Returns
+    (centroids, labels, masses, iterations_run, total_time_ns)."""
+    pts = np.ascontiguousarray(points, dtype=np.float32)
+    d, n = pts.shape
+    cen = np.array(centroids, dtype=np.float32, copy=True, order="C")
+    k = cen.shape[1]
+    masses = np.zeros(k, np.uint32)
+    labels = np.zeros(max(n, 1), np.uint32)
+    it = ctypes.c_uint64()
+    ns = ctypes.c_uint64()
+    check(load().clk_kmeans_run(conf_text.encode(), device, d, n, k, pts.ctypes.data, cen.ctypes.data,
+                                masses.ctypes.data, labels.ctypes.data, int(buffer_size), float(tolerance),
+                                ctypes.byref(it), ctypes.byref(ns)))
+    return cen, labels[:n], masses, it.value, ns.value
 
 
 def _p(x):

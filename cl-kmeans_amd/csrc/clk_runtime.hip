@@ -306,6 +306,9 @@ using namespace clk;
 
 extern "C" const char *clk_last_error(void) { return g_last_error.c_str(); }
 
+// For the host layer's C entry points (C++ exceptions -> clk_last_error).
+extern "C" void clk_set_error_message(const char *msg) { clk::set_error(msg ? msg : ""); }
+
 extern "C" int clk_context_create(int device, clk_context **out)
 {
     CLK_REQUIRE(out != nullptr, CLK_ERR_INVALID_ARGUMENT, "clk_context_create: null out");
