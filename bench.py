@@ -43,6 +43,8 @@ def main():
     ap.add_argument("--strategy", default="cluster_merge")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (scales rocprofv3 byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -52,6 +54,22 @@ def main():
 
     import clkmeans
     from clkmeans import hip
+
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        a, b = hip.DeviceArray((nbytes,), np.uint8), hip.DeviceArray((nbytes,), np.uint8)
+        e0, e1 = hip.Event(), hip.Event()
+        hip.check(hip.lib().hipMemcpy(b.ptr, a.ptr, nbytes, hip.hipMemcpyDeviceToDevice), "hipMemcpy D2D")
+        e0.record()
+        for _ in range(3):
+            hip.check(hip.lib().hipMemcpyAsync(b.ptr, a.ptr, nbytes, hip.hipMemcpyDeviceToDevice, None),
+                      "hipMemcpyAsync D2D")
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / 3
+        print(json.dumps({"calibration": "hipMemcpy D2D", "bytes_each": nbytes, "ms_each": ms,
+                          "read_plus_write_GBps": 2 * nbytes / (ms * 1e-3) / 1e9}))
+        return
 
     n, d, k = 1 << args.log2n, args.d, args.k
     threads = min(16, os.cpu_count() or 1)
@@ -75,15 +93,16 @@ def main():
 
     def iteration():
         old, new = cen[0], cen[1]
-        ctx.fill_zero(masses, stream=s)
-        ctx.fill_zero(sums, stream=s)
         if args.pipeline == "single_stage":
-            ctx.fused(fcfg, d, n, k, dp, old, sums, labels, masses, stream=s)
+            # zero + fused pass + FP64 divide: one launch
+            ctx.lloyd_step(fcfg, d, n, k, dp, old, new, sums, labels, masses, stream=s)
         else:
+            ctx.fill_zero(masses, stream=s)
+            ctx.fill_zero(sums, stream=s)
             ctx.labeling(lcfg, d, n, k, dp, old, labels, stream=s)
             ctx.mass_update(mcfg, n, k, labels, masses, stream=s)
             ctx.centroid_update(ccfg, d, n, k, dp, sums, labels, masses, stream=s)
-        ctx.divide(d, k, sums, masses, new, stream=s)
+            ctx.divide(d, k, sums, masses, new, stream=s)
         cen[0], cen[1] = new, old
 
     for _ in range(args.warmup):

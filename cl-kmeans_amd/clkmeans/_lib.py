@@ -54,7 +54,7 @@ EXPORTS = [
     "clk_labeling", "clk_mass_update", "clk_centroid_update", "clk_matrix_divide_row",
     "clk_matrix_add_u32", "clk_matrix_add_f64", "clk_reduce_vector_parcol_u32",
     "clk_reduce_vector_parcol_f64", "clk_fill_zero", "clk_init_first_x", "clk_generate_blobs",
-    "clk_kmeans_run", "clk_conf_describe",
+    "clk_kmeans_run", "clk_conf_describe", "clk_lloyd_step",
 ]
 
 
@@ -84,6 +84,7 @@ def load(path=None):
                                     ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float)]),
         "clk_fused": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp, vp, vp]),
         "clk_labeling": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp]),
+        "clk_lloyd_step": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp, vp, vp, vp, vp]),
         "clk_mass_update": (i32, [vp, cfgp, u64, u64, vp, vp, vp]),
         "clk_centroid_update": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp, vp]),
         "clk_matrix_divide_row": (i32, [vp, u64, u64, vp, vp, vp, vp, vp, vp]),
@@ -218,6 +219,12 @@ class Context:
     def fused(self, cfg, d, n, k, points, old_centroids, sums, labels, masses, stream=None):
         check(self.L.clk_fused(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),
                                _p(old_centroids), _p(sums), _p(labels), _p(masses), _s(stream)))
+
+    def lloyd_step(self, cfg, d, n, k, points, old_centroids, new_centroids, sums, labels, masses,
+                   max_shift=None, stream=None):
+        check(self.L.clk_lloyd_step(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),
+                                    _p(old_centroids), _p(new_centroids), _p(sums), _p(labels), _p(masses),
+                                    _p(max_shift), _s(stream)))
 
     def labeling(self, cfg, d, n, k, points, centroids, labels, stream=None):
         check(self.L.clk_labeling(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),

@@ -165,10 +165,15 @@ __device__ __forceinline__ void load_points(const float *__restrict__ pts,
                                             uint64_t n, uint32_t d, uint64_t p,
                                             float (&x)[DMAX][VEC])
 {
+    // Row bases are work-group uniform (SGPRs); when every row offset fits in
+    // 32 bits the per-lane address is one 32-bit offset (saddr addressing).
+    const bool off32 = n < (1ull << 30);
+    const uint32_t p32 = (uint32_t)p;
 #pragma unroll
     for (int f = 0; f < DMAX; ++f) {
         if (EXACT || f < (int)d) {
-            const float *row = pts + (uint64_t)f * n + p;
+            const float *rowbase = pts + (uint64_t)f * n;
+            const float *row = off32 ? rowbase + p32 : rowbase + p;
             if constexpr (VEC == 1) {
                 x[f][0] = stream_load(row);
             } else if constexpr (VEC == 2) {

@@ -17,6 +17,7 @@
 // partials; a fixed-order column reduction adds the partials into the
 // caller's FP64 sums and uint32 masses.
 #include "clk_common.hpp"
+#include "clk_finalize.hpp"
 #include "clk_kernels.hpp"
 
 #include <algorithm>
@@ -58,7 +59,9 @@ fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
         for (int v = 0; v < VEC; ++v) {
             float *t = mine + __umul24(lab[v], row_stride);
             if (PRIVATE) {
-                lds_acc(reinterpret_cast<uint32_t *>(t), 1u);
+                // u32 LDS atomics are full speed (unlike ds_add_f32): one
+                // ds_add_u32, no VALU, for the mass counter.
+                lds_add(reinterpret_cast<uint32_t *>(t), 1u);
 #pragma unroll
                 for (int f = 0; f < DMAX; ++f)
                     if (EXACT || f < (int)d)
@@ -169,15 +172,19 @@ static int block_size(clk_context *ctx, const clk_kernel_config *cfg)
     return (int)b;
 }
 
+// mode 1: sums/masses accumulate (clk_fused).  mode 2: sums/masses are
+// overwritten and new_centroids = divide (clk_lloyd_step).
 int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64_t n, uint64_t k,
               const float *points, const float *old_centroids, double *sums, uint32_t *labels,
-              uint32_t *masses, hipStream_t stream)
+              uint32_t *masses, hipStream_t stream, uint32_t mode, float *new_centroids,
+              double *max_shift)
 {
     CLK_TRY(check_sizes(d, n, k));
     CLK_REQUIRE(points && old_centroids && sums && labels && masses, CLK_ERR_INVALID_ARGUMENT,
                 "fused: null buffer");
-    if (n == 0)
-        return CLK_OK;
+    CLK_REQUIRE(mode == 1 || new_centroids, CLK_ERR_INVALID_ARGUMENT, "lloyd_step: null new_centroids");
+    CLK_REQUIRE(mode == 1 || new_centroids != old_centroids, CLK_ERR_INVALID_ARGUMENT,
+                "lloyd_step: new_centroids must not alias old_centroids");
     const int strategy = cfg ? cfg->strategy : CLK_FUSED_CLUSTER_MERGE;
     CLK_REQUIRE(strategy == CLK_FUSED_CLUSTER_MERGE || strategy == CLK_FUSED_FEATURE_SUM,
                 CLK_ERR_INVALID_ARGUMENT, "fused: unknown strategy id");
@@ -196,9 +203,19 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
     bool use_private = dmax && strategy == CLK_FUSED_CLUSTER_MERGE && priv_bytes <= ctx->lds_per_block;
     bool use_shared = dmax && !use_private && shared_bytes <= ctx->lds_per_block;
 
-    if (!use_private && !use_shared) {
-        CLK_TRY(launch_labeling(ctx, cfg, d, n, k, points, old_centroids, labels, stream));
-        return launch_accumulate_global(ctx, d, n, k, points, labels, sums, masses, stream);
+    if (!use_private && !use_shared || n == 0) {
+        if (mode == 2) {
+            CLK_TRY(clk_fill_zero(ctx, sums, d * k * sizeof(double), stream));
+            CLK_TRY(clk_fill_zero(ctx, masses, k * sizeof(uint32_t), stream));
+        }
+        if (n) {
+            CLK_TRY(launch_labeling(ctx, cfg, d, n, k, points, old_centroids, labels, stream));
+            CLK_TRY(launch_accumulate_global(ctx, d, n, k, points, labels, sums, masses, stream));
+        }
+        if (mode == 2)
+            return clk_matrix_divide_row(ctx, d, k, sums, masses, new_centroids,
+                                         max_shift ? old_centroids : nullptr, max_shift, stream);
+        return CLK_OK;
     }
 
     const bool b256 = (B == 256);
@@ -213,15 +230,39 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
 
     CLK_TRY(ensure_scratch(ctx->part_sums, grid * d * k * sizeof(double)));
     CLK_TRY(ensure_scratch(ctx->part_masses, grid * k * sizeof(uint32_t)));
-    double *ps = static_cast<double *>(ctx->part_sums.ptr);
-    uint32_t *pm = static_cast<uint32_t *>(ctx->part_masses.ptr);
+    FinalizeArgs fa;
+    fa.part_sums = static_cast<double *>(ctx->part_sums.ptr);
+    fa.part_masses = static_cast<uint32_t *>(ctx->part_masses.ptr);
+    fa.rows = (uint32_t)grid;
+    fa.sums = sums;
+    fa.masses = masses;
+    fa.centroids = new_centroids;
+    fa.old_centroids = max_shift ? old_centroids : nullptr;
+    fa.max_shift = max_shift;
+    fa.d = (uint32_t)d;
+    fa.k = (uint32_t)k;
+    fa.mode = mode;
     {
         KernelTimer t(ctx, use_private ? "FusedClusterMerge" : "FusedFeatureSum", stream);
-        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(B), lds, stream, points, n,
-                           (uint32_t)d, (uint32_t)k, old_centroids, labels, ps, pm);
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(B), lds, stream, points, n, (uint32_t)d,
+                           (uint32_t)k, old_centroids, labels, static_cast<double *>(ctx->part_sums.ptr),
+                           static_cast<uint32_t *>(ctx->part_masses.ptr));
         CLK_HIP_CHECK(hipGetLastError());
     }
-    return launch_reduce_partials(ctx, ps, d * k, sums, pm, k, masses, grid, stream);
+    const size_t fin_lds = d * k * sizeof(double) + k * sizeof(uint32_t) + 12 * 1024 + 16;
+    if (fin_lds <= ctx->lds_per_block)
+        return launch_finalize(ctx, fa, stream);
+    // d*k beyond the one-work-group merge: column reduction, then divide.
+    if (mode == 2) {
+        CLK_TRY(clk_fill_zero(ctx, sums, d * k * sizeof(double), stream));
+        CLK_TRY(clk_fill_zero(ctx, masses, k * sizeof(uint32_t), stream));
+    }
+    CLK_TRY(launch_reduce_partials(ctx, fa.part_sums, d * k, sums, fa.part_masses, k, masses, grid,
+                                   stream));
+    if (mode == 2)
+        return clk_matrix_divide_row(ctx, d, k, sums, masses, new_centroids,
+                                     max_shift ? old_centroids : nullptr, max_shift, stream);
+    return CLK_OK;
 }
 
 }  // namespace clk
@@ -234,5 +275,17 @@ extern "C" int clk_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_
     CLK_REQUIRE(!cfg || cfg->stage == CLK_STAGE_FUSED, CLK_ERR_INVALID_ARGUMENT,
                 "clk_fused: config is not a fused-stage config");
     return clk::run_fused(ctx, cfg, d, n, k, points, old_centroids, sums, labels, masses,
-                          clk::as_stream(stream));
+                          clk::as_stream(stream), 1, nullptr, nullptr);
+}
+
+extern "C" int clk_lloyd_step(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
+                              uint64_t n, uint64_t k, const float *points,
+                              const float *old_centroids, float *new_centroids, double *sums,
+                              uint32_t *labels, uint32_t *masses, double *max_shift, void *stream)
+{
+    CLK_TRY(clk::enter(ctx));
+    CLK_REQUIRE(!cfg || cfg->stage == CLK_STAGE_FUSED, CLK_ERR_INVALID_ARGUMENT,
+                "clk_lloyd_step: config is not a fused-stage config");
+    return clk::run_fused(ctx, cfg, d, n, k, points, old_centroids, sums, labels, masses,
+                          clk::as_stream(stream), 2, new_centroids, max_shift);
 }

@@ -17,6 +17,11 @@ int launch_reduce_partials(clk_context *ctx, const double *ps, uint64_t rows_s, 
                            const uint32_t *pm, uint64_t rows_m, uint32_t *masses, uint64_t cols,
                            hipStream_t stream);
 
+struct FinalizeArgs;
+// Merge per-work-group partials into sums/masses (mode 1 accumulate, mode 2
+// overwrite + FP64 divide + optional shift): one work-group launch.
+int launch_finalize(clk_context *ctx, const FinalizeArgs &a, hipStream_t stream);
+
 // Smallest register bucket >= d among 1,2,4,8,16,32,64; 0 when d > 64.
 inline int dmax_bucket(uint64_t d)
 {

@@ -3,6 +3,7 @@
 // column-vector reduction (ReduceVectorParcol) and the FP64 divide
 // (MatrixBinaryOp::row).
 #include "clk_common.hpp"
+#include "clk_finalize.hpp"
 #include "clk_kernels.hpp"
 
 #include <algorithm>
@@ -274,6 +275,126 @@ int launch_reduce_partials(clk_context *ctx, const double *ps, uint64_t rows_s, 
     return CLK_OK;
 }
 
+namespace {
+
+// One work-group of kFinalizeThreads threads.  Thread t sums values
+// v = t mod 64 + 64*i over the row slice {g, g+G, ...}, g = t / 64,
+// G = 16, with four interleaved accumulators; the 16 slice sums of a value
+// are then added in slice order.  Dynamic LDS: d*k doubles + k uint32.
+__global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(FinalizeArgs a)
+{
+    extern __shared__ __align__(16) double fin_lds[];
+    __shared__ double part_d[kFinalizeThreads];
+    __shared__ uint32_t part_u[kFinalizeThreads];
+    __shared__ unsigned long long s_max;  // bits of a non-negative double
+    const uint32_t tid = threadIdx.x, lane = tid % 64, g = tid / 64;
+    constexpr uint32_t G = kFinalizeThreads / 64;
+    const uint64_t dk = (uint64_t)a.d * a.k;
+    const uint64_t V = dk + a.k;
+    double *tot_s = fin_lds;
+    uint32_t *tot_m = reinterpret_cast<uint32_t *>(tot_s + dk);
+
+    for (uint64_t v0 = 0; v0 < V; v0 += 64) {
+        const uint64_t v = v0 + lane;
+        double sd = 0.0;
+        uint32_t su = 0;
+        if (v < dk) {
+            // eight independent load chains per thread
+            double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            uint32_t r = g;
+            for (; r + 7 * G < a.rows; r += 8 * G) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    acc[u] += a.part_sums[(uint64_t)(r + u * G) * dk + v];
+            }
+            for (; r < a.rows; r += G)
+                acc[0] += a.part_sums[(uint64_t)r * dk + v];
+            sd = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        } else if (v < V) {
+            const uint64_t c = v - dk;
+            uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            uint32_t r = g;
+            for (; r + 7 * G < a.rows; r += 8 * G) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    acc[u] += a.part_masses[(uint64_t)(r + u * G) * a.k + c];
+            }
+            for (; r < a.rows; r += G)
+                acc[0] += a.part_masses[(uint64_t)r * a.k + c];
+            for (int u = 0; u < 8; ++u)
+                su += acc[u];
+        }
+        part_d[tid] = sd;
+        part_u[tid] = su;
+        __syncthreads();
+        if (g == 0 && v < V) {
+            if (v < dk) {
+                double t = part_d[lane];
+                for (uint32_t j = 1; j < G; ++j)
+                    t += part_d[j * 64 + lane];
+                if (a.mode == 1)
+                    a.sums[v] += t;
+                else
+                    a.sums[v] = t;
+                tot_s[v] = t;
+            } else {
+                uint32_t t = part_u[lane];
+                for (uint32_t j = 1; j < G; ++j)
+                    t += part_u[j * 64 + lane];
+                const uint64_t c = v - dk;
+                if (a.mode == 1)
+                    a.masses[c] += t;
+                else
+                    a.masses[c] = t;
+                tot_m[c] = t;
+            }
+        }
+        __syncthreads();
+    }
+    if (a.mode != 2)
+        return;
+    if (tid == 0)
+        s_max = 0ull;
+    __syncthreads();
+    // FP64 divide, one rounding to float; empty cluster -> 0/0 = NaN.
+    double local = 0.0;
+    for (uint32_t c = tid; c < a.k; c += kFinalizeThreads) {
+        const double m = (double)tot_m[c];
+        double sq = 0.0;
+        for (uint32_t f = 0; f < a.d; ++f) {
+            const uint64_t i = (uint64_t)f * a.k + c;
+            const float v = (float)(tot_s[i] / m);
+            a.centroids[i] = v;
+            if (a.old_centroids) {
+                const double diff = (double)v - (double)a.old_centroids[i];
+                sq += diff * diff;
+            }
+        }
+        if (tot_m[c] != 0u)
+            local = fmax(local, sqrt(sq));
+    }
+    if (!a.max_shift || !a.old_centroids)
+        return;
+    // Non-negative doubles order like their bit patterns: integer max.
+    atomicMax(&s_max, (unsigned long long)__double_as_longlong(local));
+    __syncthreads();
+    if (tid == 0)
+        *a.max_shift = __longlong_as_double((long long)s_max);
+}
+
+}  // namespace
+
+int launch_finalize(clk_context *ctx, const FinalizeArgs &a, hipStream_t stream)
+{
+    const size_t lds = (size_t)a.d * a.k * sizeof(double) + (size_t)a.k * sizeof(uint32_t);
+    CLK_REQUIRE(lds + 12 * 1024 + 16 <= ctx->lds_per_block, CLK_ERR_UNSUPPORTED,
+                "finalize: d*k too large for the one-work-group merge");
+    KernelTimer t(ctx, a.mode == 2 ? "FinalizeDivide" : "ReduceVectorParcol", stream);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kFinalizeThreads), lds, stream, a);
+    CLK_HIP_CHECK(hipGetLastError());
+    return CLK_OK;
+}
+
 int launch_reduce_f64(clk_context *ctx, const double *in, uint64_t cols, uint64_t rows,
                       double *out, bool accumulate, hipStream_t stream)
 {
@@ -337,8 +458,9 @@ extern "C" int clk_context_destroy(clk_context *ctx)
     (void)hipDeviceSynchronize();
     if (ctx->part_sums.ptr)
         (void)hipFree(ctx->part_sums.ptr);
-    if (ctx->part_masses.ptr)
-        (void)hipFree(ctx->part_masses.ptr);
+    for (clk::Scratch *sc : {&ctx->part_masses})
+        if (sc->ptr)
+            (void)hipFree(sc->ptr);
     for (auto &r : ctx->records) {
         (void)hipEventDestroy(r.start);
         (void)hipEventDestroy(r.stop);

@@ -149,15 +149,16 @@ public:
     void set_fused(FusedConfiguration config)
     {
         FusedFactory<PointT, LabelT, MassT, ColMajor> factory;
-        f_fused = factory.create(context, config, *this->measurement);
+        f_step = factory.create_step(context, config, *this->measurement);
     }
 
+    // Each iteration is one clk_lloyd_step: zero + fused pass + FP64 divide
+    // (kmeans_single_stage.hpp:95-148), then the centroid buffers swap.
     void run() override
     {
         const size_t d = this->num_features, n = this->num_points, k = this->num_clusters;
         auto &m = *this->measurement;
         buf.setup(queue, d, n, k, *this->host_points, *this->host_centroids, m);
-        matrix_divide.prepare(context, MatrixBinaryOp<PointT, MassT>::Divide);
         if (this->centroids_initializer)
             this->centroids_initializer(queue, d, n, k, buf.points.data(), buf.centroids.data());
         queue.finish();
@@ -165,18 +166,11 @@ public:
         auto t0 = std::chrono::steady_clock::now();
         WaitList none;
         this->iterations_done = 0;
+        const bool track = this->convergence_tolerance > 0.0;
         for (size_t it = 0; it < this->max_iterations; ++it) {
-            check_status(clk_fill_zero(context.get(), buf.masses.data(), k * sizeof(MassT), queue.get()), "fill");
-            check_status(clk_fill_zero(context.get(), buf.sums.data(), k * d * sizeof(AccT), queue.get()), "fill");
-            f_fused(queue, d, n, k, buf.points.begin(), buf.points.begin() + n * d, buf.centroids.begin(),
-                    buf.centroids.end(), buf.sums.begin(), buf.sums.end(), buf.labels.begin(),
-                    buf.labels.begin() + n, buf.masses.begin(), buf.masses.end(), m.add_datapoint((int)it),
-                    none);
-            const bool track = this->convergence_tolerance > 0.0;
-            matrix_divide.row(queue, d, k, buf.sums.begin(), buf.sums.end(), buf.masses.begin(),
-                              buf.masses.end(), buf.new_centroids.begin(), buf.new_centroids.end(),
-                              m.add_datapoint((int)it), none, track ? buf.centroids.data() : nullptr,
-                              track ? buf.shift.data() : nullptr);
+            f_step(queue, d, n, k, buf.points.data(), buf.centroids.data(), buf.new_centroids.data(),
+                   buf.sums.data(), buf.labels.data(), buf.masses.data(), track ? buf.shift.data() : nullptr,
+                   m.add_datapoint((int)it), none);
             std::swap(buf.centroids, buf.new_centroids);
             ++this->iterations_done;
             if (this->converged(queue, buf.shift))
@@ -191,8 +185,7 @@ public:
 private:
     ContextHandle context;
     Queue queue;
-    FusedFunction f_fused;
-    MatrixBinaryOp<PointT, MassT> matrix_divide;
+    typename FusedFactory<PointT, LabelT, MassT, ColMajor>::LloydStepFunction f_step;
     ResidentBuffers<PointT, LabelT, MassT> buf;
 };
 

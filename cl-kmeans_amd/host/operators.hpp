@@ -73,6 +73,36 @@ public:
         BufferIterator<MassT> masses_begin, BufferIterator<MassT> masses_end,
         Measurement::DataPoint &datapoint, const WaitList &events)>;
 
+    // One whole single_stage iteration (zero, fused pass, FP64 divide) as one
+    // operator: clk_lloyd_step.  sums/masses are outputs; max_shift optional.
+    using LloydStepFunction = std::function<Event(
+        const Queue &queue, size_t num_features, size_t num_points, size_t num_clusters,
+        const PointT *points, const PointT *old_centroids, PointT *new_centroids, AccT *sums,
+        LabelT *labels, MassT *masses, double *max_shift, Measurement::DataPoint &datapoint,
+        const WaitList &events)>;
+
+    LloydStepFunction create_step(ContextHandle context, FusedConfiguration config,
+                                  Measurement::Measurement &measurement)
+    {
+        (void)create(context, config, measurement);  // validates the strategy, logs parameters
+        int id = clk_strategy_id(CLK_STAGE_FUSED, config.strategy.c_str());
+        clk_kernel_config k;
+        check_status(clk_default_config(context.get(), CLK_STAGE_FUSED, id, &k), "clk_default_config");
+        fill_sizes(k, config.global_size, config.local_size);
+        k.vector_length = config.vector_length;
+        const std::string name = id == CLK_FUSED_CLUSTER_MERGE ? "FusedClusterMerge" : "FusedFeatureSum";
+        return [context, k, name](const Queue &q, size_t d, size_t n, size_t kk, const PointT *pts,
+                                  const PointT *oc, PointT *nc, AccT *sums, LabelT *labels, MassT *masses,
+                                  double *shift, Measurement::DataPoint &dp, const WaitList &events) {
+            dp.set_name(name);
+            return timed_enqueue(q, dp, events, [&](hipStream_t s) {
+                check_status(clk_lloyd_step(context.get(), &k, d, n, kk, pts, oc, nc, sums, labels,
+                                            masses, shift, s),
+                             "clk_lloyd_step");
+            });
+        };
+    }
+
     FusedFunction create(ContextHandle context, FusedConfiguration config,
                          Measurement::Measurement &measurement)
     {

@@ -133,6 +133,21 @@ int clk_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
               uint32_t *masses, void *stream);
 
 /*
+ * One whole Lloyd iteration of the single_stage pipeline: the body of
+ * KmeansSingleStage::run's loop (kmeans_single_stage.hpp:92-151: zero
+ * masses and sums, fused pass, divide), as one kernel launch on the LDS-tile
+ * paths.  labels, masses and sums are OVERWRITTEN (not accumulated);
+ * new_centroids[f,c] = (float)(sums[f,c] / (double)masses[c]) (NaN for an
+ * empty cluster); when max_shift (device double) is non-NULL it receives
+ * max_c ||new_c - old_c||_2 over non-empty clusters.  new_centroids must not
+ * alias old_centroids.  Calls sharing a context must be ordered on one stream.
+ */
+int clk_lloyd_step(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
+                   uint64_t n, uint64_t k, const float *points,
+                   const float *old_centroids, float *new_centroids, double *sums,
+                   uint32_t *labels, uint32_t *masses, double *max_shift, void *stream);
+
+/*
  * Labeling only.  Replaces LabelingFactory::LabelingFunction
  * (labeling_factory.hpp:33-48) / LabelingUnrollVector
  * (labeling_unroll_vector.hpp:168-272): same arithmetic as clk_fused.

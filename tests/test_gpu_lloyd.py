@@ -33,13 +33,16 @@ def lloyd_gpu(ctx, pts, init, iterations, pipeline, check_each=False):
             c_in = old.to_host()
         ctx.fill_zero(masses)
         ctx.fill_zero(sums)
-        if pipeline == "single_stage":
+        if pipeline == "single_stage_step":
+            ctx.lloyd_step(fcfg, d, n, k, dp, old, new, sums, labels, masses)
+        elif pipeline == "single_stage":
             ctx.fused(fcfg, d, n, k, dp, old, sums, labels, masses)
         else:
             ctx.labeling(lcfg, d, n, k, dp, old, labels)
             ctx.mass_update(mcfg, n, k, labels, masses)
             ctx.centroid_update(ccfg, d, n, k, dp, sums, labels, masses)
-        ctx.divide(d, k, sums, masses, new)
+        if pipeline != "single_stage_step":
+            ctx.divide(d, k, sums, masses, new)
         if check_each:
             hip.synchronize()
             lab = labels.to_host()
@@ -62,7 +65,7 @@ def c1_data():
     return pts, init, ref
 
 
-@pytest.mark.parametrize("pipeline", ["single_stage", "three_stage"])
+@pytest.mark.parametrize("pipeline", ["single_stage", "single_stage_step", "three_stage"])
 def test_lloyd_30_iterations_c1(gpu_ctx, c1_data, pipeline):
     pts, init, (rc, rl, rm) = c1_data
     c, l, m = lloyd_gpu(gpu_ctx, pts, init, 30, pipeline)
@@ -78,6 +81,7 @@ def test_lloyd_each_iteration_pinned(gpu_ctx):
     pts, _ = clkmeans.generate_blobs(1 << 16, 4, 10, sigma=3.0, box=10.0, seed=5)
     init = pts[:, ::4][:, :10].copy()
     lloyd_gpu(gpu_ctx, pts, init, 12, "single_stage", check_each=True)
+    lloyd_gpu(gpu_ctx, pts, init, 12, "single_stage_step", check_each=True)
     lloyd_gpu(gpu_ctx, pts, init, 6, "three_stage", check_each=True)
 
 

@@ -235,3 +235,80 @@ def test_errors_are_raised(gpu_ctx):
     with pytest.raises(AssertionError):
         gpu_ctx.fused(gpu_ctx.config(FUSED, "cluster_merge"), 0, 10, 3, zeros(1, np.float32),
                       zeros(1, np.float32), zeros(1, np.float64), zeros(1, np.uint32), zeros(1, np.uint32))
+
+
+@pytest.mark.parametrize("strategy", ["cluster_merge", "feature_sum"])
+def test_lloyd_step_and_repeated_launches(gpu_ctx, strategy):
+    """One-launch iteration (in-kernel merge + divide) for grids that are not
+    multiples of the merge group, repeated so that the merge counters must
+    have been reset by every launch; accumulate mode doubles on a 2nd call."""
+    n, d, k = 100000, 4, 10
+    pts = blobs(n, d, k, seed=21)
+    cen = O.first_x(pts, k)
+    ref_lab = O.label(pts, cen)
+    ref_m = O.mass(ref_lab, k)
+    ref_s = O.sums_f64(pts, ref_lab, k)
+    tol = sum_tolerance(pts, ref_lab, k)
+    dp, dc = dev(pts), dev(cen)
+    for grid_blocks in (1, 15, 16, 17, 33, 400, 0, 17, 1):
+        kw = {"global_size": grid_blocks * 256} if grid_blocks else {}
+        cfg = gpu_ctx.config(FUSED, strategy, **kw)
+        newc, sums, lab, mass = zeros((d, k), np.float32), dev(np.full((d, k), 7.0)), zeros(n, np.uint32), \
+            dev(np.full(k, 5, np.uint32))
+        shift = zeros(1, np.float64)
+        gpu_ctx.lloyd_step(cfg, d, n, k, dp, dc, newc, sums, lab, mass, max_shift=shift)
+        hip.synchronize()
+        np.testing.assert_array_equal(lab.to_host(), ref_lab)
+        m = mass.to_host()
+        np.testing.assert_array_equal(m, ref_m)                    # overwritten, not accumulated
+        s = sums.to_host()
+        assert np.all(np.abs(s - ref_s) <= tol)
+        np.testing.assert_array_equal(newc.to_host(), O.divide_from_f64(s, m))  # divide of its own sums
+        ref_shift = np.sqrt(((O.divide_from_f64(s, m).astype(np.float64) - cen) ** 2).sum(0)).max()
+        assert abs(shift.to_host()[0] - ref_shift) <= 1e-12 * ref_shift
+        # accumulate mode, twice
+        s2, m2 = zeros((d, k), np.float64), zeros(k, np.uint32)
+        gpu_ctx.fused(cfg, d, n, k, dp, dc, s2, lab, m2)
+        gpu_ctx.fused(cfg, d, n, k, dp, dc, s2, lab, m2)
+        hip.synchronize()
+        np.testing.assert_array_equal(m2.to_host(), 2 * ref_m)
+        assert np.all(np.abs(s2.to_host() - 2 * ref_s) <= 2 * tol)
+
+
+def test_fused_is_deterministic(gpu_ctx):
+    """Fixed reduction order: identical FP64 sums bit for bit across runs."""
+    n, d, k = 1 << 18, 4, 10
+    pts = blobs(n, d, k, seed=3)
+    cen = O.first_x(pts, k)
+    dp, dc = dev(pts), dev(cen)
+    outs = []
+    for _ in range(3):
+        newc, sums, lab, mass = zeros((d, k), np.float32), zeros((d, k), np.float64), zeros(n, np.uint32), \
+            zeros(k, np.uint32)
+        gpu_ctx.lloyd_step(gpu_ctx.config(FUSED, "cluster_merge"), d, n, k, dp, dc, newc, sums, lab, mass)
+        hip.synchronize()
+        outs.append(sums.to_host().view(np.uint64).copy())
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
+def test_lloyd_step_large_dk_uses_column_merge(gpu_ctx):
+    """d*k past the one-work-group merge but within the shared LDS tile
+    (d=32, k=600: 156 KiB tile, 150 KiB of FP64 totals)."""
+    n, d, k = 20000, 32, 600
+    pts = blobs(n, d, 64, seed=8)
+    cen = np.random.default_rng(3).standard_normal((d, k)).astype(np.float32) * 20
+    newc, sums, lab, mass = zeros((d, k), np.float32), zeros((d, k), np.float64), zeros(n, np.uint32), \
+        zeros(k, np.uint32)
+    gpu_ctx.lloyd_step(gpu_ctx.config(FUSED, "feature_sum"), d, n, k, dev(pts), dev(cen), newc, sums, lab, mass)
+    hip.synchronize()
+    ref = O.label(pts, cen)
+    np.testing.assert_array_equal(lab.to_host(), ref)
+    m = mass.to_host()
+    np.testing.assert_array_equal(m, O.mass(ref, k))
+    s = sums.to_host()
+    assert np.all(np.abs(s - O.sums_f64(pts, ref, k)) <= sum_tolerance(pts, ref, k))
+    got = newc.to_host()
+    exp = O.divide_from_f64(s, m)
+    live = m != 0
+    np.testing.assert_array_equal(got[:, live], exp[:, live])
+    assert np.all(np.isnan(got[:, ~live]))
