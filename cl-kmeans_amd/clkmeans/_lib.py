@@ -54,7 +54,7 @@ EXPORTS = [
     "clk_labeling", "clk_mass_update", "clk_centroid_update", "clk_matrix_divide_row",
     "clk_matrix_add_u32", "clk_matrix_add_f64", "clk_reduce_vector_parcol_u32",
     "clk_reduce_vector_parcol_f64", "clk_fill_zero", "clk_init_first_x", "clk_generate_blobs",
-    "clk_kmeans_run", "clk_conf_describe", "clk_lloyd_step",
+    "clk_kmeans_run", "clk_conf_describe", "clk_lloyd_step", "clk_labeling_refined_points",
 ]
 
 
@@ -98,6 +98,7 @@ def load(path=None):
         "clk_kmeans_run": (i32, [ctypes.c_char_p, i32, u64, u64, u64, vp, vp, vp, vp, u64, dp,
                                  ctypes.POINTER(u64), ctypes.POINTER(u64)]),
         "clk_conf_describe": (i32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
+        "clk_labeling_refined_points": (i32, [vp, ctypes.POINTER(ctypes.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -225,6 +226,12 @@ class Context:
         check(self.L.clk_lloyd_step(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),
                                     _p(old_centroids), _p(new_centroids), _p(sums), _p(labels), _p(masses),
                                     _p(max_shift), _s(stream)))
+
+    def labeling_refined_points(self):
+        """Points of the last MFMA labeling call that went through the exact refinement."""
+        out = ctypes.c_uint64(0)
+        check(self.L.clk_labeling_refined_points(self.handle, ctypes.byref(out)))
+        return out.value
 
     def labeling(self, cfg, d, n, k, points, centroids, labels, stream=None):
         check(self.L.clk_labeling(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),

@@ -65,6 +65,10 @@ struct clk_context {
     int max_threads_per_block = 0;
     clk::Scratch part_sums;       // per-block FP64 partial sums
     clk::Scratch part_masses;     // per-block uint32 partial masses
+    clk::Scratch mfma_tables;     // 16-bit centroid operand table, norms, bounds
+    clk::Scratch mfma_queue;      // point ids queued for exact refinement
+    uint32_t *mfma_qlen = nullptr;  // device queue length inside mfma_tables
+    int mfma_subnormals = -1;     // bit 0 FP16, bit 1 BF16: MFMA keeps subnormal inputs (-1 unknown)
     bool profiling = false;
     int32_t iteration = -1;
     std::vector<clk::TimingRecord> records;

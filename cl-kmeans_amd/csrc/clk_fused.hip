@@ -203,7 +203,7 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
     bool use_private = dmax && strategy == CLK_FUSED_CLUSTER_MERGE && priv_bytes <= ctx->lds_per_block;
     bool use_shared = dmax && !use_private && shared_bytes <= ctx->lds_per_block;
 
-    if (!use_private && !use_shared || n == 0) {
+    if ((!use_private && !use_shared) || n == 0) {
         if (mode == 2) {
             CLK_TRY(clk_fill_zero(ctx, sums, d * k * sizeof(double), stream));
             CLK_TRY(clk_fill_zero(ctx, masses, k * sizeof(uint32_t), stream));

@@ -37,6 +37,14 @@ int launch_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, 
                     uint64_t k, const float *points, const float *centroids, uint32_t *labels,
                     hipStream_t stream);
 
+// Labels from x·cᵀ on MFMA (16-bit operands, FP32 accumulation) with an exact
+// FP32 relabel of near ties: the same labels as launch_labeling.  Supported
+// for d <= 128 and n < 2^32 when the 16-bit centroid table fits in LDS.
+bool mfma_labeling_supported(const clk_context *ctx, uint64_t d, uint64_t n, uint64_t k);
+int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, uint64_t k,
+                         const float *points, const float *centroids, uint32_t *labels,
+                         hipStream_t stream);
+
 // sums[f,c] += x_f and masses[c] += 1 for every point, with FP64 / u32
 // global atomics (any k*d; summation order not fixed).  masses may be null.
 int launch_accumulate_global(clk_context *ctx, uint64_t d, uint64_t n, uint64_t k,

@@ -194,10 +194,15 @@ extern "C" int clk_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint
     CLK_TRY(clk::enter(ctx));
     CLK_REQUIRE(!cfg || cfg->stage == CLK_STAGE_LABELING, CLK_ERR_INVALID_ARGUMENT,
                 "clk_labeling: config is not a labeling-stage config");
-    CLK_REQUIRE(!cfg || cfg->strategy == CLK_LABELING_UNROLL_VECTOR, CLK_ERR_INVALID_ARGUMENT,
-                "clk_labeling: unknown strategy id");
+    const int strategy = cfg ? cfg->strategy : CLK_LABELING_UNROLL_VECTOR;
+    CLK_REQUIRE(strategy == CLK_LABELING_UNROLL_VECTOR || strategy == CLK_LABELING_MFMA_BF16 ||
+                    strategy == CLK_LABELING_MFMA_FP16,
+                CLK_ERR_INVALID_ARGUMENT, "clk_labeling: unknown strategy id");
     CLK_REQUIRE(d >= 1 && k >= 1 && k <= 0xFFFFFFFFull, CLK_ERR_SIZE, "labeling: bad d or k");
     CLK_REQUIRE(points && centroids && labels, CLK_ERR_INVALID_ARGUMENT, "labeling: null buffer");
+    if (strategy != CLK_LABELING_UNROLL_VECTOR && clk::mfma_labeling_supported(ctx, d, n, k))
+        return clk::launch_labeling_mfma(ctx, strategy == CLK_LABELING_MFMA_FP16, d, n, k, points,
+                                         centroids, labels, clk::as_stream(stream));
     return clk::launch_labeling(ctx, cfg, d, n, k, points, centroids, labels,
                                 clk::as_stream(stream));
 }

@@ -458,7 +458,7 @@ extern "C" int clk_context_destroy(clk_context *ctx)
     (void)hipDeviceSynchronize();
     if (ctx->part_sums.ptr)
         (void)hipFree(ctx->part_sums.ptr);
-    for (clk::Scratch *sc : {&ctx->part_masses})
+    for (clk::Scratch *sc : {&ctx->part_masses, &ctx->mfma_tables, &ctx->mfma_queue})
         if (sc->ptr)
             (void)hipFree(sc->ptr);
     for (auto &r : ctx->records) {
@@ -482,6 +482,8 @@ extern "C" int clk_strategy_id(int stage, const char *name)
         break;
     case CLK_STAGE_LABELING:
         if (s == "unroll_vector") return CLK_LABELING_UNROLL_VECTOR;
+        if (s == "mfma_bf16") return CLK_LABELING_MFMA_BF16;
+        if (s == "mfma_fp16") return CLK_LABELING_MFMA_FP16;
         break;
     case CLK_STAGE_MASS_UPDATE:
         if (s == "global_atomic") return CLK_MASS_GLOBAL_ATOMIC;

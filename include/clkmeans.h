@@ -58,6 +58,14 @@ typedef struct clk_context clk_context;
 #define CLK_FUSED_CLUSTER_MERGE 0   /* "cluster_merge"  fused_factory.hpp:73-77 */
 #define CLK_FUSED_FEATURE_SUM 1     /* "feature_sum"    fused_factory.hpp:78-82 */
 #define CLK_LABELING_UNROLL_VECTOR 0 /* "unroll_vector" labeling_factory.hpp:63-80 */
+/* Extensions (no reference counterpart; the factory would throw for them):
+ * x·c^T on MFMA with 16-bit operands and FP32 accumulation, then the exact
+ * FP32 distance for every point whose best-to-second margin is inside the
+ * derived error bound -- the labels equal "unroll_vector"'s.  Used for
+ * d <= 128 and n < 2^32 when the 16-bit centroid table fits in LDS; other
+ * sizes run the "unroll_vector" kernel. */
+#define CLK_LABELING_MFMA_BF16 1    /* "mfma_bf16" */
+#define CLK_LABELING_MFMA_FP16 2    /* "mfma_fp16" */
 #define CLK_MASS_GLOBAL_ATOMIC 0    /* "global_atomic"  mass_update_factory.hpp:71-75 */
 #define CLK_MASS_PART_GLOBAL 1      /* "part_global"    mass_update_factory.hpp:76-80 */
 #define CLK_MASS_PART_LOCAL 2       /* "part_local"     mass_update_factory.hpp:81-85 */
@@ -155,6 +163,12 @@ int clk_lloyd_step(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
 int clk_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
                  uint64_t n, uint64_t k, const float *points,
                  const float *centroids, uint32_t *labels, void *stream);
+
+/* Diagnostic of the MFMA labeling strategies: number of points of the last
+ * MFMA labeling call on this context whose margin test failed and that went
+ * through the exact refinement (synchronizes the device; 0 before any MFMA
+ * call).  No reference counterpart. */
+int clk_labeling_refined_points(clk_context *ctx, uint64_t *points);
 
 /*
  * masses[c] += #{p : labels[p] == c}.  Replaces
