@@ -123,26 +123,35 @@ __global__ void __launch_bounds__(64) mfma_subnormal_check_kernel(float *out)
     }
 }
 
-// Per-iteration centroid tables:
+// Per-iteration centroid tables (three small launches: clear, build, finish):
 //   table[t][kb][lane] = 8 x 16-bit of -2c for centroid t*16 + lane%16,
 //                        features kb*32 + 8*(lane/16) + j (0 past d or k)
 //   norm[c] = RN(||c||^2) (FP32); +inf for a padded or non-finite centroid,
 //             one whose ||c||^2 overflows, or (FP16) whose -2c overflows
+//   cenT[c][f] = the FP32 centroids transposed (ntiles*16 rows of nkb*32, 0
+//             past d or k), for the exact distances of the refinement
 //   stats[0] = C >= max ||c||, stats[1] = N >= max ||c||^2, stats[3] = R >=
 //   max ||c̃ + 2c|| over the others; stats[2] = 1 when a centroid with finite
 //   coordinates had to be left out (its 16-bit or squared norm overflows):
 //   then every point is scanned exactly; stats[4] (as uint32) = refinement
-//   queue length, cleared here.
+//   queue length; stats[12..15] = the build's running maxima (FP64 bits of
+//   non-negative values order like the values) and the dropped flag.
+__global__ void __launch_bounds__(64) mfma_prepare_clear_kernel(float *__restrict__ stats)
+{
+    if (threadIdx.x < 16)
+        stats[threadIdx.x] = 0.0f;
+}
+
 template <bool FP16>
 __global__ void __launch_bounds__(256)
 mfma_prepare_kernel(const float *__restrict__ cen, uint32_t d, uint32_t k, uint32_t ntiles,
                     uint32_t nkb, u16x8 *__restrict__ table, float *__restrict__ norm,
-                    float *__restrict__ stats)
+                    float *__restrict__ stats, float *__restrict__ cenT)
 {
-    __shared__ double s_n[256], s_r[256];
-    const uint32_t tid = threadIdx.x;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t entries = (uint64_t)ntiles * nkb * 64;
-    for (uint64_t e = tid; e < entries; e += blockDim.x) {
+    for (uint64_t e = tid; e < entries; e += stride) {
         const uint32_t lane = (uint32_t)(e % 64);
         const uint64_t tk = e / 64;
         const uint32_t kb = (uint32_t)(tk % nkb), t = (uint32_t)(tk / nkb);
@@ -157,9 +166,14 @@ mfma_prepare_kernel(const float *__restrict__ cen, uint32_t d, uint32_t k, uint3
         }
         table[e] = v;
     }
+    const uint32_t P = nkb * kKBlock;
+    for (uint64_t e = tid; e < (uint64_t)ntiles * kTile * P; e += stride) {
+        const uint32_t c = (uint32_t)(e / P), f = (uint32_t)(e % P);
+        cenT[e] = (c < k && f < d) ? cen[(uint64_t)f * k + c] : 0.0f;
+    }
     double nmax = 0.0, rmax = 0.0;
     bool dropped = false;  // a centroid with finite coordinates left out
-    for (uint32_t c = tid; c < ntiles * kTile; c += blockDim.x) {
+    for (uint64_t c = tid; c < (uint64_t)ntiles * kTile; c += stride) {
         double s = 0.0, r = 0.0;
         bool fin = c < k, in_range = true;
         for (uint32_t f = 0; f < d && fin; ++f) {
@@ -178,23 +192,25 @@ mfma_prepare_kernel(const float *__restrict__ cen, uint32_t d, uint32_t k, uint3
             rmax = fmax(rmax, r);
         }
     }
-    s_n[tid] = nmax;
-    s_r[tid] = rmax;
-    dropped = __syncthreads_or(dropped);
-    for (uint32_t s = blockDim.x / 2; s > 0; s /= 2) {
-        if (tid < s) {
-            s_n[tid] = fmax(s_n[tid], s_n[tid + s]);
-            s_r[tid] = fmax(s_r[tid], s_r[tid + s]);
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        const double n = s_n[0] * (1.0 + 1e-12);
+    unsigned long long *acc = reinterpret_cast<unsigned long long *>(stats + 12);
+    if (nmax > 0.0)
+        atomicMax(&acc[0], (unsigned long long)__double_as_longlong(nmax));
+    if (rmax > 0.0)
+        atomicMax(&acc[1], (unsigned long long)__double_as_longlong(rmax));
+    if (dropped)
+        atomicOr(reinterpret_cast<uint32_t *>(stats) + 2, 1u);
+}
+
+__global__ void __launch_bounds__(64) mfma_prepare_finish_kernel(float *__restrict__ stats)
+{
+    if (threadIdx.x == 0) {
+        const unsigned long long *acc = reinterpret_cast<const unsigned long long *>(stats + 12);
+        const double n = __longlong_as_double((long long)acc[0]) * (1.0 + 1e-12);
+        const double r = __longlong_as_double((long long)acc[1]);
         stats[0] = (float)(sqrt(n) * (1.0 + 1e-6));
         stats[1] = (float)(n * (1.0 + 1e-6));
-        stats[2] = dropped ? 1.0f : 0.0f;  // 1: every point takes the exact scan
-        stats[3] = (float)(sqrt(s_r[0]) * (1.0 + 1e-6) + 1e-38);
-        reinterpret_cast<uint32_t *>(stats)[4] = 0u;
+        stats[2] = reinterpret_cast<const uint32_t *>(stats)[2] ? 1.0f : 0.0f;  // 1: exact scan
+        stats[3] = (float)(sqrt(r) * (1.0 + 1e-6) + 1e-38);
     }
 }
 
@@ -301,29 +317,24 @@ __device__ __forceinline__ void merge_pair(uint32_t &b, uint32_t &s, uint32_t b2
 }
 
 // Exact FP32 distance, the reference's fma sequence from 0
-// (lloyd_fused_feature_sum.cl:210-239, kmeans_naive.cpp:49-63).  Features are
-// loaded in chunks of 16 (32 loads in flight) ahead of the sequential fma
-// chain; past d the chunk reads feature 0 and is not used.
-template <int NKB, bool FULL>
-__device__ __forceinline__ float exact_dist(const float *__restrict__ pts, uint64_t n, uint64_t p,
-                                            const float *__restrict__ cen, uint32_t d, uint32_t k,
-                                            uint32_t c)
+// (lloyd_fused_feature_sum.cl:210-239, kmeans_naive.cpp:49-63), from the
+// point's FP32 row staged in LDS and the centroid's row of the transposed
+// copy (both padded to a multiple of 4 features, read as float4).
+__device__ __forceinline__ float exact_dist(const float *xrow, const float *__restrict__ crow,
+                                            uint32_t d)
 {
-    constexpr uint32_t kChunk = 16;
-    const uint32_t dd = FULL ? (uint32_t)(NKB * kKBlock) : d;
     float dist = 0.0f;
-    for (uint32_t f0 = 0; f0 < dd; f0 += kChunk) {
-        float xv[kChunk], cv[kChunk];
+    for (uint32_t f0 = 0; f0 < d; f0 += 16) {
+        f32v4 xv[4], cv[4];
 #pragma unroll
-        for (uint32_t j = 0; j < kChunk; ++j) {
-            const uint32_t f = (FULL || f0 + j < dd) ? f0 + j : 0;
-            xv[j] = pts[(uint64_t)f * n + p];
-            cv[j] = cen[(uint64_t)f * k + c];
+        for (int j = 0; j < 4; ++j) {
+            xv[j] = *reinterpret_cast<const f32v4 *>(xrow + f0 + 4 * j);
+            cv[j] = *reinterpret_cast<const f32v4 *>(crow + f0 + 4 * j);
         }
 #pragma unroll
-        for (uint32_t j = 0; j < kChunk; ++j) {
-            if (FULL || f0 + j < dd) {
-                const float t = xv[j] - cv[j];
+        for (uint32_t j = 0; j < 16; ++j) {
+            if (f0 + j < d) {
+                const float t = xv[j / 4][j % 4] - cv[j / 4][j % 4];
                 dist = __builtin_fmaf(t, t, dist);
             }
         }
@@ -341,7 +352,8 @@ template <bool FP16, int NKB, bool FULL>
 __device__ __forceinline__ void load_points(PointOps<NKB> &x, float (&X2)[kGroups],
                                             float (&RX2)[kGroups],
                                             const float *__restrict__ pts, uint64_t n, uint32_t d,
-                                            const uint64_t (&pid)[kGroups], uint32_t h)
+                                            const uint64_t (&pid)[kGroups], uint32_t q, uint32_t h,
+                                            float *xs = nullptr)
 {
 #pragma unroll
     for (int g = 0; g < kGroups; ++g) {
@@ -363,6 +375,16 @@ __device__ __forceinline__ void load_points(PointOps<NKB> &x, float (&X2)[kGroup
                     raw[kb * 8 + j] = (valid && fv) ? v : 0.0f;
                 }
             }
+        if (xs) {  // row slot 16g + q, features kb*32 + 8h .. +7
+            float *row = xs + (uint32_t)(g * 16 + q) * (NKB * kKBlock) + 8 * h;
+#pragma unroll
+            for (int kb = 0; kb < NKB; ++kb) {
+                *reinterpret_cast<f32v4 *>(row + kb * kKBlock) =
+                    f32v4{raw[kb * 8 + 0], raw[kb * 8 + 1], raw[kb * 8 + 2], raw[kb * 8 + 3]};
+                *reinterpret_cast<f32v4 *>(row + kb * kKBlock + 4) =
+                    f32v4{raw[kb * 8 + 4], raw[kb * 8 + 5], raw[kb * 8 + 6], raw[kb * 8 + 7]};
+            }
+        }
         float s = 0.0f, r = 0.0f;
 #pragma unroll
         for (int kb = 0; kb < NKB; ++kb) {
@@ -519,7 +541,7 @@ label_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
             pid[g] = bt * 64 + g * 16 + q;
         PointOps<NKB> x;
         float X2[kGroups], RX2[kGroups];
-        load_points<FP16, NKB, FULL>(x, X2, RX2, pts, n, d, pid, h);
+        load_points<FP16, NKB, FULL>(x, X2, RX2, pts, n, d, pid, q, h);
         const WaveBounds w = wave_bounds<NKB>(X2, RX2, stats, prm);
         uint32_t best[kGroups], second[kGroups];
         rank_all<FP16, NKB>(lds_table, lds_norm, ntiles, lane, h, w.K, kmask, x, best, second);
@@ -567,40 +589,38 @@ label_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
 }
 
 constexpr uint32_t kPairCap = 1024;  // candidate pairs per wave and flush
+constexpr int kRefineWaves = 4;
 
-// Refinement kernel for the queued points (see the header).  LDS: the table
-// and norms, then per wave a pair list, the point ids and the running
-// (distance bits << 32 | centroid) minimum of each of its 64 points, and a
-// pair counter.
+// Refinement kernel for the queued points (see the header).  The 16-bit table
+// and norms are read from L2 (tile t+1 prefetched while tile t is scored);
+// LDS holds, per wave, the 64 points' FP32 rows, a pair list, the point ids,
+// the running (distance bits << 32 | centroid) minimum of each point and a
+// pair counter.  cenT = the centroids transposed, [ntiles*16][P] FP32.
 template <bool FP16, int NKB, bool FULL>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * kRefineWaves)
 refine_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
-                   const float *__restrict__ cen, const u16x8 *__restrict__ table,
+                   const float *__restrict__ cenT, const u16x8 *__restrict__ table,
                    const float *__restrict__ norm, const float *__restrict__ stats,
                    uint32_t ntiles, MfmaParams prm, uint32_t *__restrict__ labels,
                    const uint32_t *__restrict__ qidx, const float *__restrict__ qthr,
                    const uint32_t *__restrict__ qlen)
 {
+    constexpr uint32_t P = NKB * kKBlock;
     const uint32_t qn = *qlen;
     const uint64_t batches = ((uint64_t)qn + 63) / 64;
-    if ((uint64_t)blockIdx.x * 4 >= batches)
-        return;  // uniform across the work-group
-    extern __shared__ __align__(16) u16x8 lds_table[];
-    float *lds_norm = reinterpret_cast<float *>(lds_table + (uint64_t)ntiles * NKB * 64);
-    uint32_t *lds_pairs = reinterpret_cast<uint32_t *>(lds_norm + ntiles * kTile);
-    uint64_t *lds_pid = reinterpret_cast<uint64_t *>(lds_pairs + 4 * kPairCap);
-    unsigned long long *lds_min = reinterpret_cast<unsigned long long *>(lds_pid + 4 * 64);
-    uint32_t *lds_count = reinterpret_cast<uint32_t *>(lds_min + 4 * 64);
-    stage_table<NKB>(lds_table, lds_norm, table, norm, ntiles);
     const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    extern __shared__ __align__(16) float lds_x[];
+    float *xs = lds_x + wave * 64 * P;
+    uint32_t *pairs = reinterpret_cast<uint32_t *>(lds_x + kRefineWaves * 64 * P) + wave * kPairCap;
+    unsigned long long *lds_min = reinterpret_cast<unsigned long long *>(
+        reinterpret_cast<uint32_t *>(lds_x + kRefineWaves * 64 * P) + kRefineWaves * kPairCap);
+    unsigned long long *wmin = lds_min + wave * 64;
+    uint32_t *wcount = reinterpret_cast<uint32_t *>(lds_min + kRefineWaves * 64) + wave;
     const uint32_t q = lane % 16, h = lane / 16;
     const uint32_t kmask = (1u << prm.key_bits) - 1u;
-    uint32_t *pairs = lds_pairs + wave * kPairCap;
-    uint64_t *wpid = lds_pid + wave * 64;
-    unsigned long long *wmin = lds_min + wave * 64;
-    uint32_t *wcount = lds_count + wave;
 
-    for (uint64_t bt = (uint64_t)blockIdx.x * 4 + wave; bt < batches; bt += (uint64_t)gridDim.x * 4) {
+    for (uint64_t bt = (uint64_t)blockIdx.x * kRefineWaves + wave; bt < batches;
+         bt += (uint64_t)gridDim.x * kRefineWaves) {
         uint64_t pid[kGroups];
         float thr_in[kGroups];
 #pragma unroll
@@ -612,7 +632,7 @@ refine_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32
         }
         PointOps<NKB> x;
         float X2[kGroups], RX2[kGroups];
-        load_points<FP16, NKB, FULL>(x, X2, RX2, pts, n, d, pid, h);
+        load_points<FP16, NKB, FULL>(x, X2, RX2, pts, n, d, pid, q, h, xs);
         const WaveBounds w = wave_bounds<NKB>(X2, RX2, stats, prm);
         // candidate threshold on this kernel's scores: T + K' + E', rounded up;
         // NaN or infinite -> every centroid is a candidate
@@ -630,8 +650,7 @@ refine_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32
         for (int g = 1; g < kGroups; ++g)
             if ((uint32_t)g == h)
                 mypid = pid[g];
-        wpid[lane] = mypid;  // slot lane = 16 h + q
-        wmin[lane] = ~0ull;
+        wmin[lane] = ~0ull;  // slot lane = 16 h + q
         if (lane == 0)
             *wcount = 0;
         wave_lds_order();
@@ -642,7 +661,7 @@ refine_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32
             for (uint32_t r = lane; r < count; r += 64) {
                 const uint32_t pr = pairs[r];
                 const uint32_t slot = pr >> 20, c = pr & 0xFFFFFu;
-                const float e = exact_dist<NKB, FULL>(pts, n, wpid[slot], cen, d, k, c);
+                const float e = exact_dist(xs + slot * P, cenT + (uint64_t)c * P, d);
                 if (e < FLT_MAX)  // the reference's strict '<' from FLT_MAX
                     atomicMin(&wmin[slot], ((unsigned long long)__float_as_uint(e) << 32) | c);
             }
@@ -651,9 +670,13 @@ refine_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32
                 *wcount = 0;
             wave_lds_order();
         };
+        const uint32_t last = ntiles - 1;
+        TileOps<NKB> nxt = load_ops<NKB>(table, norm, 0, lane, h);
         for (uint32_t t = 0; t < ntiles; ++t) {
+            const TileOps<NKB> cur = nxt;
+            nxt = load_ops<NKB>(table, norm, min(t + 1, last), lane, h);
             f32v4 acc[kGroups];
-            score_tile<FP16, NKB>(acc, load_ops<NKB>(lds_table, lds_norm, t, lane, h), w.K, x);
+            score_tile<FP16, NKB>(acc, cur, w.K, x);
             uint32_t cmask = 0;  // bit 4g + i: (group g's point, centroid t*16 + 4h + i)
 #pragma unroll
             for (int g = 0; g < kGroups; ++g)
@@ -709,7 +732,7 @@ void pick_mfma(uint32_t nkb, LabelKernelM &lk, RefineKernelM &rk)
 
 struct MfmaLayout {
     uint32_t ntiles, nkb;
-    size_t table_bytes, norm_bytes, refine_extra;
+    size_t table_bytes, norm_bytes, cent_bytes, refine_lds;
 };
 
 MfmaLayout mfma_layout(uint64_t d, uint64_t k)
@@ -719,7 +742,9 @@ MfmaLayout mfma_layout(uint64_t d, uint64_t k)
     l.nkb = (uint32_t)div_up(d, (uint64_t)kKBlock);
     l.table_bytes = (size_t)l.ntiles * l.nkb * 64 * sizeof(u16x8);
     l.norm_bytes = (size_t)l.ntiles * kTile * sizeof(float);
-    l.refine_extra = 4 * kPairCap * sizeof(uint32_t) + 4 * 64 * (2 * sizeof(uint64_t)) + 4 * sizeof(uint32_t);
+    l.cent_bytes = (size_t)l.ntiles * kTile * l.nkb * kKBlock * sizeof(float);
+    l.refine_lds = kRefineWaves * (64 * l.nkb * kKBlock * sizeof(float) + kPairCap * sizeof(uint32_t) +
+                                   64 * sizeof(unsigned long long) + sizeof(uint32_t));
     return l;
 }
 
@@ -737,7 +762,7 @@ bool mfma_labeling_supported(const clk_context *ctx, uint64_t d, uint64_t n, uin
     if (d == 0 || d > (uint64_t)kMaxKb * kKBlock || k == 0 || k > (1u << 20) || n >= (1ull << 32))
         return false;
     const MfmaLayout l = mfma_layout(d, k);
-    return l.table_bytes + l.norm_bytes + l.refine_extra <= ctx->lds_per_block;
+    return l.table_bytes + l.norm_bytes <= ctx->lds_per_block && l.refine_lds <= ctx->lds_per_block;
 }
 
 int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, uint64_t k,
@@ -747,23 +772,29 @@ int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, ui
     CLK_REQUIRE(mfma_labeling_supported(ctx, d, n, k), CLK_ERR_UNSUPPORTED,
                 "MFMA labeling: needs d <= 128, n < 2^32 and a 16-bit centroid table that fits in LDS");
     const MfmaLayout l = mfma_layout(d, k);
-    CLK_TRY(ensure_scratch(ctx->mfma_tables, l.table_bytes + l.norm_bytes + 64));  // + stats[16]
+    CLK_TRY(ensure_scratch(ctx->mfma_tables, l.table_bytes + l.norm_bytes + 64 + l.cent_bytes));
     CLK_TRY(ensure_scratch(ctx->mfma_queue, std::max<uint64_t>(n, 1) * 2 * sizeof(uint32_t)));
     u16x8 *table = static_cast<u16x8 *>(ctx->mfma_tables.ptr);
     float *norm = reinterpret_cast<float *>(static_cast<char *>(ctx->mfma_tables.ptr) + l.table_bytes);
-    float *stats = norm + (size_t)l.ntiles * kTile;
+    float *stats = norm + (size_t)l.ntiles * kTile;  // 16 floats
+    float *cenT = stats + 16;
     uint32_t *qlen = reinterpret_cast<uint32_t *>(stats + 4);
     uint32_t *qidx = static_cast<uint32_t *>(ctx->mfma_queue.ptr);
     float *qthr = reinterpret_cast<float *>(qidx + std::max<uint64_t>(n, 1));
     ctx->mfma_qlen = qlen;
     {
         KernelTimer t(ctx, "MfmaPrepare", stream);
+        hipLaunchKernelGGL(mfma_prepare_clear_kernel, dim3(1), dim3(64), 0, stream, stats);
+        const uint64_t work = std::max<uint64_t>((uint64_t)l.ntiles * kTile * l.nkb * kKBlock,
+                                                 (uint64_t)l.ntiles * l.nkb * 64);
+        const unsigned pgrid = (unsigned)std::min<uint64_t>(div_up(work, (uint64_t)256), 1024);
         if (fp16)
-            hipLaunchKernelGGL(mfma_prepare_kernel<true>, dim3(1), dim3(256), 0, stream, centroids,
-                               (uint32_t)d, (uint32_t)k, l.ntiles, l.nkb, table, norm, stats);
+            hipLaunchKernelGGL(mfma_prepare_kernel<true>, dim3(pgrid), dim3(256), 0, stream, centroids,
+                               (uint32_t)d, (uint32_t)k, l.ntiles, l.nkb, table, norm, stats, cenT);
         else
-            hipLaunchKernelGGL(mfma_prepare_kernel<false>, dim3(1), dim3(256), 0, stream, centroids,
-                               (uint32_t)d, (uint32_t)k, l.ntiles, l.nkb, table, norm, stats);
+            hipLaunchKernelGGL(mfma_prepare_kernel<false>, dim3(pgrid), dim3(256), 0, stream, centroids,
+                               (uint32_t)d, (uint32_t)k, l.ntiles, l.nkb, table, norm, stats, cenT);
+        hipLaunchKernelGGL(mfma_prepare_finish_kernel, dim3(1), dim3(64), 0, stream, stats);
         CLK_HIP_CHECK(hipGetLastError());
     }
     if (n == 0)
@@ -798,7 +829,7 @@ int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, ui
     else
         full ? pick_mfma<false, true>(l.nkb, lk, rk) : pick_mfma<false, false>(l.nkb, lk, rk);
     const size_t lds = l.table_bytes + l.norm_bytes;
-    const size_t lds_r = lds + l.refine_extra;
+    const size_t lds_r = l.refine_lds;
     CLK_HIP_CHECK(allow_lds((const void *)lk, lds));
     CLK_HIP_CHECK(allow_lds((const void *)rk, lds_r));
     const int fast_block = 64 * fast_waves((int)l.nkb);
@@ -811,11 +842,12 @@ int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, ui
         CLK_HIP_CHECK(hipGetLastError());
     }
     // a resident grid; blocks past the queue return at once
-    const uint64_t grid_r = auto_grid(ctx, (const void *)rk, 256, lds_r, div_up(n, (uint64_t)64) * 64);
+    const int refine_block = 64 * kRefineWaves;
+    const uint64_t grid_r = auto_grid(ctx, (const void *)rk, refine_block, lds_r, div_up(n, (uint64_t)64) * 64);
     KernelTimer t(ctx, "LabelingMfmaRefine", stream);
-    hipLaunchKernelGGL(rk, dim3((unsigned)grid_r), dim3(256), lds_r, stream, points, n, (uint32_t)d,
-                       (uint32_t)k, centroids, table, norm, stats, l.ntiles, prm, labels, qidx,
-                       qthr, qlen);
+    hipLaunchKernelGGL(rk, dim3((unsigned)grid_r), dim3(refine_block), lds_r, stream, points, n,
+                       (uint32_t)d, (uint32_t)k, cenT, table, norm, stats, l.ntiles, prm, labels,
+                       qidx, qthr, qlen);
     CLK_HIP_CHECK(hipGetLastError());
     return CLK_OK;
 }
