@@ -13,7 +13,11 @@
 //                       smallest LDS footprint, for large k*d
 // Tiles are summed per work-group in FP64 and merged by a fixed-order column
 // reduction, except feature_sum whose in-tile FP32 addition order is the
-// hardware's.  When no tile fits in LDS the FP64 global-atomic pass is used.
+// hardware's.  For large k*d (when neither fits) every strategy uses the
+// cluster-major slice kernel below (per-wave LDS tables [k][4] per feature
+// slice, updated by plain read-modify-writes made collision-free by an owner
+// claim; LDS float atomics run ~9x slower than read-modify-writes on
+// gfx950); the FP64 global-atomic pass remains for k too large for that.
 // `masses` is not read, as in the reference kernels.
 #include "clk_common.hpp"
 #include "clk_kernels.hpp"
@@ -126,6 +130,84 @@ cu_feature_sum_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uin
         part[(uint64_t)blockIdx.x * rows + r] = (double)tile[r];
 }
 
+// Cluster-major feature slices for large k*d.  blockIdx.y selects features
+// [4*blockIdx.y, +4); each wave owns a private FP32 table T[c] (one float4 per
+// cluster) and an owner array O[c] in LDS, so no two waves ever touch one
+// table.  A wave takes 64 consecutive points (lane = point: 4 coalesced
+// feature loads and the label) and adds each point's float4 into T[label]
+// with a plain read-modify-write: lanes write their id to O[label], read it
+// back, and the lanes that find their own id -- one per distinct label --
+// update their row; the others retry (at most the largest label
+// multiplicity in the wave, usually 1-3 rounds).  The wave tables are summed
+// in FP64 into the work-group's partial row.  Work-groups are sized so a
+// table cell takes about 64 points (FP32 partial sums with few roundings).
+constexpr uint32_t kSliceF = 4, kSliceWaves = 4;
+
+size_t slice_lds_bytes(uint64_t k)
+{
+    return (size_t)kSliceWaves * k * (kSliceF * 4 + 4);
+}
+
+__global__ void __launch_bounds__(64 * kSliceWaves)
+cu_slice_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
+                const uint32_t *__restrict__ labels, double *__restrict__ part)
+{
+    extern __shared__ __align__(16) float tab[];  // [waves][k] float4, then [waves][k] owners
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const uint32_t tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+    f4 *T = reinterpret_cast<f4 *>(tab) + (size_t)wave * k;
+    uint32_t *O = reinterpret_cast<uint32_t *>(reinterpret_cast<f4 *>(tab) + (size_t)kSliceWaves * k) +
+                  (size_t)wave * k;
+    for (uint32_t i = tid; i < kSliceWaves * k; i += blockDim.x)
+        reinterpret_cast<f4 *>(tab)[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    __syncthreads();
+    const uint32_t f0 = blockIdx.y * kSliceF;
+    const uint32_t nf = min(kSliceF, d - f0);
+    const uint64_t tiles = (n + 63) / 64;
+    const uint64_t tstride = (uint64_t)gridDim.x * kSliceWaves;
+    // the next tile's features and label are loaded while this one is added
+    auto load = [&](uint64_t tile, f4 &v, uint32_t &L) {
+        const uint64_t p = tile * 64 + lane;
+        const uint64_t pc = p < n ? p : n - 1;
+#pragma unroll
+        for (uint32_t j = 0; j < kSliceF; ++j)
+            v[j] = j < nf ? pts[(uint64_t)(f0 + j) * n + pc] : 0.0f;
+        L = labels[pc];
+    };
+    uint64_t tile = (uint64_t)blockIdx.x * kSliceWaves + wave;
+    f4 vn;
+    uint32_t Ln = 0;
+    if (tile < tiles)
+        load(tile, vn, Ln);
+    for (; tile < tiles; tile += tstride) {
+        const f4 v = vn;
+        const uint32_t L = Ln;
+        if (tile + tstride < tiles)
+            load(tile + tstride, vn, Ln);
+        const bool valid = tile * 64 + lane < n;
+        bool pending = valid && L < k;
+        while (__ballot(pending) != 0) {
+            if (pending)
+                O[L] = lane;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            const bool win = pending && O[L] == lane;
+            if (win)
+                T[L] = T[L] + v;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            pending = pending && !win;
+        }
+    }
+    __syncthreads();
+    const f4 *all = reinterpret_cast<const f4 *>(tab);
+    for (uint32_t i = tid; i < k * nf; i += blockDim.x) {
+        const uint32_t c = i / nf, j = i % nf;
+        double s = 0.0;
+        for (uint32_t w = 0; w < kSliceWaves; ++w)
+            s += (double)all[(size_t)w * k + c][j];
+        part[(uint64_t)blockIdx.x * ((uint64_t)d * k) + (uint64_t)(f0 + j) * k + c] = s;
+    }
+}
+
 using CmKernel = void (*)(const float *, uint64_t, uint32_t, uint32_t, const uint32_t *, double *);
 
 template <int DMAX>
@@ -225,6 +307,28 @@ int run_centroid_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t
         gx = std::max<uint64_t>(1, std::min<uint64_t>(gx, 1u << 20));
         grid = dim3((unsigned)gx);
         name = "CentroidUpdateFeatureSum";
+    }
+    if (!kern && slice_lds_bytes(k) <= ctx->lds_per_block) {
+        // Large k*d: cluster-major feature slices, every strategy.
+        const size_t sl = slice_lds_bytes(k);
+        if (sl > 65536)
+            CLK_HIP_CHECK(hipFuncSetAttribute((const void *)cu_slice_kernel,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl));
+        const uint64_t gy = div_up(d, (uint64_t)kSliceF);
+        // about 64 points per table cell and wave, at least two resident
+        // rounds of work-groups
+        uint64_t gxs = std::max<uint64_t>(div_up(n, 64 * k * kSliceWaves), div_up(4 * (uint64_t)ctx->num_cus, gy));
+        gxs = std::max<uint64_t>(1, std::min<uint64_t>(gxs, div_up(n, 64 * kSliceWaves)));
+        gxs = std::min<uint64_t>(gxs, 65535);
+        CLK_TRY(ensure_scratch(ctx->part_sums, gxs * dk * sizeof(double)));
+        double *ps = static_cast<double *>(ctx->part_sums.ptr);
+        {
+            KernelTimer t(ctx, "CentroidUpdateSlices", stream);
+            hipLaunchKernelGGL(cu_slice_kernel, dim3((unsigned)gxs, (unsigned)gy), dim3(64 * kSliceWaves), sl,
+                               stream, points, n, (uint32_t)d, (uint32_t)k, labels, ps);
+            CLK_HIP_CHECK(hipGetLastError());
+        }
+        return launch_reduce_f64(ctx, ps, gxs, dk, sums, true, stream);
     }
     if (!kern)
         return launch_accumulate_global(ctx, d, n, k, points, labels, sums, nullptr, stream);

@@ -141,7 +141,9 @@ def test_mass_update_exact(gpu_ctx, strategy, n, k, kw):
                                       (4096, 4, 10, {"local_size": 64, "local_features": 4,
                                                      "thread_features": 1, "vector_length": 4}),
                                       (3000, 16, 30, {"local_features": 2, "thread_features": 3}),
-                                      (2000, 64, 300, {}), (1000, 70, 4, {})])
+                                      (2000, 64, 300, {}), (1000, 70, 4, {}),
+                                      (50000, 64, 1024, {}),   # k*d past LDS: feature slices
+                                      (30001, 45, 1000, {})])  # partial last slice
 def test_centroid_update_matches_oracle(gpu_ctx, strategy, n, d, k, kw):
     pts = blobs(n, d, k, seed=11)
     labels = np.random.default_rng(5).integers(0, k, size=n).astype(np.uint32)
@@ -152,6 +154,21 @@ def test_centroid_update_matches_oracle(gpu_ctx, strategy, n, d, k, kw):
     got = ds.to_host() - s0
     ref = O.sums_f64(pts, labels, k)
     assert np.all(np.abs(got - ref) <= sum_tolerance(pts, labels, k, per_item=n))
+
+
+@pytest.mark.parametrize("hot", [1, 3, 64])
+def test_centroid_slices_repeated_labels(gpu_ctx, hot):
+    # Large k*d (feature-slice path) with labels drawn from only `hot`
+    # clusters: every 64-point wave carries many equal labels, so the owner
+    # claim of each slice table resolves them over several rounds.
+    n, d, k = 40000, 33, 1500
+    pts = blobs(n, d, 8, seed=hot)
+    labels = (np.random.default_rng(hot).integers(0, hot, size=n) * 7 % k).astype(np.uint32)
+    ds = zeros((d, k), np.float64)
+    gpu_ctx.centroid_update(gpu_ctx.config(CU, "cluster_merge"), d, n, k, dev(pts), ds, dev(labels))
+    hip.synchronize()
+    ref = O.sums_f64(pts, labels, k)
+    assert np.all(np.abs(ds.to_host() - ref) <= sum_tolerance(pts, labels, k, per_item=n))
 
 
 def test_divide_bit_exact_and_nan(gpu_ctx):
