@@ -10,6 +10,13 @@ Prints one JSON line: iter/s, the fused kernel's achieved HBM bandwidth
 against the 8 TB/s peak (algorithmic bytes N*(4d+4) per launch over its
 HIP-event-timed average duration), and the multi-threaded CPU restatement
 timed on the host cores.
+
+--workload dense: N=2^24 points, d=64, k=1024, blobs sigma=0.5 with centres
+uniform in [-1,1]^64, init = first 1024 points; one step = one three_stage
+iteration with MFMA labeling (mfma_fp16 by default, --labeling), masses,
+feature-slice FP64 centroid sums and the FP64 divide.  The roofline is the
+labeling kernel's x·c^T rate (2*N*k*d flops per launch) against the 2.5
+PFLOP/s dense FP16/BF16 MFMA peak.
 """
 import argparse
 import json
@@ -24,6 +31,7 @@ sys.path.insert(0, os.path.join(REPO, "cl-kmeans_amd"))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+MFMA16_PEAK_TFLOPS = 2500.0  # dense FP16 / BF16
 METRIC = "Lloyd iter/s+%HBM (4f/10c 2GiB); served/batched vs plain solves/s @tol; p99"
 
 
@@ -43,6 +51,8 @@ def main():
     ap.add_argument("--strategy", default="cluster_merge")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="fused", choices=["fused", "dense"])
+    ap.add_argument("--labeling", default="mfma_fp16", help="dense workload: labeling strategy")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scales rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -70,6 +80,9 @@ def main():
         print(json.dumps({"calibration": "hipMemcpy D2D", "bytes_each": nbytes, "ms_each": ms,
                           "read_plus_write_GBps": 2 * nbytes / (ms * 1e-3) / 1e9}))
         return
+
+    if args.workload == "dense":
+        return dense(args)
 
     n, d, k = 1 << args.log2n, args.d, args.k
     threads = min(16, os.cpu_count() or 1)
@@ -190,6 +203,127 @@ def main():
         "cpu_baseline": cpu,
         "wall_s": wall,
         "parity": {"labels_sample": int(idx.size), "labels_equal": parity, "mass_sum_equal_n": mass_ok},
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def dense(args):
+    """The dense-distance configuration (d=64, k=1024) on the MFMA labeling."""
+    import clkmeans
+    from clkmeans import hip
+    from oracle import oracle as O
+
+    log2n = args.log2n if args.log2n != 27 else 24
+    n, d, k = 1 << log2n, 64, 1024
+    steps = args.steps if args.steps != 30 else 20
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.time()
+    pts, _ = clkmeans.generate_blobs(n, d, k, sigma=0.5, box=1.0, seed=0, threads=threads)
+    log("generated %d x %d points in %.1fs" % (n, d, time.time() - t0))
+    init = np.ascontiguousarray(pts[:, :k])
+    ctx = clkmeans.Context(0)
+    stream = hip.Stream()
+    s = stream.handle
+    dp = hip.DeviceArray.from_host(pts)
+    cen = [hip.DeviceArray.from_host(init), hip.DeviceArray((d, k), np.float32)]
+    sums = hip.DeviceArray((d, k), np.float64)
+    masses = hip.DeviceArray((k,), np.uint32)
+    labels = hip.DeviceArray((n,), np.uint32)
+    lcfg = ctx.config(clkmeans.STAGE_LABELING, args.labeling)
+    mcfg = ctx.config(clkmeans.STAGE_MASS_UPDATE, "part_local")
+    ccfg = ctx.config(clkmeans.STAGE_CENTROID_UPDATE, "cluster_merge")
+
+    def iteration(lab_cfg):
+        old, new = cen[0], cen[1]
+        ctx.fill_zero(masses, stream=s)
+        ctx.fill_zero(sums, stream=s)
+        ctx.labeling(lab_cfg, d, n, k, dp, old, labels, stream=s)
+        ctx.mass_update(mcfg, n, k, labels, masses, stream=s)
+        ctx.centroid_update(ccfg, d, n, k, dp, sums, labels, masses, stream=s)
+        ctx.divide(d, k, sums, masses, new, stream=s)
+        cen[0], cen[1] = new, old
+
+    for _ in range(args.warmup):
+        iteration(lcfg)
+    stream.synchronize()
+    ctx.clear_records()
+    ctx.profiling(True)
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for it in range(steps):
+        ctx.set_iteration(it)
+        iteration(lcfg)
+    e1.record(stream)
+    e1.synchronize()
+    ctx.profiling(False)
+    ms_per_step = e0.elapsed_ms(e1) / steps
+    by_name = {}
+    for name, _, ms in ctx.records():
+        by_name.setdefault(name, []).append(ms)
+    log("kernels (avg ms): " + ", ".join("%s=%.4f x%d" % (kk, np.mean(v), len(v)) for kk, v in by_name.items()))
+    hot = "LabelingMfmaF16" if args.labeling == "mfma_fp16" else \
+          "LabelingMfmaBF16" if args.labeling == "mfma_bf16" else "LabelingGlobal"
+    hot_ms = float(np.mean(by_name[hot])) if hot in by_name else float("nan")
+    flops = 2.0 * n * k * d
+    achieved = flops / (hot_ms * 1e-3) / 1e12
+    refined = ctx.labeling_refined_points() if args.labeling.startswith("mfma") else None
+
+    # parity spot check of one more labeling against the oracle
+    c_in = cen[0].to_host()
+    ctx.labeling(lcfg, d, n, k, dp, cen[0], labels, stream=s)
+    stream.synchronize()
+    lab = labels.to_host()
+    idx = np.sort(np.random.default_rng(0).choice(n, size=4096, replace=False))
+    parity = bool(np.array_equal(lab[idx], O.label(np.ascontiguousarray(pts[:, idx]), c_in)))
+
+    # the plain FP32 labeling (unroll_vector) for comparison, one call
+    plain_ms = None
+    if not args.no_cpu:
+        pcfg = ctx.config(clkmeans.STAGE_LABELING, "unroll_vector")
+        ctx.clear_records()
+        ctx.profiling(True)
+        ctx.labeling(pcfg, d, n, k, dp, cen[0], labels, stream=s)
+        stream.synchronize()
+        ctx.profiling(False)
+        plain_ms = float(sum(ms for _, _, ms in ctx.records()))
+        plain_equal = bool(np.array_equal(labels.to_host(), lab))
+    cpu = None
+    if not args.no_cpu:
+        ns = 1 << 16
+        t = time.perf_counter()
+        O.lloyd_mt(np.ascontiguousarray(pts[:, :ns]), init, 1, threads)
+        dt = (time.perf_counter() - t) * (n / ns)
+        cpu = {"value": 1.0 / dt, "unit": "iter/s", "cores": threads, "kind": "port",
+               "sample": "one fused cluster_merge iteration (oracle/kmeans_oracle.c ora_lloyd_mt_f32) over "
+                         "the first 2^16 of the 2^%d points against the same 1024 centroids, %d threads, "
+                         "scaled by N/2^16" % (log2n, threads)}
+    out = {
+        "metric": "Lloyd iter/s, dense distances (d=64, k=1024), MFMA labeling with exact refinement",
+        "value": 1000.0 / ms_per_step,
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16-mfma+f32-exact" if args.labeling == "mfma_fp16" else
+                 "bf16-mfma+f32-exact" if args.labeling == "mfma_bf16" else "f32",
+        "data": "synthetic",
+        "config": {"workload": "dense Lloyd, N=2^%d d=64 k=1024 float32 feature-major, sigma=0.5 blobs in "
+                               "[-1,1]^64, seed 0, first-x init; three_stage: %s labeling, part_local masses, "
+                               "feature-slice FP64 sums, FP64 divide" % (log2n, args.labeling),
+                   "n": n, "d": d, "k": k},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / MFMA16_PEAK_TFLOPS, "traffic": None, "kernel": hot,
+                     "kernel_ms": hot_ms, "flops_per_launch": flops},
+        "cpu_baseline": cpu,
+        "refined_points_last_call": refined,
+        "plain_fp32_labeling_ms": plain_ms,
+        "parity": {"labels_sample": int(idx.size), "labels_equal": parity,
+                   "plain_fp32_labels_equal": plain_equal if plain_ms is not None else None},
     }
     print(json.dumps(out), flush=True)
     ctx.close()

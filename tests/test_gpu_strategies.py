@@ -42,6 +42,8 @@ CASES = [
     ("three_stage", {"mass_update": {"strategy": "part_local"}, "centroid_update": {"strategy": "feature_sum"}}),
     ("single_stage_buffered", {"fused": {"strategy": "cluster_merge"}}),
     ("three_stage_buffered", {"mass_update": {"strategy": "part_global"}}),
+    ("three_stage", {"labeling": {"strategy": "mfma_fp16"}}),
+    ("three_stage_buffered", {"labeling": {"strategy": "mfma_bf16"}, "mass_update": {"strategy": "part_local"}}),
 ]
 
 
@@ -55,6 +57,26 @@ def test_strategy_matches_oracle(data, pipeline, stages):
     np.testing.assert_array_equal(m, rm)
     ref64 = O.sums_f64(pts, rl, 10) / rm.astype(np.float64)
     assert centroid_rel_err(c, ref64, pts, rl) <= 1e-5
+
+
+@pytest.mark.parametrize("labeling", ["mfma_fp16", "mfma_bf16"])
+def test_dense_three_stage_matches_oracle(labeling):
+    # d=64, k=1024: MFMA labeling with refinement and the feature-slice
+    # centroid update, three iterations against the oracle's FP64-merge loop.
+    n, d, k = 1 << 14, 64, 1024
+    pts, _ = clkmeans.generate_blobs(n, d, k, sigma=0.5, box=1.0, seed=3)
+    init = O.first_x(pts, k)
+    rc, rl, rm = O.lloyd(pts, init, 3)
+    text = conf("three_stage", iterations=3, k=k, labeling={"strategy": labeling},
+                centroid_update={"strategy": "cluster_merge"})
+    c, l, m, it, _ = clkmeans.kmeans_run(text, pts, init)
+    assert it == 3
+    np.testing.assert_array_equal(l, rl)
+    np.testing.assert_array_equal(m, rm)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ref64 = O.sums_f64(pts, rl, k) / rm.astype(np.float64)
+    assert centroid_rel_err(c, ref64, pts, rl) <= 1e-5
+    np.testing.assert_array_equal(np.isnan(c).all(axis=0), rm == 0)  # empty clusters: NaN
 
 
 def test_convergence_tolerance_stops_early(data):
