@@ -11,6 +11,13 @@ against the 8 TB/s peak (algorithmic bytes N*(4d+4) per launch over its
 HIP-event-timed average duration), and the multi-threaded CPU restatement
 timed on the host cores.
 
+--workload batch: 4096 independent jobs, N log-uniform in [2^10, 2^18], d
+uniform in {3, 8, 16, 32}, k log-uniform in [4, 256] capped at N/16, blobs
+sigma=1 with centres uniform in [-10,10]^d (seed per job), first-x init, 25
+iterations each; value = solves/s of the batch engine (one persistent launch
+per iteration over all jobs, replayed from a HIP graph) next to the serial
+per-job loop (clk_lloyd_step per job and iteration) on the same GPU.
+
 --workload dense: N=2^24 points, d=64, k=1024, blobs sigma=0.5 with centres
 uniform in [-1,1]^64, init = first 1024 points; one step = one three_stage
 iteration with MFMA labeling (mfma_fp16 by default, --labeling), masses,
@@ -51,7 +58,10 @@ def main():
     ap.add_argument("--strategy", default="cluster_merge")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="fused", choices=["fused", "dense"])
+    ap.add_argument("--workload", default="fused", choices=["fused", "dense", "batch"])
+    ap.add_argument("--jobs", type=int, default=4096, help="batch workload: number of jobs")
+    ap.add_argument("--batch-labeling", default="mfma_bf16",
+                    help="batch workload: labeling strategy of the batch engine")
     ap.add_argument("--labeling", default="mfma_fp16", help="dense workload: labeling strategy")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scales rocprofv3 byte counters)")
@@ -83,6 +93,8 @@ def main():
 
     if args.workload == "dense":
         return dense(args)
+    if args.workload == "batch":
+        return batch(args)
 
     n, d, k = 1 << args.log2n, args.d, args.k
     threads = min(16, os.cpu_count() or 1)
@@ -324,6 +336,133 @@ def dense(args):
         "plain_fp32_labeling_ms": plain_ms,
         "parity": {"labels_sample": int(idx.size), "labels_equal": parity,
                    "plain_fp32_labels_equal": plain_equal if plain_ms is not None else None},
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def batch(args):
+    """Many uneven jobs: the batch engine against the serial per-job loop."""
+    import clkmeans
+    from clkmeans import hip
+    from oracle import oracle as O
+
+    iters = 25
+    mix = clkmeans.job_mix(args.jobs, seed=0)
+    threads = min(16, os.cpu_count() or 1)
+    ctx = clkmeans.Context(0)
+    stream = hip.Stream()
+    s = stream.handle
+    probe = clkmeans.Batch(ctx, 1, 1 << 20, 32, 256)
+    need = sum(probe.job_bytes(n, d, k) for n, d, k, _ in mix)
+    probe.close()
+    arena = int(need * 1.02) + (64 << 20)
+    eng = clkmeans.Batch(ctx, len(mix), arena, 32, 256, labeling=args.batch_labeling)
+    ser = clkmeans.Batch(ctx, len(mix), arena, 32, 256)
+    t0 = time.time()
+    slots = []
+    checks = {}
+    small = [j for j, (n, d, k, _) in enumerate(mix) if n * k * d <= 2e7]
+    check_ids = set(np.random.default_rng(1).choice(small, size=min(12, len(small)), replace=False).tolist())
+    for j, (n, d, k, seed) in enumerate(mix):
+        pts, _ = clkmeans.generate_blobs(n, d, k, sigma=1.0, box=10.0, seed=seed, threads=threads)
+        init = np.ascontiguousarray(pts[:, :k])
+        slots.append((eng.add(pts, init, iters, stream=s), ser.add(pts, init, iters, stream=s)))
+        if j in check_ids:
+            checks[j] = (pts, init)
+    log("generated and loaded %d jobs (%.2f GB of points) in %.1fs" %
+        (len(mix), sum(n * d * 4 for n, d, _, _ in mix) / 1e9, time.time() - t0))
+    points = sum(n for n, _, _, _ in mix)
+    bytes_per_iter = sum(n * (4 * d + 4) for n, d, _, _ in mix)
+
+    # batch engine: graph capture on the first step, then 25 replays timed
+    e0, e1 = hip.Event(), hip.Event()
+    stream.synchronize()
+    ctx.clear_records()
+    ctx.profiling(True)
+    e0.record(stream)
+    for _ in range(iters):
+        eng.step(s)
+    e1.record(stream)
+    e1.synchronize()
+    ctx.profiling(False)
+    batch_ms = e0.elapsed_ms(e1)
+    step_ms = [ms for name, _, ms in ctx.records() if name == "BatchStep"]
+    states, active = eng.poll(s)
+    # serial per-job loop on the same GPU
+    stream.synchronize()
+    w0 = time.perf_counter()
+    ser.run_serial(s)
+    stream.synchronize()
+    serial_ms = (time.perf_counter() - w0) * 1e3
+
+    # parity: every job batch == serial (labels, masses exact; centroids), a
+    # sample against the oracle
+    lab_equal, mass_equal, worst = 0, 0, 0.0
+    oracle_ok = True
+    for j, (a, b_) in enumerate(slots):
+        ca, la, ma, ita, _ = eng.take(a, s)
+        cb, lb, mb, itb, _ = ser.take(b_, s)
+        lab_equal += int(bool(np.array_equal(la, lb)) and ita == itb == iters)
+        mass_equal += int(np.array_equal(ma, mb))
+        live = mb > 0
+        if live.any():
+            worst = max(worst, float(np.max(np.abs(ca[:, live].astype(np.float64) - cb[:, live]) /
+                                            np.maximum(np.abs(cb[:, live]), 1e-3))))
+        if j in checks:
+            pts, init = checks[j]
+            rc, rl, rm = O.lloyd(pts, init, iters)
+            oracle_ok &= bool(np.array_equal(la, rl)) and bool(np.array_equal(ma, rm))
+    eng.close()
+    ser.close()
+
+    cpu = None
+    if not args.no_cpu:
+        # oracle (multi-threaded fused restatement) on the first 64 jobs,
+        # scaled by total work n*k*d
+        sub = mix[:64]
+        t = time.perf_counter()
+        for n, d, k, seed in sub:
+            pts, _ = clkmeans.generate_blobs(n, d, k, sigma=1.0, box=10.0, seed=seed, threads=threads)
+            O.lloyd_mt(pts, np.ascontiguousarray(pts[:, :k]), iters, threads)
+        dt = time.perf_counter() - t
+        w_sub = sum(n * k * d for n, d, k, _ in sub)
+        w_all = sum(n * k * d for n, d, k, _ in mix)
+        cpu = {"value": len(mix) / (dt * w_all / w_sub), "unit": "solves/s", "cores": threads, "kind": "port",
+               "sample": "the first 64 jobs, 25 iterations each, fused cluster_merge restatement "
+                         "(oracle/kmeans_oracle.c ora_lloyd_mt_f32, %d threads, includes data generation), "
+                         "scaled by total n*k*d work" % threads}
+    achieved = bytes_per_iter / (np.mean(step_ms) * 1e-3) / 1e9 if step_ms else None
+    out = {
+        "metric": "batched vs plain per-job solves/s (4096 uneven jobs, 25 iterations)",
+        "value": len(mix) / (batch_ms * 1e-3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": iters,
+        "warmup": 0,
+        "ms_per_step": batch_ms / iters,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.batch_labeling == "unroll_vector" else
+                 args.batch_labeling.split("_")[1] + "-mfma+f32-exact",
+        "data": "synthetic",
+        "config": {"workload": "%d jobs: N log-uniform [2^10,2^18], d in {3,8,16,32}, k log-uniform [4,256] "
+                               "capped at N/16, sigma=1 blobs in [-10,10]^d, per-job seeds, first-x init, "
+                               "%d iterations; batch engine (one persistent launch per iteration, HIP graph), "
+                               "%s labeling" % (len(mix), iters, args.batch_labeling),
+                   "jobs": len(mix), "points": points, "labeling": args.batch_labeling},
+        "serial_per_job_loop": {"solves_per_s": len(mix) / (serial_ms * 1e-3), "total_ms": serial_ms},
+        "speedup_vs_serial": serial_ms / batch_ms,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "kernel": "batch_step_kernel", "bytes_per_launch": bytes_per_iter,
+                     "kernel_ms": float(np.mean(step_ms)) if step_ms else None},
+        "cpu_baseline": cpu,
+        "parity": {"jobs_labels_equal_serial": lab_equal, "jobs_masses_equal_serial": mass_equal,
+                   "max_centroid_rel_diff_vs_serial": worst, "oracle_sample_jobs": len(checks),
+                   "oracle_sample_equal": oracle_ok, "jobs_done": int((states == clkmeans.JOB_DONE).sum()),
+                   "still_active": int(active)},
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -55,7 +55,22 @@ EXPORTS = [
     "clk_matrix_add_u32", "clk_matrix_add_f64", "clk_reduce_vector_parcol_u32",
     "clk_reduce_vector_parcol_f64", "clk_fill_zero", "clk_init_first_x", "clk_generate_blobs",
     "clk_kmeans_run", "clk_conf_describe", "clk_lloyd_step", "clk_labeling_refined_points",
+    "clk_batch_job_bytes", "clk_batch_create", "clk_batch_destroy", "clk_batch_add", "clk_batch_step",
+    "clk_batch_poll", "clk_batch_take", "clk_batch_job_buffers", "clk_batch_autotune", "clk_batch_run_serial",
 ]
+
+JOB_FREE, JOB_ACTIVE, JOB_DONE = 0, 1, 2
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [
+        ("max_jobs", ctypes.c_uint32),
+        ("arena_bytes", ctypes.c_uint64),
+        ("tile_work", ctypes.c_uint32),
+        ("max_d", ctypes.c_uint32),
+        ("max_k", ctypes.c_uint32),
+        ("labeling", ctypes.c_int32),
+    ]
 
 
 def load(path=None):
@@ -99,6 +114,19 @@ def load(path=None):
                                  ctypes.POINTER(u64), ctypes.POINTER(u64)]),
         "clk_conf_describe": (i32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
         "clk_labeling_refined_points": (i32, [vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "clk_batch_job_bytes": (u64, [ctypes.POINTER(BatchConfig), u64, ctypes.c_uint32, ctypes.c_uint32]),
+        "clk_batch_create": (i32, [vp, ctypes.POINTER(BatchConfig), ctypes.POINTER(vp)]),
+        "clk_batch_destroy": (i32, [vp]),
+        "clk_batch_add": (i32, [vp, u64, ctypes.c_uint32, ctypes.c_uint32, vp, vp, ctypes.c_uint32, dp,
+                                ctypes.POINTER(ctypes.c_int32), vp]),
+        "clk_batch_step": (i32, [vp, vp]),
+        "clk_batch_poll": (i32, [vp, vp, ctypes.POINTER(ctypes.c_uint32), vp]),
+        "clk_batch_take": (i32, [vp, ctypes.c_int32, vp, vp, vp, ctypes.POINTER(ctypes.c_uint32),
+                                 ctypes.POINTER(ctypes.c_double), vp]),
+        "clk_batch_job_buffers": (i32, [vp, ctypes.c_int32, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                        ctypes.POINTER(vp)]),
+        "clk_batch_autotune": (i32, [vp, ctypes.POINTER(ctypes.c_uint32), vp]),
+        "clk_batch_run_serial": (i32, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -285,3 +313,93 @@ class Context:
 
     def clear_records(self):
         check(self.L.clk_profiling_clear(self.handle))
+
+
+class Batch:
+    """Many independent k-means jobs resident on the GPU (clkmeans_batch.h)."""
+
+    def __init__(self, ctx, max_jobs, arena_bytes, max_d, max_k, tile_work=0, labeling="unroll_vector"):
+        self.L = load()
+        self.ctx = ctx
+        lid = strategy_id(STAGE_LABELING, labeling) if isinstance(labeling, str) else int(labeling)
+        self.cfg = BatchConfig(int(max_jobs), int(arena_bytes), int(tile_work), int(max_d), int(max_k), lid)
+        self.handle = ctypes.c_void_p()
+        check(self.L.clk_batch_create(ctx.handle, ctypes.byref(self.cfg), ctypes.byref(self.handle)))
+        self.shapes = {}
+
+    def job_bytes(self, n, d, k):
+        return self.L.clk_batch_job_bytes(ctypes.byref(self.cfg), int(n), int(d), int(k))
+
+    def add(self, points, centroids, max_iterations, tolerance=0.0, stream=None):
+        """Admit a job (host float32 [d][n] points, [d][k] initial centroids);
+        returns its slot.  Raises ClkError(CLK_ERR_NOMEM) when it does not fit."""
+        pts = np.ascontiguousarray(points, dtype=np.float32)
+        cen = np.ascontiguousarray(centroids, dtype=np.float32)
+        d, n = pts.shape
+        k = cen.shape[1]
+        slot = ctypes.c_int32(-1)
+        check(self.L.clk_batch_add(self.handle, n, d, k, pts.ctypes.data, cen.ctypes.data, int(max_iterations),
+                                   float(tolerance), ctypes.byref(slot), _s(stream)))
+        self.shapes[slot.value] = (n, d, k)
+        return slot.value
+
+    def step(self, stream=None):
+        check(self.L.clk_batch_step(self.handle, _s(stream)))
+
+    def poll(self, stream=None):
+        states = np.zeros(self.cfg.max_jobs, np.uint32)
+        active = ctypes.c_uint32()
+        check(self.L.clk_batch_poll(self.handle, states.ctypes.data, ctypes.byref(active), _s(stream)))
+        return states, active.value
+
+    def take(self, slot, stream=None):
+        """(centroids [d][k], labels [n], masses [k], iterations, shift); frees the slot."""
+        n, d, k = self.shapes.pop(slot)
+        cen = np.empty((d, k), np.float32)
+        lab = np.empty(n, np.uint32)
+        mas = np.empty(k, np.uint32)
+        it = ctypes.c_uint32()
+        sh = ctypes.c_double()
+        check(self.L.clk_batch_take(self.handle, slot, cen.ctypes.data, lab.ctypes.data, mas.ctypes.data,
+                                    ctypes.byref(it), ctypes.byref(sh), _s(stream)))
+        return cen, lab, mas, it.value, sh.value
+
+    def job_buffers(self, slot):
+        p, c, l = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        check(self.L.clk_batch_job_buffers(self.handle, slot, ctypes.byref(p), ctypes.byref(c), ctypes.byref(l)))
+        return p.value, c.value, l.value
+
+    def run_serial(self, stream=None):
+        """Every active job alone through clk_lloyd_step, one after another."""
+        check(self.L.clk_batch_run_serial(self.handle, _s(stream)))
+
+    def autotune(self, stream=None):
+        chosen = ctypes.c_uint32()
+        check(self.L.clk_batch_autotune(self.handle, ctypes.byref(chosen), _s(stream)))
+        return chosen.value
+
+    def close(self):
+        if self.handle:
+            self.L.clk_batch_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def job_mix(count, seed=0, log2n=(10, 18), ds=(3, 8, 16, 32), k_range=(4, 256), k_cap_div=16):
+    """The batched-instances mix: N log-uniform in [2^10, 2^18], d uniform over
+    {3, 8, 16, 32}, k log-uniform in [4, 256] capped at N/16; per-job seeds.
+    Returns a list of (n, d, k, seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for j in range(count):
+        n = int(round(2.0 ** rng.uniform(*log2n)))
+        d = int(rng.choice(ds))
+        k = int(round(np.exp(rng.uniform(np.log(k_range[0]), np.log(k_range[1])))))
+        k = max(1, min(k, n // k_cap_div))
+        out.append((n, d, k, int(seed) * 1000003 + j))
+    return out

@@ -41,6 +41,9 @@ int launch_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, 
 // FP32 relabel of near ties: the same labels as launch_labeling.  Supported
 // for d <= 128 and n < 2^32 when the 16-bit centroid table fits in LDS.
 bool mfma_labeling_supported(const clk_context *ctx, uint64_t d, uint64_t n, uint64_t k);
+// Once per context: whether the matrix cores keep FP16 (bit 0) / BF16 (bit 1)
+// subnormal inputs, into ctx->mfma_subnormals (synchronizes the stream).
+int mfma_subnormal_flags(clk_context *ctx, hipStream_t stream);
 int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, uint64_t k,
                          const float *points, const float *centroids, uint32_t *labels,
                          hipStream_t stream);

@@ -45,6 +45,7 @@
 // every centroid is a candidate.
 #include "clk_common.hpp"
 #include "clk_kernels.hpp"
+#include "clk_mfma_device.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -52,54 +53,7 @@
 namespace clk {
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef float f32v4 __attribute__((ext_vector_type(4)));
-typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
-
-constexpr int kTile = 16;    // centroids per MFMA row tile
-constexpr int kKBlock = 32;  // features per MFMA K block
-constexpr int kGroups = 4;   // 16-point column groups per wave (64 points)
-constexpr int kMaxKb = 4;    // d <= 128
-constexpr float kU = 5.9604645e-8f;  // 2^-24
-
-struct MfmaParams {
-    float mu;      // absolute input error bound of a flushed subnormal
-    float gamma;   // MFMA summation bound factor
-    float trunc;   // relative truncation of the key, 2^(KB-23)
-    float eps;     // exact-distance relative error bound
-    float sqrt_d;  // sqrt(d), rounded up
-    uint32_t key_bits;
-};
-
-template <bool FP16>
-__device__ __forceinline__ uint16_t to16(float x)
-{
-    if constexpr (FP16)
-        return __builtin_bit_cast(uint16_t, (_Float16)x);
-    else
-        return __builtin_bit_cast(uint16_t, (__bf16)x);
-}
-
-template <bool FP16>
-__device__ __forceinline__ float from16(uint16_t v)
-{
-    if constexpr (FP16)
-        return (float)__builtin_bit_cast(_Float16, v);
-    else
-        return __uint_as_float((uint32_t)v << 16);
-}
-
-template <bool FP16>
-__device__ __forceinline__ f32v4 mfma(u16x8 a, u16x8 b, f32v4 c)
-{
-    if constexpr (FP16)
-        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a),
-                                                      __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
-    else
-        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
-                                                       __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
-}
+using namespace mfma;
 
 // Whether the matrix cores keep 16-bit subnormal inputs: one wave multiplies
 // 2^-20 (FP16 subnormal) and 2^-130 (BF16 subnormal) by 1.0 over K = 32;
@@ -115,8 +69,8 @@ __global__ void __launch_bounds__(64) mfma_subnormal_check_kernel(float *out)
         one_bf[j] = to16<false>(1.0f);
     }
     const f32v4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-    const f32v4 r16 = mfma<true>(a16, one16, z);
-    const f32v4 rbf = mfma<false>(a_bf, one_bf, z);
+    const f32v4 r16 = mma16<true>(a16, one16, z);
+    const f32v4 rbf = mma16<false>(a_bf, one_bf, z);
     if (threadIdx.x == 0) {
         out[0] = r16[0] == 0x1p-15f ? 1.0f : 0.0f;
         out[1] = rbf[0] == 0x1p-125f ? 1.0f : 0.0f;
@@ -215,297 +169,6 @@ __global__ void __launch_bounds__(64) mfma_prepare_finish_kernel(float *__restri
 }
 
 
-__device__ __forceinline__ uint32_t min3_u32(uint32_t a, uint32_t b, uint32_t c)
-{
-    return min(a, min(b, c));  // v_min3_u32
-}
-
-// Middle value of three (v_med3_u32).  The operands are VALU results, never
-// MFMA results, so the asm needs no MFMA-read wait states.
-__device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c)
-{
-    uint32_t r;
-    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return r;
-}
-
-// Two new keys a, b into (best, second), best <= second: the two smallest of
-// {best, second, a, b} are min3(best, a, b) and min(second, med3(best, a, b)).
-__device__ __forceinline__ void push2(uint32_t &best, uint32_t &second, uint32_t a, uint32_t b)
-{
-    const uint32_t m = med3_u32(best, a, b);
-    best = min3_u32(best, a, b);
-    second = min(second, m);
-}
-
-// Hide known bits of a value from the optimizer (emits no instruction), so
-// (score & mask) | index stays one v_and_or_b32.
-__device__ __forceinline__ uint32_t opaque(uint32_t v)
-{
-    asm volatile("" : "+v"(v));
-    return v;
-}
-
-// A operands (one per 32-feature block) and norms of centroid tile t, read
-// from LDS ahead of use.
-template <int NKB>
-struct TileOps {
-    u16x8 a[NKB];
-    f32v4 nrm;
-};
-
-template <int NKB>
-__device__ __forceinline__ TileOps<NKB> load_ops(const u16x8 *lds_table, const float *lds_norm,
-                                                 uint32_t t, uint32_t lane, uint32_t h)
-{
-    TileOps<NKB> o;
-#pragma unroll
-    for (int kb = 0; kb < NKB; ++kb)
-        o.a[kb] = lds_table[((uint64_t)t * NKB + kb) * 64 + lane];
-    o.nrm = *reinterpret_cast<const f32v4 *>(lds_norm + t * kTile + 4 * h);
-    return o;
-}
-
-// The wave's 64 points as 16-bit MFMA B operands: lane (q, h) holds, for
-// group g, features kb*32 + 8h + j of that group's point.
-template <int NKB>
-struct PointOps {
-    u16x8 b[kGroups][NKB];
-};
-
-// Scores of one centroid tile for the wave's 4 point groups: lane (q, h)
-// gets rows 4h..4h+3 (centroids t*16 + 4h + i) of point column q.
-template <bool FP16, int NKB>
-__device__ __forceinline__ void score_tile(f32v4 (&acc)[kGroups], const TileOps<NKB> &o, float K,
-                                           const PointOps<NKB> &x)
-{
-#pragma unroll
-    for (int g = 0; g < kGroups; ++g)
-        acc[g] = o.nrm + K;
-#pragma unroll
-    for (int kb = 0; kb < NKB; ++kb)
-#pragma unroll
-        for (int g = 0; g < kGroups; ++g)
-            acc[g] = mfma<FP16>(o.a[kb], x.b[g][kb], acc[g]);
-}
-
-// Keys of one scored tile into the running (best, second) of each group.
-__device__ __forceinline__ void rank_tile(const f32v4 (&acc)[kGroups], uint32_t c0, uint32_t kmask,
-                                          uint32_t (&best)[kGroups], uint32_t (&second)[kGroups])
-{
-    uint32_t idx[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-        idx[i] = opaque(c0 + i);
-#pragma unroll
-    for (int g = 0; g < kGroups; ++g) {
-        uint32_t key[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            key[i] = (__float_as_uint(acc[g][i]) & ~kmask) | idx[i];
-        push2(best[g], second[g], key[0], key[1]);
-        push2(best[g], second[g], key[2], key[3]);
-    }
-}
-
-// Merge two (best, second) key pairs into the two smallest of the four.
-__device__ __forceinline__ void merge_pair(uint32_t &b, uint32_t &s, uint32_t b2, uint32_t s2)
-{
-    const uint32_t nb = min(b, b2);
-    s = min(max(b, b2), min(s, s2));
-    b = nb;
-}
-
-// Exact FP32 distance, the reference's fma sequence from 0
-// (lloyd_fused_feature_sum.cl:210-239, kmeans_naive.cpp:49-63), from the
-// point's FP32 row staged in LDS and the centroid's row of the transposed
-// copy (both padded to a multiple of 4 features, read as float4).
-__device__ __forceinline__ float exact_dist(const float *xrow, const float *__restrict__ crow,
-                                            uint32_t d)
-{
-    float dist = 0.0f;
-    for (uint32_t f0 = 0; f0 < d; f0 += 16) {
-        f32v4 xv[4], cv[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            xv[j] = *reinterpret_cast<const f32v4 *>(xrow + f0 + 4 * j);
-            cv[j] = *reinterpret_cast<const f32v4 *>(crow + f0 + 4 * j);
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < 16; ++j) {
-            if (f0 + j < d) {
-                const float t = xv[j / 4][j % 4] - cv[j / 4][j % 4];
-                dist = __builtin_fmaf(t, t, dist);
-            }
-        }
-    }
-    return dist;
-}
-
-// Load the wave's 64 points and build their operands: lane (q, h) handles,
-// for group g, point pid[g] (>= n: none; zeros), features kb*32 + 8h + j
-// (0 past d).  Every load address is in bounds (a missing point reads point
-// n-1, a missing feature reads feature 0; the value is then replaced by 0).
-// X2[g] >= ||x||^2 and RX2[g] >= ||x̃ - x||^2 (FP32 sums over the 4 lanes of
-// a point: at most P/4 + 2 roundings, plus underflow; x - x̃ is exact).
-template <bool FP16, int NKB, bool FULL>
-__device__ __forceinline__ void load_points(PointOps<NKB> &x, float (&X2)[kGroups],
-                                            float (&RX2)[kGroups],
-                                            const float *__restrict__ pts, uint64_t n, uint32_t d,
-                                            const uint64_t (&pid)[kGroups], uint32_t q, uint32_t h,
-                                            float *xs = nullptr)
-{
-#pragma unroll
-    for (int g = 0; g < kGroups; ++g) {
-        const bool valid = pid[g] < n;
-        const uint64_t pc = valid ? pid[g] : n - 1;
-        const float *row = pts + pc + (uint64_t)(8 * h) * n;  // feature 8h
-        float raw[NKB * 8];
-#pragma unroll
-        for (int kb = 0; kb < NKB; ++kb)
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint64_t uoff = (uint64_t)(kb * kKBlock + j) * n;
-                if constexpr (FULL) {
-                    const float v = row[uoff];
-                    raw[kb * 8 + j] = valid ? v : 0.0f;
-                } else {
-                    const bool fv = (uint32_t)(kb * kKBlock + 8 * h + j) < d;
-                    const float v = *(fv ? row + uoff : pts + pc);
-                    raw[kb * 8 + j] = (valid && fv) ? v : 0.0f;
-                }
-            }
-        if (xs) {  // row slot 16g + q, features kb*32 + 8h .. +7
-            float *row = xs + (uint32_t)(g * 16 + q) * (NKB * kKBlock) + 8 * h;
-#pragma unroll
-            for (int kb = 0; kb < NKB; ++kb) {
-                *reinterpret_cast<f32v4 *>(row + kb * kKBlock) =
-                    f32v4{raw[kb * 8 + 0], raw[kb * 8 + 1], raw[kb * 8 + 2], raw[kb * 8 + 3]};
-                *reinterpret_cast<f32v4 *>(row + kb * kKBlock + 4) =
-                    f32v4{raw[kb * 8 + 4], raw[kb * 8 + 5], raw[kb * 8 + 6], raw[kb * 8 + 7]};
-            }
-        }
-        float s = 0.0f, r = 0.0f;
-#pragma unroll
-        for (int kb = 0; kb < NKB; ++kb) {
-            u16x8 vb;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const float v = raw[kb * 8 + j];
-                s = __builtin_fmaf(v, v, s);
-                vb[j] = to16<FP16>(v);
-                const float res = v - from16<FP16>(vb[j]);
-                r = __builtin_fmaf(res, res, r);
-            }
-            x.b[g][kb] = vb;
-        }
-        s += __shfl_xor(s, 16);
-        s += __shfl_xor(s, 32);
-        r += __shfl_xor(r, 16);
-        r += __shfl_xor(r, 32);
-        X2[g] = s * (1.0f + (float)(2 * NKB * kKBlock + 8) * kU) + 1e-37f;
-        RX2[g] = r * (1.0f + (float)(2 * NKB * kKBlock + 8) * kU) + 1e-37f;
-    }
-}
-
-// Everything the margin test needs about one wave's points.
-struct WaveBounds {
-    float K;               // shared score offset
-    float beta;            // K-proportional part of the score error
-    float E0[kGroups];     // the rest of the score error bound, per point
-    float Xn[kGroups];     // >= ||x||
-};
-
-template <int NKB>
-__device__ __forceinline__ WaveBounds wave_bounds(const float (&X2)[kGroups],
-                                                  const float (&RX2)[kGroups], const float *stats,
-                                                  const MfmaParams &prm)
-{
-    const float Cx = stats[0], Nx = stats[1], R = stats[3];
-    const float gm = prm.gamma, ms = prm.mu * prm.sqrt_d;
-    const float absP = (float)(NKB * kKBlock + 2) * 2.3509887e-38f;  // (P+2) 2^-125
-    WaveBounds w;
-    float kneed = 0.0f;
-#pragma unroll
-    for (int g = 0; g < kGroups; ++g) {
-        const float X = sqrtf(X2[g]) * (1.0f + 4.0f * kU);
-        const float Rx = sqrtf(RX2[g]) * (1.0f + 4.0f * kU) + ms;
-        const float in = (X + Rx) * (R + ms) + Rx * 2.0f * Cx;
-        const float Pm = (X + Rx) * (2.0f * Cx + R + ms);
-        w.Xn[g] = X;
-        w.E0[g] = 1.01f * (in + gm * (Nx + Pm) + 2.0f * kU * Nx) + absP;
-        kneed = fmaxf(kneed, X2[g] + w.E0[g]);  // a NaN point does not poison K
-    }
-    for (int o = 1; o < 64; o <<= 1)
-        kneed = fmaxf(kneed, __shfl_xor(kneed, o));
-    w.beta = 1.01f * (gm + kU);
-    w.K = kneed * (1.0f + 1e-5f) / (1.0f - w.beta) + 1e-30f;
-    return w;
-}
-
-// Best / second keys over all centroid tiles, software-pipelined: the MFMAs
-// of tile t run while the keys of tile t-1 are ranked, and the LDS operands
-// of the next tile are already in flight (two accumulator and two operand
-// sets in turn; at the loop head accA holds the scored, unranked tile t-1
-// and o1 the operands of tile t).
-template <bool FP16, int NKB>
-__device__ __forceinline__ void rank_all(const u16x8 *lds_table, const float *lds_norm,
-                                         uint32_t ntiles, uint32_t lane, uint32_t h, float K,
-                                         uint32_t kmask, const PointOps<NKB> &x,
-                                         uint32_t (&best)[kGroups], uint32_t (&second)[kGroups])
-{
-#pragma unroll
-    for (int g = 0; g < kGroups; ++g)
-        best[g] = second[g] = 0xFFFFFFFFu;
-    const uint32_t last = ntiles - 1;
-    f32v4 accA[kGroups], accB[kGroups];
-    TileOps<NKB> o0 = load_ops<NKB>(lds_table, lds_norm, 0, lane, h);
-    TileOps<NKB> o1 = load_ops<NKB>(lds_table, lds_norm, min(1u, last), lane, h);
-    score_tile<FP16, NKB>(accA, o0, K, x);
-    uint32_t t = 1;
-    for (; t + 1 < ntiles; t += 2) {
-        o0 = load_ops<NKB>(lds_table, lds_norm, t + 1, lane, h);
-        score_tile<FP16, NKB>(accB, o1, K, x);
-        rank_tile(accA, (t - 1) * kTile + 4 * h, kmask, best, second);
-        o1 = load_ops<NKB>(lds_table, lds_norm, min(t + 2, last), lane, h);
-        score_tile<FP16, NKB>(accA, o0, K, x);
-        rank_tile(accB, t * kTile + 4 * h, kmask, best, second);
-    }
-    if (t < ntiles) {
-        score_tile<FP16, NKB>(accB, o1, K, x);
-        rank_tile(accA, (t - 1) * kTile + 4 * h, kmask, best, second);
-        rank_tile(accB, t * kTile + 4 * h, kmask, best, second);
-    } else {
-        rank_tile(accA, (t - 1) * kTile + 4 * h, kmask, best, second);
-    }
-}
-
-// Order this wave's LDS accesses (lanes of one wave exchange data through
-// LDS below; a wave's LDS operations complete in issue order, so only the
-// compiler has to be kept from moving them -- no wait on global memory).
-__device__ __forceinline__ void wave_lds_order()
-{
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-}
-
-__device__ __forceinline__ uint64_t lanemask_lt(uint32_t lane)
-{
-    return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-}
-
-// Stage the 16-bit operand table and the norms in LDS (whole work-group).
-template <int NKB>
-__device__ __forceinline__ void stage_table(u16x8 *lds_table, float *lds_norm,
-                                            const u16x8 *__restrict__ table,
-                                            const float *__restrict__ norm, uint32_t ntiles)
-{
-    const uint64_t entries = (uint64_t)ntiles * NKB * 64;
-    for (uint64_t e = threadIdx.x; e < entries; e += blockDim.x)
-        lds_table[e] = table[e];
-    for (uint32_t c = threadIdx.x; c < ntiles * kTile; c += blockDim.x)
-        lds_norm[c] = norm[c];
-    __syncthreads();
-}
 
 // Streaming kernel: work-group of fast_waves(NKB) waves (up to d = 64 two per
 // SIMD sharing one LDS table, so each wave's point loads hide behind the
@@ -757,6 +420,25 @@ hipError_t allow_lds(const void *kern, size_t lds)
 
 }  // namespace
 
+int mfma_subnormal_flags(clk_context *ctx, hipStream_t stream)
+{
+    if (ctx->mfma_subnormals >= 0)
+        return CLK_OK;
+    float *flags_d = nullptr;
+    CLK_HIP_CHECK(hipMalloc(&flags_d, 2 * sizeof(float)));
+    hipLaunchKernelGGL(mfma_subnormal_check_kernel, dim3(1), dim3(64), 0, stream, flags_d);
+    hipError_t e = hipGetLastError();
+    float flags[2] = {0.0f, 0.0f};
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(flags, flags_d, sizeof(flags), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(stream);
+    (void)hipFree(flags_d);
+    CLK_HIP_CHECK(e);
+    ctx->mfma_subnormals = (flags[0] == 1.0f ? 1 : 0) | (flags[1] == 1.0f ? 2 : 0);
+    return CLK_OK;
+}
+
 bool mfma_labeling_supported(const clk_context *ctx, uint64_t d, uint64_t n, uint64_t k)
 {
     if (d == 0 || d > (uint64_t)kMaxKb * kKBlock || k == 0 || k > (1u << 20) || n >= (1ull << 32))
@@ -799,27 +481,8 @@ int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, ui
     }
     if (n == 0)
         return CLK_OK;
-    if (ctx->mfma_subnormals < 0) {  // once per context
-        hipLaunchKernelGGL(mfma_subnormal_check_kernel, dim3(1), dim3(64), 0, stream, stats + 8);
-        CLK_HIP_CHECK(hipGetLastError());
-        float flags[2];
-        CLK_HIP_CHECK(hipMemcpyAsync(flags, stats + 8, sizeof(flags), hipMemcpyDeviceToHost, stream));
-        CLK_HIP_CHECK(hipStreamSynchronize(stream));
-        ctx->mfma_subnormals = (flags[0] == 1.0f ? 1 : 0) | (flags[1] == 1.0f ? 2 : 0);
-    }
-    MfmaParams prm;
-    const double u = std::ldexp(1.0, -24);
-    const double P = (double)l.nkb * kKBlock;
-    // a flushed input subnormal is off by at most the smallest normal
-    const bool kept = (ctx->mfma_subnormals & (fp16 ? 1 : 2)) != 0;
-    prm.mu = kept ? 0.0f : (fp16 ? (float)std::ldexp(1.0, -14) : (float)std::ldexp(1.0, -126));
-    prm.gamma = (float)(2.0 * (P + 1.0) * u / (1.0 - 2.0 * (P + 1.0) * u));
-    prm.key_bits = 1;
-    while ((1ull << prm.key_bits) < (uint64_t)l.ntiles * kTile)
-        ++prm.key_bits;
-    prm.trunc = (float)std::ldexp(1.0, (int)prm.key_bits - 23);
-    prm.eps = (float)((double)(d + 3) * u * 1.001);
-    prm.sqrt_d = (float)(std::sqrt((double)d) * (1.0 + 1e-6));
+    CLK_TRY(mfma_subnormal_flags(ctx, stream));
+    const MfmaParams prm = make_params(fp16, d, k, (ctx->mfma_subnormals & (fp16 ? 1 : 2)) != 0);
 
     const bool full = d == (uint64_t)l.nkb * kKBlock;
     LabelKernelM lk;

@@ -1,0 +1,96 @@
+/*
+ * clkmeans_batch.h -- many independent k-means jobs on one GPU.
+ *
+ * No reference counterpart: CL-kmeans runs one clustering per process
+ * (bench.cpp:200-452 drives one AbstractKmeans at a time).  A batch keeps many
+ * jobs (uneven n, d, k) resident in device memory and advances all active jobs
+ * by one Lloyd iteration per persistent kernel launch, captured once as a HIP
+ * graph.  Every job follows the reference's iteration exactly as
+ * clk_lloyd_step does (labels: the sequential-fma distance with strict '<'
+ * and lowest index on ties; masses exact; FP64 sums and divide), and a job's
+ * result does not depend on the other jobs of the batch: its points are cut
+ * into fixed tiles, each tile's partial sums go to its own FP64 row, and the
+ * work-group that finishes a job's last tile merges the rows in tile order.
+ *
+ * Layouts as in clkmeans.h: points float[d][n], centroids float[d][k], labels
+ * and masses uint32.  Jobs are addressed by slot; a slot is free again after
+ * clk_batch_take.  Functions return CLK_OK or a negative CLK_ERR_* code.
+ */
+#ifndef CLKMEANS_BATCH_H
+#define CLKMEANS_BATCH_H
+
+#include "clkmeans.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct clk_batch clk_batch;
+
+/* Job states. */
+#define CLK_JOB_FREE 0
+#define CLK_JOB_ACTIVE 1
+#define CLK_JOB_DONE 2
+
+typedef struct clk_batch_config {
+    uint32_t max_jobs;        /* slots */
+    uint64_t arena_bytes;     /* device memory for job data (points, labels, centroids, partials) */
+    uint32_t tile_work;       /* target k*d*points per tile; 0 = default (autotuned value if set) */
+    uint32_t max_d;           /* largest d admitted (<= 64) */
+    uint32_t max_k;           /* largest k admitted */
+    int32_t labeling;         /* CLK_LABELING_UNROLL_VECTOR (plain FP32), CLK_LABELING_MFMA_BF16 or
+                                 CLK_LABELING_MFMA_FP16: x.c^T on the matrix cores with the exact
+                                 refinement (same labels) */
+} clk_batch_config;
+
+/* Device memory one job needs in the arena (for admission by footprint). */
+uint64_t clk_batch_job_bytes(const clk_batch_config *cfg, uint64_t n, uint32_t d, uint32_t k);
+
+int clk_batch_create(clk_context *ctx, const clk_batch_config *cfg, clk_batch **out);
+int clk_batch_destroy(clk_batch *b);
+
+/* Admit a job: copy host points float[d][n] and initial centroids float[d][k]
+ * to the device.  It runs until max_iterations iterations or, when
+ * tolerance > 0, until the max centroid shift (FP64, ||new_c - old_c||_2 over
+ * non-empty clusters) falls below tolerance.  CLK_ERR_NOMEM when no slot or
+ * arena space is free (the caller retries after taking finished jobs). */
+int clk_batch_add(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *host_points,
+                  const float *host_centroids, uint32_t max_iterations, double tolerance,
+                  int32_t *slot, void *stream);
+
+/* One Lloyd iteration of every active job: one persistent kernel launch
+ * (replayed from a captured HIP graph while the set of jobs is unchanged). */
+int clk_batch_step(clk_batch *b, void *stream);
+
+/* Number of active jobs and, in states[] (max_jobs entries, may be NULL),
+ * each slot's CLK_JOB_* state.  Synchronizes the stream. */
+int clk_batch_poll(clk_batch *b, uint32_t *states, uint32_t *active, void *stream);
+
+/* Copy a job's results to the host (any pointer may be NULL) and free its
+ * slot.  centroids float[d][k], labels uint32[n] (of the last labeling),
+ * masses uint32[k]; iterations run and the last max shift. */
+int clk_batch_take(clk_batch *b, int32_t slot, float *host_centroids, uint32_t *host_labels,
+                   uint32_t *host_masses, uint32_t *iterations, double *shift, void *stream);
+
+/* Device buffers of a slot (points [d][n], current centroids [d][k], labels
+ * [n]), e.g. to run the same job through clk_lloyd_step for comparison. */
+int clk_batch_job_buffers(clk_batch *b, int32_t slot, const float **points, float **centroids,
+                          uint32_t **labels);
+
+/* The plain per-job loop on the same device buffers, for comparison: every
+ * active job in slot order runs its remaining iterations alone through
+ * clk_lloyd_step (fused cluster_merge pass + FP64 merge and divide), one job
+ * after another; with a tolerance the shift is read back after every
+ * iteration.  Jobs end in the same states as with clk_batch_step. */
+int clk_batch_run_serial(clk_batch *b, void *stream);
+
+/* Autotune tile_work on the current jobs: times label+accumulate passes
+ * (without advancing any job) for a few tile sizes and keeps the fastest.
+ * chosen may be NULL. */
+int clk_batch_autotune(clk_batch *b, uint32_t *chosen, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLKMEANS_BATCH_H */
