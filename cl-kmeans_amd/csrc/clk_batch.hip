@@ -27,7 +27,10 @@
 // the work-group's owner array O[L] and reads it back; the lanes that find
 // their own id -- one per distinct label -- add their point to their wave's
 // T[L] / M[L] with plain read-modify-writes, the others retry (a lane of
-// another wave may also win O[L]; that only costs a round).
+// another wave may also win O[L]; that only costs a round).  A job with few
+// clusters gets R replicas of each wave's table (R = 2..16, lanes spread over
+// them), so a wave's 64 points collide less; every replica is summed in the
+// tile's partial row in a fixed order.
 #include "clk_common.hpp"
 #include "clk_kernels.hpp"
 #include "clk_mfma_device.hpp"
@@ -61,6 +64,7 @@ struct BJob {
     float *stats;
     mfma::MfmaParams prm;
     uint32_t nt16, nkb;
+    uint32_t rep;       // table replicas per wave (power of two)
 };
 
 struct BTile {
@@ -124,17 +128,21 @@ __device__ void batch_tile(const BJob &J, uint32_t tindex, float *lds, uint32_t 
 {
     const uint32_t tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     const uint32_t d = J.d, k = J.k;
+    const uint32_t R = J.rep, rk = R * k;
     float *cen_s = lds;
     float *tabs = cen_s + (size_t)k * DMAX;
-    uint32_t *mass = reinterpret_cast<uint32_t *>(tabs + (size_t)waves * k * DMAX);
-    uint32_t *owner = mass + (size_t)waves * k;
-    float *T = tabs + (size_t)wave * k * DMAX;
-    uint32_t *M = mass + (size_t)wave * k;
+    uint32_t *mass = reinterpret_cast<uint32_t *>(tabs + (size_t)waves * rk * DMAX);
+    uint32_t *owner = mass + (size_t)waves * rk;
+    // this lane's replica of its wave's table: rows r*k + c
+    const uint32_t roff = (lane & (R - 1)) * k;
+    float *T = tabs + ((size_t)wave * rk + roff) * DMAX;
+    uint32_t *M = mass + (size_t)wave * rk + roff;
+    uint32_t *O = owner + roff;
 
     stage_centroids<DMAX>(J.cen[J.iter & 1], d, k, cen_s);
-    for (uint32_t i = tid; i < waves * k * DMAX; i += blockDim.x)
+    for (uint32_t i = tid; i < waves * rk * DMAX; i += blockDim.x)
         tabs[i] = 0.0f;
-    for (uint32_t i = tid; i < waves * k; i += blockDim.x)
+    for (uint32_t i = tid; i < waves * rk; i += blockDim.x)
         mass[i] = 0u;
     __syncthreads();
 
@@ -157,9 +165,9 @@ __device__ void batch_tile(const BJob &J, uint32_t tindex, float *lds, uint32_t 
         bool pending = valid;
         while (__ballot(pending) != 0) {
             if (pending)
-                owner[L] = tid;
+                O[L] = tid;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            const bool win = pending && owner[L] == tid;
+            const bool win = pending && O[L] == tid;
             if (win) {
                 float *row = T + (size_t)L * DMAX;
 #pragma unroll
@@ -179,13 +187,13 @@ __device__ void batch_tile(const BJob &J, uint32_t tindex, float *lds, uint32_t 
     for (uint32_t i = tid; i < k * d; i += blockDim.x) {
         const uint32_t f = i / k, c = i % k;
         double s = 0.0;
-        for (uint32_t w = 0; w < waves; ++w)
+        for (uint32_t w = 0; w < waves * R; ++w)
             s += (double)tabs[((size_t)w * k + c) * DMAX + f];
         row[i] = s;
     }
     for (uint32_t c = tid; c < k; c += blockDim.x) {
         uint32_t m = 0;
-        for (uint32_t w = 0; w < waves; ++w)
+        for (uint32_t w = 0; w < waves * R; ++w)
             m += mass[(size_t)w * k + c];
         row[(size_t)k * d + c] = (double)m;
     }
@@ -309,9 +317,12 @@ __device__ void batch_tile_mfma(const BJob &J, uint32_t tindex, char *lds, const
     const uint64_t n = J.n;
     u16x8 *ltab = reinterpret_cast<u16x8 *>(lds);
     float *lnorm = reinterpret_cast<float *>(lds + Lo.o_norm);
-    float *T = reinterpret_cast<float *>(lds + Lo.o_tabs) + (size_t)wave * Lo.kmax * Lo.pmax;
-    uint32_t *M = reinterpret_cast<uint32_t *>(lds + Lo.o_mass) + (size_t)wave * Lo.kmax;
-    uint32_t *owner = reinterpret_cast<uint32_t *>(lds + Lo.o_owner);
+    const uint32_t R = J.rep, rk = R * k;
+    // this column's replica of its wave's table: rows r*k + c
+    const uint32_t roff = (q & (R - 1)) * k;
+    float *T = reinterpret_cast<float *>(lds + Lo.o_tabs) + (size_t)wave * Lo.kmax * Lo.pmax + (size_t)roff * P;
+    uint32_t *M = reinterpret_cast<uint32_t *>(lds + Lo.o_mass) + (size_t)wave * Lo.kmax + roff;
+    uint32_t *owner = reinterpret_cast<uint32_t *>(lds + Lo.o_owner) + roff;
     uint32_t *pl = reinterpret_cast<uint32_t *>(lds + Lo.o_pairs) + (size_t)wave * kBatchPairs;
     unsigned long long *wmin = reinterpret_cast<unsigned long long *>(lds + Lo.o_min) + (size_t)wave * 64;
     uint32_t *pc = reinterpret_cast<uint32_t *>(lds + Lo.o_count) + wave;
@@ -324,9 +335,9 @@ __device__ void batch_tile_mfma(const BJob &J, uint32_t tindex, char *lds, const
     for (uint32_t w = 0; w < waves; ++w) {
         float *Tw = reinterpret_cast<float *>(lds + Lo.o_tabs) + (size_t)w * Lo.kmax * Lo.pmax;
         uint32_t *Mw = reinterpret_cast<uint32_t *>(lds + Lo.o_mass) + (size_t)w * Lo.kmax;
-        for (uint32_t i = tid; i < k * P; i += blockDim.x)
+        for (uint32_t i = tid; i < rk * P; i += blockDim.x)
             Tw[i] = 0.0f;
-        for (uint32_t i = tid; i < k; i += blockDim.x)
+        for (uint32_t i = tid; i < rk; i += blockDim.x)
             Mw[i] = 0u;
     }
     __syncthreads();
@@ -461,15 +472,20 @@ __device__ void batch_tile_mfma(const BJob &J, uint32_t tindex, char *lds, const
     for (uint32_t i = tid; i < k * d; i += blockDim.x) {
         const uint32_t f = i / k, c = i % k;
         double sum = 0.0;
-        for (uint32_t w = 0; w < waves; ++w)
-            sum += (double)(reinterpret_cast<float *>(lds + Lo.o_tabs) +
-                            (size_t)w * Lo.kmax * Lo.pmax)[(size_t)c * P + f];
+        for (uint32_t w = 0; w < waves; ++w) {
+            const float *Tw = reinterpret_cast<float *>(lds + Lo.o_tabs) + (size_t)w * Lo.kmax * Lo.pmax;
+            for (uint32_t r = 0; r < R; ++r)
+                sum += (double)Tw[((size_t)r * k + c) * P + f];
+        }
         prow[i] = sum;
     }
     for (uint32_t c = tid; c < k; c += blockDim.x) {
         uint32_t m = 0;
-        for (uint32_t w = 0; w < waves; ++w)
-            m += (reinterpret_cast<uint32_t *>(lds + Lo.o_mass) + (size_t)w * Lo.kmax)[c];
+        for (uint32_t w = 0; w < waves; ++w) {
+            const uint32_t *Mw = reinterpret_cast<uint32_t *>(lds + Lo.o_mass) + (size_t)w * Lo.kmax;
+            for (uint32_t r = 0; r < R; ++r)
+                m += Mw[(size_t)r * k + c];
+        }
         prow[(size_t)k * d + c] = (double)m;
     }
 }
@@ -827,6 +843,16 @@ extern "C" int clk_batch_add(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, c
     J.stats = reinterpret_cast<float *>(p);
     p += 256;
     J.prm = clk::mfma::make_params(b->fp16, d, k, (b->ctx->mfma_subnormals & (b->fp16 ? 1 : 2)) != 0);
+    {
+        // replicas: R*k rows must fit the rows and cells sized for max_k x max_d
+        const uint64_t p_job = b->mfma ? (uint64_t)J.nkb * clk::mfma::kKBlock : clk::bucket_of(d);
+        const uint64_t p_max = b->mfma ? (uint64_t)b->layout.pmax : clk::bucket_of(b->cfg.max_d);
+        const uint64_t lim = std::min<uint64_t>(std::min<uint64_t>(16, b->cfg.max_k / k),
+                                                (uint64_t)b->cfg.max_k * p_max / ((uint64_t)k * p_job));
+        J.rep = 1;
+        while (J.rep * 2 <= lim)
+            J.rep *= 2;
+    }
     J.pts = reinterpret_cast<const float *>(p);
     p += clk::align_up(n * d * 4, 256);
     J.labels = reinterpret_cast<uint32_t *>(p);
