@@ -1,36 +1,37 @@
 // Many independent Lloyd jobs per launch (the batch API, clkmeans_batch.h).
 //
 // Device state: a job table (BJob, one per slot), a tile list (each job's
-// points cut into fixed tiles of tile_pts points, contiguous per job) and a
-// control block.  One iteration of every active job is ONE persistent kernel
-// launch: work-groups take tiles from a global counter; a tile is labeled
-// with the reference arithmetic (centroids staged in LDS, sequential fma
-// distance, strict '<', lowest index) and accumulated into per-wave LDS
-// tables; the tile's sums and masses go to the job's FP64 partial row for
-// that tile.  The work-group that completes a job's last tile (agent-scope
-// release / acquire around a per-job counter) merges the job's rows in tile
-// order in FP64, divides (one rounding to float, NaN for an empty cluster),
-// computes the FP64 max centroid shift and advances the job.  The last
-// work-group to leave resets the work counter, so the launch can be replayed
-// from a HIP graph without any other node.
+// points cut into fixed tiles of tile_pts points, contiguous per job), the
+// list of active slots and a control block.  One iteration of every active
+// job is two launches, replayed from one HIP graph.  The tile kernel is
+// persistent: work-groups take tiles from a global counter (the next one is
+// requested while the current one is processed); a tile is labeled with the
+// reference arithmetic (centroids staged in LDS, sequential fma distance,
+// strict '<', lowest index) and accumulated into per-wave LDS tables, and
+// the tile's sums and masses go to the job's FP64 partial row for that tile.
+// The merge kernel (one work-group per active job) then sums the job's rows
+// in a fixed order in FP64, divides (one rounding to float, NaN for an empty
+// cluster), computes the FP64 max centroid shift, advances the job and
+// resets the work counter.  The kernel boundary publishes the partial rows:
+// no tile needs a device-scope fence, which on this GPU writes back and
+// invalidates the L2 of the tile's XCD.
 //
 // With an MFMA labeling strategy each tile is labeled like the dense-distance
 // kernels of clk_mfma.hip (the job's 16-bit centroid table staged in LDS,
 // scores on the matrix cores, the margin test; undecided points of a wave
 // are settled in the wave by exact FP32 distances to their candidate
-// centroids), and accumulated straight from the MFMA lane layout (the 4 lanes
-// of a point column own 8 features each).  The finalizing work-group also
-// builds the job's table for the next iteration.
+// centroids).  The merge kernel also builds the job's table for the next
+// iteration.
 //
-// Accumulation without float atomics: each wave owns an LDS table T[k][DMAX]
-// (FP32) and a mass row M[k]; a lane whose point has label L writes its id to
-// the work-group's owner array O[L] and reads it back; the lanes that find
-// their own id -- one per distinct label -- add their point to their wave's
-// T[L] / M[L] with plain read-modify-writes, the others retry (a lane of
-// another wave may also win O[L]; that only costs a round).  A job with few
-// clusters gets R replicas of each wave's table (R = 2..16, lanes spread over
-// them), so a wave's 64 points collide less; every replica is summed in the
-// tile's partial row in a fixed order.
+// Accumulation without float atomics: each wave owns an LDS table T[k][ts]
+// (FP32; columns 0..d-1 the feature sums, column d the point count, ts the
+// odd row stride d+1 or d+2).  For every batch of 64 points the wave sorts
+// its (label, lane) keys (a bitonic network over the lanes), so the points of
+// one label form a run of lanes; a segmented scan along the sorted lanes
+// (DPP row shifts and row broadcasts) leaves each run's sum on its last lane,
+// which adds it to T[label] -- one read-modify-write per distinct label, in
+// an order fixed by the batch's labels alone.  The tile's partial row sums
+// the waves' tables in wave order.
 #include "clk_common.hpp"
 #include "clk_kernels.hpp"
 #include "clk_mfma_device.hpp"
@@ -54,9 +55,9 @@ struct BJob {
     uint32_t d, k;
     uint32_t tile0, ntiles, tile_pts;
     uint32_t max_iter, iter;
-    uint32_t done;      // tiles finished in the current iteration
     uint32_t state;     // CLK_JOB_*
     uint32_t dmax;      // register bucket of d
+    uint32_t kind;      // 0: FP32 labeling, 1: MFMA labeling
     double tol, shift;
     // MFMA labeling: 16-bit table [nt16][nkb][64], norms [nt16*16], stats[16]
     mfma::u16x8 *table;
@@ -64,7 +65,6 @@ struct BJob {
     float *stats;
     mfma::MfmaParams prm;
     uint32_t nt16, nkb;
-    uint32_t rep;       // table replicas per wave (power of two)
 };
 
 struct BTile {
@@ -72,21 +72,25 @@ struct BTile {
 };
 
 struct BCtl {
-    uint32_t next;    // work counter
-    uint32_t exited;  // work-groups that left this launch
+    uint32_t next;    // work counter of the tile kernel
     uint32_t ntiles;  // tiles in the list
-    uint32_t dry;     // 1: label and accumulate only (no merge, no job advance)
+    uint32_t dry;     // 1: the merge kernel only resets the counter (no job advances)
+    uint32_t pad;
 };
 
 constexpr uint32_t kBatchMaxWaves = 4;
 constexpr uint32_t kBatchPairs = 256;  // candidate pairs per wave and flush (MFMA refinement)
 
+// Row stride of a wave table: d sums and the count, made odd so that the
+// last lanes of 64 runs (distinct labels) spread over the LDS banks.
+__host__ __device__ inline uint32_t table_stride(uint32_t d) { return (d + 1) | 1u; }
+
 // LDS of the MFMA batch kernel for the largest admitted job (byte offsets):
-// table | norms | W tables [kmax][pmax] | W mass rows [kmax] | owners [kmax] |
-// W pair lists | W running minima [64] | W pair counters
+// table | norms | W tables [kmax][ts(dmax)] | W pair lists | W running
+// minima [64] | W pair counters
 struct BLayout {
-    uint32_t kmax, pmax, waves;
-    uint32_t o_norm, o_tabs, o_mass, o_owner, o_pairs, o_min, o_count, bytes;
+    uint32_t kmax, waves;
+    uint32_t o_norm, o_tabs, o_pairs, o_min, o_count, bytes;
 };
 
 inline BLayout mfma_layout_for(uint32_t waves, uint32_t kmax, uint32_t dmax_feat)
@@ -95,17 +99,12 @@ inline BLayout mfma_layout_for(uint32_t waves, uint32_t kmax, uint32_t dmax_feat
     const uint32_t nt16 = (kmax + mfma::kTile - 1) / mfma::kTile;
     const uint32_t nkb = (dmax_feat + mfma::kKBlock - 1) / mfma::kKBlock;
     L.kmax = kmax;
-    L.pmax = nkb * mfma::kKBlock;
     L.waves = waves;
     size_t o = (size_t)nt16 * nkb * 64 * 16;
     L.o_norm = (uint32_t)o;
     o += (size_t)nt16 * mfma::kTile * 4;
     L.o_tabs = (uint32_t)o;
-    o += (size_t)waves * kmax * L.pmax * 4;
-    L.o_mass = (uint32_t)o;
-    o += (size_t)waves * kmax * 4;
-    L.o_owner = (uint32_t)o;
-    o += (size_t)kmax * 4;
+    o += (size_t)waves * kmax * table_stride(dmax_feat) * 4;
     L.o_pairs = (uint32_t)o;
     o += (size_t)waves * kBatchPairs * 4;
     o = (o + 15) / 16 * 16;
@@ -117,33 +116,130 @@ inline BLayout mfma_layout_for(uint32_t waves, uint32_t kmax, uint32_t dmax_feat
     return L;
 }
 
-// LDS: centroids [k][DMAX] | W tables [k][DMAX] | W mass rows [k] | owners [k]
-__host__ __device__ inline size_t batch_lds_bytes(uint32_t waves, uint32_t k, uint32_t dm)
+// LDS of the plain kernel: centroids [k][DMAX] | W tables [k][ts(dmax)]
+__host__ __device__ inline size_t batch_lds_bytes(uint32_t waves, uint32_t k, uint32_t dm, uint32_t dmax_feat)
 {
-    return (size_t)k * dm * 4 * (1 + waves) + (size_t)k * 4 * (waves + 1);
+    return (size_t)k * dm * 4 + (size_t)waves * k * table_stride(dmax_feat) * 4;
+}
+
+// ---- runs of equal labels in a wave -------------------------------------
+
+constexpr uint32_t kNoLabel = 0x3FFFFFFu;  // sorts after every label (< 2^26)
+
+// Bitonic sort of one 32-bit key per lane, ascending in lane order.
+__device__ __forceinline__ uint32_t wave_sort(uint32_t key, uint32_t lane)
+{
+#pragma unroll
+    for (uint32_t size = 2; size <= 64; size *= 2)
+#pragma unroll
+        for (uint32_t j = size / 2; j > 0; j /= 2) {
+            const uint32_t other = (uint32_t)__shfl_xor((int)key, (int)j);
+            const bool desc = (lane & size) != 0;
+            const bool lower = (lane & j) == 0;
+            key = (lower != desc) ? min(key, other) : max(key, other);
+        }
+    return key;
+}
+
+// DPP controls: row_shr:n (n = 1..15) within rows of 16 lanes; row_bcast:15
+// (the last lane of each row to the next row) and row_bcast:31 (lane 31 to
+// rows 2 and 3).
+constexpr int kDppRowShr = 0x110, kDppBcast15 = 0x142, kDppBcast31 = 0x143;
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_move(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, false);
+}
+
+// The wave's points sorted by label.  Lane i holds the point of lane src;
+// the lanes of one label are contiguous; on the last lane of a run, last is
+// set and count is the run's length.  step[s] says whether scan step s adds
+// a value of the same run (steps: shifts by 1, 2, 4, 8 within rows of 16, then
+// the row broadcasts of lanes 15 / 47 to rows 1 / 3 and of lane 31 to rows 2, 3).
+struct WaveRuns {
+    uint32_t label, src, count;
+    bool last;
+    bool step[6];
+};
+
+__device__ __forceinline__ WaveRuns wave_runs(uint32_t label, bool valid, uint32_t lane)
+{
+    const uint32_t key = wave_sort(valid ? (label << 6) | lane : 0xFFFFFFFFu, lane);
+    WaveRuns r;
+    r.label = key >> 6;
+    r.src = key & 63u;
+    const uint32_t prev = (uint32_t)__shfl((int)r.label, (int)((lane + 63) & 63));
+    const uint64_t heads = __ballot(lane == 0 || prev != r.label);
+    const bool run_end = lane == 63 || (((heads >> lane) >> 1) & 1ull) != 0;
+    r.last = run_end && r.label != kNoLabel;
+    const uint64_t upto = heads & ((2ull << lane) - 1ull);  // lanes 0..lane
+    r.count = lane - (63u - (uint32_t)__builtin_clzll(upto)) + 1u;
+    // every lane takes part in every move (a lane that sits out would feed
+    // the old value to its neighbours)
+    const uint32_t l1 = dpp_move<kDppRowShr + 1, 0xF>(r.label);
+    const uint32_t l2 = dpp_move<kDppRowShr + 2, 0xF>(r.label);
+    const uint32_t l4 = dpp_move<kDppRowShr + 4, 0xF>(r.label);
+    const uint32_t l8 = dpp_move<kDppRowShr + 8, 0xF>(r.label);
+    const uint32_t l15 = dpp_move<kDppBcast15, 0xA>(r.label);
+    const uint32_t l31 = dpp_move<kDppBcast31, 0xC>(r.label);
+    const uint32_t row_lane = lane & 15u, row = lane >> 4;
+    r.step[0] = (row_lane >= 1) & (l1 == r.label);
+    r.step[1] = (row_lane >= 2) & (l2 == r.label);
+    r.step[2] = (row_lane >= 4) & (l4 == r.label);
+    r.step[3] = (row_lane >= 8) & (l8 == r.label);
+    r.step[4] = ((row & 1u) != 0) & (l15 == r.label);
+    r.step[5] = (row >= 2) & (l31 == r.label);
+    return r;
+}
+
+// Inclusive scan of v along the sorted lanes within runs: the last lane of a
+// run ends with the run's sum (a fixed summation tree).
+__device__ __forceinline__ float run_sum(float v, const WaveRuns &r)
+{
+    float t;
+    t = __uint_as_float(dpp_move<kDppRowShr + 1, 0xF>(__float_as_uint(v)));
+    v = r.step[0] ? v + t : v;
+    t = __uint_as_float(dpp_move<kDppRowShr + 2, 0xF>(__float_as_uint(v)));
+    v = r.step[1] ? v + t : v;
+    t = __uint_as_float(dpp_move<kDppRowShr + 4, 0xF>(__float_as_uint(v)));
+    v = r.step[2] ? v + t : v;
+    t = __uint_as_float(dpp_move<kDppRowShr + 8, 0xF>(__float_as_uint(v)));
+    v = r.step[3] ? v + t : v;
+    t = __uint_as_float(dpp_move<kDppBcast15, 0xA>(__float_as_uint(v)));
+    v = r.step[4] ? v + t : v;
+    t = __uint_as_float(dpp_move<kDppBcast31, 0xC>(__float_as_uint(v)));
+    v = r.step[5] ? v + t : v;
+    return v;
+}
+
+// Partial row of a tile: the waves' tables summed in wave order (FP64).
+__device__ __forceinline__ void write_partial_row(const float *tabs, uint32_t waves, uint32_t kstride,
+                                                  uint32_t d, uint32_t k, double *prow)
+{
+    const uint32_t ts = table_stride(d);
+    for (uint32_t i = threadIdx.x; i < k * (d + 1); i += blockDim.x) {
+        // sums [d][k] then masses [k]: element i is (feature f, cluster c)
+        const uint32_t f = i / k, c = i % k;
+        double s = 0.0;
+        for (uint32_t w = 0; w < waves; ++w)
+            s += (double)tabs[((size_t)w * kstride + c) * ts + f];
+        prow[i] = s;
+    }
 }
 
 template <int DMAX>
 __device__ void batch_tile(const BJob &J, uint32_t tindex, float *lds, uint32_t waves)
 {
     const uint32_t tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-    const uint32_t d = J.d, k = J.k;
-    const uint32_t R = J.rep, rk = R * k;
+    const uint32_t d = J.d, k = J.k, ts = table_stride(d);
     float *cen_s = lds;
     float *tabs = cen_s + (size_t)k * DMAX;
-    uint32_t *mass = reinterpret_cast<uint32_t *>(tabs + (size_t)waves * rk * DMAX);
-    uint32_t *owner = mass + (size_t)waves * rk;
-    // this lane's replica of its wave's table: rows r*k + c
-    const uint32_t roff = (lane & (R - 1)) * k;
-    float *T = tabs + ((size_t)wave * rk + roff) * DMAX;
-    uint32_t *M = mass + (size_t)wave * rk + roff;
-    uint32_t *O = owner + roff;
+    float *T = tabs + (size_t)wave * k * ts;
 
     stage_centroids<DMAX>(J.cen[J.iter & 1], d, k, cen_s);
-    for (uint32_t i = tid; i < waves * rk * DMAX; i += blockDim.x)
+    for (uint32_t i = tid; i < waves * k * ts; i += blockDim.x)
         tabs[i] = 0.0f;
-    for (uint32_t i = tid; i < waves * rk; i += blockDim.x)
-        mass[i] = 0u;
     __syncthreads();
 
     const uint64_t first = (uint64_t)tindex * J.tile_pts;
@@ -161,63 +257,78 @@ __device__ void batch_tile(const BJob &J, uint32_t tindex, float *lds, uint32_t 
         argmin_lds<DMAX, 1>(x, cen_s, k, lab);
         if (valid)
             J.labels[p] = lab[0];
-        const uint32_t L = lab[0];
-        bool pending = valid;
-        while (__ballot(pending) != 0) {
-            if (pending)
-                O[L] = tid;
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            const bool win = pending && O[L] == tid;
-            if (win) {
-                float *row = T + (size_t)L * DMAX;
+        const WaveRuns r = wave_runs(lab[0], valid, lane);
+        float *row = T + (size_t)(r.last ? r.label : 0u) * ts;
 #pragma unroll
-                for (int f = 0; f < DMAX; ++f)
-                    if (f < (int)d)
-                        row[f] = row[f] + x[f][0];
-                M[L] = M[L] + 1u;
+        for (int f = 0; f < DMAX; ++f)
+            if (f < (int)d) {
+                const float v = run_sum(__shfl(x[f][0], (int)r.src), r);
+                if (r.last)
+                    row[f] += v;
             }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            pending = pending && !win;
-        }
-        (void)lane;
+        if (r.last)
+            row[d] += (float)r.count;
+        mfma::wave_lds_order();
     }
     __syncthreads();
-    // partial row of this tile (FP64, waves summed in a fixed order)
-    double *row = J.part + (size_t)tindex * ((size_t)k * (d + 1));
-    for (uint32_t i = tid; i < k * d; i += blockDim.x) {
-        const uint32_t f = i / k, c = i % k;
-        double s = 0.0;
-        for (uint32_t w = 0; w < waves * R; ++w)
-            s += (double)tabs[((size_t)w * k + c) * DMAX + f];
-        row[i] = s;
-    }
-    for (uint32_t c = tid; c < k; c += blockDim.x) {
-        uint32_t m = 0;
-        for (uint32_t w = 0; w < waves * R; ++w)
-            m += mass[(size_t)w * k + c];
-        row[(size_t)k * d + c] = (double)m;
-    }
+    write_partial_row(tabs, waves, k, d, k, J.part + (size_t)tindex * ((size_t)k * (d + 1)));
 }
 
-// Merge a job's tile rows in tile order, divide, shift, advance the job.
+// Lanes per element in the merge of a job's tile rows: a power of two <= 64
+// that depends only on the row length and the tile count (never on the
+// launch), so a job's merged sums are the same in every batch.
+__device__ inline uint32_t merge_split(uint32_t row, uint32_t nt)
+{
+    uint32_t s = 1;
+    while (s < 64 && 2 * s <= nt && (2 * s) * row <= 64 * kBatchMaxWaves)
+        s *= 2;
+    return s;
+}
+
+// Merge a job's tile rows, divide, shift, advance the job.  Element e of the
+// row (sums [d][k], then masses [k]) is summed by S lanes of one wave: lane s
+// adds rows s, s+S, s+2S, ... in order, then the S lane sums combine by an
+// xor butterfly (every lane ends with the same bits); the merged row
+// overwrites row 0, from which the divide reads.
 __device__ void batch_finalize(BJob &J, double *red)
 {
     const uint32_t tid = threadIdx.x;
     const uint32_t d = J.d, k = J.k, nt = J.ntiles;
-    const size_t rl = (size_t)k * (d + 1);
+    const uint32_t rl = k * (d + 1);
+    const uint32_t S = merge_split(rl, nt);
+    const uint32_t per = blockDim.x / S, s = tid % S;
+    for (uint32_t e0 = 0; e0 < rl; e0 += per) {
+        const uint32_t e = e0 + tid / S;
+        double acc = 0.0;
+        if (e < rl) {
+            const double *col = J.part + e;
+            uint32_t t = s;
+            for (; t + 3 * S < nt; t += 4 * S) {
+                const double a0 = col[(size_t)t * rl], a1 = col[(size_t)(t + S) * rl];
+                const double a2 = col[(size_t)(t + 2 * S) * rl], a3 = col[(size_t)(t + 3 * S) * rl];
+                acc += a0;
+                acc += a1;
+                acc += a2;
+                acc += a3;
+            }
+            for (; t < nt; t += S)
+                acc += col[(size_t)t * rl];
+        }
+        for (uint32_t o = 1; o < S; o *= 2)
+            acc += __shfl_xor(acc, (int)o);
+        if (e < rl && s == 0)  // row 0 of element e is read only by this lane
+            J.part[e] = acc;
+    }
+    __syncthreads();
+    const double *tot = J.part;
     const float *old = J.cen[J.iter & 1];
     float *nxt = J.cen[(J.iter & 1) ^ 1];
     double local = 0.0;
     for (uint32_t c = tid; c < k; c += blockDim.x) {
-        double m = 0.0;
-        for (uint32_t t = 0; t < nt; ++t)
-            m += J.part[t * rl + (size_t)k * d + c];
+        const double m = tot[(size_t)k * d + c];
         double sq = 0.0;
         for (uint32_t f = 0; f < d; ++f) {
-            double s = 0.0;
-            for (uint32_t t = 0; t < nt; ++t)
-                s += J.part[t * rl + (size_t)f * k + c];
-            const float v = (float)(s / m);
+            const float v = (float)(tot[(size_t)f * k + c] / m);
             nxt[(size_t)f * k + c] = v;
             const double diff = (double)v - (double)old[(size_t)f * k + c];
             sq += diff * diff;
@@ -243,66 +354,51 @@ __device__ void batch_finalize(BJob &J, double *red)
     if (tid == 0) {
         J.shift = red[0];
         J.iter += 1;
-        J.done = 0;
         if (J.iter >= J.max_iter || (J.tol > 0.0 && red[0] < J.tol))
             J.state = CLK_JOB_DONE;
     }
 }
 
-__global__ void __launch_bounds__(64 * kBatchMaxWaves)
-batch_step_kernel(BJob *__restrict__ jobs, const BTile *__restrict__ tiles, BCtl *__restrict__ ctl)
+// The persistent tile loop: work-group-uniform item numbers from the global
+// counter, the next one requested before the current tile is processed.
+template <typename Body>
+__device__ __forceinline__ void for_each_tile(const BJob *__restrict__ jobs, const BTile *__restrict__ tiles,
+                                              BCtl *__restrict__ ctl, Body body)
 {
-    extern __shared__ __align__(16) float lds[];
     __shared__ uint32_t s_item;
-    __shared__ int s_last;
-    __shared__ double s_red[64 * kBatchMaxWaves];
     const uint32_t tid = threadIdx.x;
-    const uint32_t waves = blockDim.x / 64;
     const uint32_t ntiles = ctl->ntiles;
-    const bool dry = ctl->dry != 0;
+    uint32_t next = 0;
+    if (tid == 0)
+        next = atomicAdd(&ctl->next, 1u);
     for (;;) {
-        __syncthreads();
         if (tid == 0)
-            s_item = atomicAdd(&ctl->next, 1u);
+            s_item = next;
         __syncthreads();
         const uint32_t item = s_item;
+        __syncthreads();  // s_item is written again at the next loop head
         if (item >= ntiles)
             break;
-        const BTile t = tiles[item];
-        BJob &J = jobs[t.job];
-        if (J.state != CLK_JOB_ACTIVE)
-            continue;
-        switch (J.dmax) {
-        case 4: batch_tile<4>(J, t.index, lds, waves); break;
-        case 8: batch_tile<8>(J, t.index, lds, waves); break;
-        case 16: batch_tile<16>(J, t.index, lds, waves); break;
-        case 32: batch_tile<32>(J, t.index, lds, waves); break;
-        default: batch_tile<64>(J, t.index, lds, waves); break;
-        }
-        // publish this tile's row; the job's last tile merges
-        __threadfence();
-        __syncthreads();
         if (tid == 0)
-            s_last = atomicAdd(&J.done, 1u) == J.ntiles - 1;
-        __syncthreads();
-        if (s_last) {
-            __threadfence();
-            if (dry) {
-                if (tid == 0)
-                    J.done = 0;
-            } else {
-                batch_finalize(J, s_red);
-            }
-        }
+            next = atomicAdd(&ctl->next, 1u);
+        const BTile t = tiles[item];
+        const BJob &J = jobs[t.job];
+        if (J.state == CLK_JOB_ACTIVE)
+            body(J, t.index);
     }
-    // the last work-group out resets the counters for the next launch
-    if (tid == 0) {
-        __threadfence();
-        if (atomicAdd(&ctl->exited, 1u) == gridDim.x - 1) {
-            ctl->next = 0;
-            ctl->exited = 0;
-        }
-    }
+}
+
+// Tiles of the jobs of one size class (FP32 labeling, d <= DMAX).
+template <int DMAX>
+__global__ void __launch_bounds__(64 * kBatchMaxWaves)
+batch_tiles_kernel(const BJob *__restrict__ jobs, const BTile *__restrict__ tiles, BCtl *__restrict__ ctl)
+{
+    extern __shared__ __align__(16) float lds[];
+    const uint32_t waves = blockDim.x / 64;
+    for_each_tile(jobs, tiles, ctl, [&](const BJob &J, uint32_t index) {
+        batch_tile<DMAX>(J, index, lds, waves);
+        __syncthreads();  // the tables are cleared for the next tile
+    });
 }
 
 template <bool FP16, int NKB>
@@ -317,12 +413,9 @@ __device__ void batch_tile_mfma(const BJob &J, uint32_t tindex, char *lds, const
     const uint64_t n = J.n;
     u16x8 *ltab = reinterpret_cast<u16x8 *>(lds);
     float *lnorm = reinterpret_cast<float *>(lds + Lo.o_norm);
-    const uint32_t R = J.rep, rk = R * k;
-    // this column's replica of its wave's table: rows r*k + c
-    const uint32_t roff = (q & (R - 1)) * k;
-    float *T = reinterpret_cast<float *>(lds + Lo.o_tabs) + (size_t)wave * Lo.kmax * Lo.pmax + (size_t)roff * P;
-    uint32_t *M = reinterpret_cast<uint32_t *>(lds + Lo.o_mass) + (size_t)wave * Lo.kmax + roff;
-    uint32_t *owner = reinterpret_cast<uint32_t *>(lds + Lo.o_owner) + roff;
+    const uint32_t ts = table_stride(d);
+    float *tabs = reinterpret_cast<float *>(lds + Lo.o_tabs);
+    float *T = tabs + (size_t)wave * k * ts;
     uint32_t *pl = reinterpret_cast<uint32_t *>(lds + Lo.o_pairs) + (size_t)wave * kBatchPairs;
     unsigned long long *wmin = reinterpret_cast<unsigned long long *>(lds + Lo.o_min) + (size_t)wave * 64;
     uint32_t *pc = reinterpret_cast<uint32_t *>(lds + Lo.o_count) + wave;
@@ -332,14 +425,8 @@ __device__ void batch_tile_mfma(const BJob &J, uint32_t tindex, char *lds, const
         ltab[e] = J.table[e];
     for (uint32_t c = tid; c < nt16 * kTile; c += blockDim.x)
         lnorm[c] = J.norm[c];
-    for (uint32_t w = 0; w < waves; ++w) {
-        float *Tw = reinterpret_cast<float *>(lds + Lo.o_tabs) + (size_t)w * Lo.kmax * Lo.pmax;
-        uint32_t *Mw = reinterpret_cast<uint32_t *>(lds + Lo.o_mass) + (size_t)w * Lo.kmax;
-        for (uint32_t i = tid; i < rk * P; i += blockDim.x)
-            Tw[i] = 0.0f;
-        for (uint32_t i = tid; i < rk; i += blockDim.x)
-            Mw[i] = 0u;
-    }
+    for (uint32_t i = tid; i < waves * k * ts; i += blockDim.x)
+        tabs[i] = 0.0f;
     __syncthreads();
     const bool table_ok = J.stats[2] == 0.0f;
     const uint32_t kmask = (1u << J.prm.key_bits) - 1u;
@@ -355,8 +442,8 @@ __device__ void batch_tile_mfma(const BJob &J, uint32_t tindex, char *lds, const
             pid[g] = p < end ? p : ~0ull;
         }
         PointOps<NKB> x;
-        float X2[kGroups], RX2[kGroups], raw[kGroups][NKB * 8];
-        load_points_keep<FP16, NKB, false>(x, X2, RX2, J.pts, n, d, pid, q, h, raw);
+        float X2[kGroups], RX2[kGroups];
+        load_points<FP16, NKB, false>(x, X2, RX2, J.pts, n, d, pid, q, h);
         const WaveBounds w = wave_bounds<NKB>(X2, RX2, J.stats, J.prm);
         uint32_t best[kGroups], second[kGroups];
         rank_all<FP16, NKB>(ltab, lnorm, nt16, lane, h, w.K, kmask, x, best, second);
@@ -438,111 +525,66 @@ __device__ void batch_tile_mfma(const BJob &J, uint32_t tindex, char *lds, const
             }
         if (mp < n)
             J.labels[mp] = mine;
-        // accumulate: the 4 lanes of a column add their 8-feature slices
+        // accumulate: lane l holds point base + l; the points are read again
+        // in the sorted order (cache hits) and summed along their runs
+        const WaveRuns r = wave_runs(mine, mp < n, lane);
+        float *row = T + (size_t)(r.last ? r.label : 0u) * ts;
+        const float *src = J.pts + (r.label != kNoLabel ? base + r.src : first);
+        float xv[P];
 #pragma unroll
-        for (int g = 0; g < kGroups; ++g) {
-            const uint32_t L = lab[g];
-            const uint32_t id = wave * 64 + g * 16 + q;
-            bool pending = pid[g] < n;
-            while (__ballot(pending) != 0) {
-                if (pending)
-                    owner[L] = id;
-                wave_lds_order();
-                const bool win = pending && owner[L] == id;
-                if (win) {
-                    float *row = T + (size_t)L * P + 8 * h;
+        for (int f = 0; f < (int)P; ++f)
+            xv[f] = (f < (int)d) ? src[(uint64_t)f * n] : 0.0f;
 #pragma unroll
-                    for (int kb = 0; kb < NKB; ++kb) {
-                        f32v4 *r4 = reinterpret_cast<f32v4 *>(row + kb * kKBlock);
-                        r4[0] = r4[0] + f32v4{raw[g][kb * 8 + 0], raw[g][kb * 8 + 1], raw[g][kb * 8 + 2],
-                                              raw[g][kb * 8 + 3]};
-                        r4[1] = r4[1] + f32v4{raw[g][kb * 8 + 4], raw[g][kb * 8 + 5], raw[g][kb * 8 + 6],
-                                              raw[g][kb * 8 + 7]};
-                    }
-                    if (h == 0)
-                        M[L] = M[L] + 1u;
-                }
-                wave_lds_order();
-                pending = pending && !win;
+        for (int f = 0; f < (int)P; ++f)
+            if (f < (int)d) {
+                const float v = run_sum(xv[f], r);
+                if (r.last)
+                    row[f] += v;
             }
-        }
+        if (r.last)
+            row[d] += (float)r.count;
+        wave_lds_order();
     }
     __syncthreads();
-    double *prow = J.part + (size_t)tindex * ((size_t)k * (d + 1));
-    for (uint32_t i = tid; i < k * d; i += blockDim.x) {
-        const uint32_t f = i / k, c = i % k;
-        double sum = 0.0;
-        for (uint32_t w = 0; w < waves; ++w) {
-            const float *Tw = reinterpret_cast<float *>(lds + Lo.o_tabs) + (size_t)w * Lo.kmax * Lo.pmax;
-            for (uint32_t r = 0; r < R; ++r)
-                sum += (double)Tw[((size_t)r * k + c) * P + f];
-        }
-        prow[i] = sum;
-    }
-    for (uint32_t c = tid; c < k; c += blockDim.x) {
-        uint32_t m = 0;
-        for (uint32_t w = 0; w < waves; ++w) {
-            const uint32_t *Mw = reinterpret_cast<uint32_t *>(lds + Lo.o_mass) + (size_t)w * Lo.kmax;
-            for (uint32_t r = 0; r < R; ++r)
-                m += Mw[(size_t)r * k + c];
-        }
-        prow[(size_t)k * d + c] = (double)m;
-    }
+    write_partial_row(tabs, waves, k, d, k, J.part + (size_t)tindex * ((size_t)k * (d + 1)));
 }
 
-template <bool FP16>
+// Tiles of the jobs of one size class (MFMA labeling, NKB K-blocks).
+template <bool FP16, int NKB>
 __global__ void __launch_bounds__(64 * kBatchMaxWaves)
-batch_step_mfma_kernel(BJob *__restrict__ jobs, const BTile *__restrict__ tiles, BCtl *__restrict__ ctl,
-                       BLayout Lo)
+batch_tiles_mfma_kernel(const BJob *__restrict__ jobs, const BTile *__restrict__ tiles, BCtl *__restrict__ ctl,
+                        BLayout Lo)
 {
     extern __shared__ __align__(16) char lds_b[];
-    __shared__ uint32_t s_item;
-    __shared__ int s_last;
-    __shared__ double s_red[64 * kBatchMaxWaves];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t ntiles = ctl->ntiles;
-    const bool dry = ctl->dry != 0;
-    for (;;) {
+    for_each_tile(jobs, tiles, ctl, [&](const BJob &J, uint32_t index) {
+        batch_tile_mfma<FP16, NKB>(J, index, lds_b, Lo);
         __syncthreads();
-        if (tid == 0)
-            s_item = atomicAdd(&ctl->next, 1u);
+    });
+}
+
+constexpr uint32_t kClasses = 14;  // 5 d buckets x 2 k ranges (FP32) + 2 K-block counts x 2 k ranges (MFMA)
+
+// One work-group per active job: merge, divide, advance, and for a job with
+// the MFMA labeling the next iteration's 16-bit table.  Work-group 0 resets
+// the tile counters of every class (the tile kernels have finished).
+template <bool FP16>
+__global__ void __launch_bounds__(256)
+batch_merge_kernel(BJob *__restrict__ jobs, const uint32_t *__restrict__ active, BCtl *__restrict__ ctl)
+{
+    __shared__ double red[256];
+    if (blockIdx.x == 0 && threadIdx.x < kClasses)
+        ctl[threadIdx.x].next = 0;
+    if (ctl[0].dry != 0)
+        return;
+    BJob &J = jobs[active[blockIdx.x]];
+    if (J.state != CLK_JOB_ACTIVE)
+        return;
+    const uint32_t nxt = (J.iter & 1) ^ 1;
+    const bool table = J.kind == 1;
+    batch_finalize(J, red);
+    if (table) {
         __syncthreads();
-        const uint32_t item = s_item;
-        if (item >= ntiles)
-            break;
-        const BTile t = tiles[item];
-        BJob &J = jobs[t.job];
-        if (J.state != CLK_JOB_ACTIVE)
-            continue;
-        if (J.nkb == 1)
-            batch_tile_mfma<FP16, 1>(J, t.index, lds_b, Lo);
-        else
-            batch_tile_mfma<FP16, 2>(J, t.index, lds_b, Lo);
-        __threadfence();
-        __syncthreads();
-        if (tid == 0)
-            s_last = atomicAdd(&J.done, 1u) == J.ntiles - 1;
-        __syncthreads();
-        if (s_last) {
-            __threadfence();
-            if (dry) {
-                if (tid == 0)
-                    J.done = 0;
-            } else {
-                const uint32_t nxt = (J.iter & 1) ^ 1;
-                batch_finalize(J, s_red);
-                __syncthreads();
-                mfma::build_table_block<FP16>(J.cen[nxt], J.d, J.k, J.nt16, J.nkb, J.table, J.norm,
-                                              J.stats, s_red);
-            }
-        }
-    }
-    if (tid == 0) {
-        __threadfence();
-        if (atomicAdd(&ctl->exited, 1u) == gridDim.x - 1) {
-            ctl->next = 0;
-            ctl->exited = 0;
-        }
+        mfma::build_table_block<FP16>(J.cen[nxt], J.d, J.k, J.nt16, J.nkb, J.table, J.norm, J.stats, red);
     }
 }
 
@@ -572,17 +614,24 @@ using clk::BCtl;
 using clk::BJob;
 using clk::BTile;
 
+// A size class: the jobs of one tile-kernel instantiation.  Its LDS and
+// work-group size follow the largest k and d among its active jobs.
+struct BClassState {
+    uint32_t tile0 = 0, ntiles = 0, grid = 0, waves = 0;
+    size_t lds = 0;
+    clk::BLayout layout{};
+};
+
 struct clk_batch {
     clk_context *ctx = nullptr;
     clk_batch_config cfg{};
-    uint32_t waves = 1;
-    size_t lds = 0;
     bool mfma = false, fp16 = false;
-    clk::BLayout layout{};
     BJob *d_jobs = nullptr;
-    BCtl *d_ctl = nullptr;
+    BCtl *d_ctl = nullptr;  // one per class
     BTile *d_tiles = nullptr;
     uint32_t tile_cap = 0;
+    uint32_t *d_active = nullptr;  // slots of the active jobs (merge work-groups)
+    uint32_t nactive = 0;
     char *arena = nullptr;
     // host mirror of the job table (device-side fields iter/state/shift are
     // read back by poll / take)
@@ -593,7 +642,7 @@ struct clk_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
-    uint32_t grid = 0;
+    BClassState cls[clk::kClasses];
 };
 
 namespace {
@@ -645,16 +694,103 @@ bool arena_alloc(clk_batch *b, uint64_t need, uint64_t *where)
     return true;
 }
 
+// Jobs with few clusters label faster with the FP32 scan than on the matrix
+// cores (the 16-bit path has a fixed cost per point: conversion, bounds,
+// margin test); the labels are the same either way.
+bool use_mfma(const clk_batch *b, uint32_t d, uint32_t k)
+{
+    return b->mfma && k >= 16 && (uint64_t)k * d >= 1024;
+}
+
+constexpr uint32_t kSmallK = 64;
+
+uint32_t class_of(uint32_t kind, uint32_t d, uint32_t k)
+{
+    const uint32_t big = k > kSmallK;
+    if (kind == 0) {
+        const uint32_t dm = clk::bucket_of(d);  // 4 .. 64
+        const uint32_t i = dm == 4 ? 0 : dm == 8 ? 1 : dm == 16 ? 2 : dm == 32 ? 3 : 4;
+        return 2 * i + big;
+    }
+    return 10 + 2 * (clk::div_up(d, clk::mfma::kKBlock) - 1) + big;
+}
+
+const void *class_kernel(const clk_batch *b, uint32_t c)
+{
+    if (c < 10) {
+        switch (c / 2) {
+        case 0: return (const void *)clk::batch_tiles_kernel<4>;
+        case 1: return (const void *)clk::batch_tiles_kernel<8>;
+        case 2: return (const void *)clk::batch_tiles_kernel<16>;
+        case 3: return (const void *)clk::batch_tiles_kernel<32>;
+        default: return (const void *)clk::batch_tiles_kernel<64>;
+        }
+    }
+    const bool nkb2 = c >= 12;
+    if (b->fp16)
+        return nkb2 ? (const void *)clk::batch_tiles_mfma_kernel<true, 2>
+                    : (const void *)clk::batch_tiles_mfma_kernel<true, 1>;
+    return nkb2 ? (const void *)clk::batch_tiles_mfma_kernel<false, 2>
+                : (const void *)clk::batch_tiles_mfma_kernel<false, 1>;
+}
+
+// LDS of class c for jobs up to kmax clusters and dmax features with the
+// given waves per work-group.
+size_t class_lds(uint32_t c, uint32_t waves, uint32_t kmax, uint32_t dmax, clk::BLayout *layout)
+{
+    if (c < 10)
+        return clk::batch_lds_bytes(waves, kmax, 4u << (c / 2), dmax);
+    const clk::BLayout L = clk::mfma_layout_for(waves, kmax, dmax);
+    if (layout)
+        *layout = L;
+    return L.bytes;
+}
+
+// The most waves per work-group (<= 4) whose LDS fits; 0 if none does.
+uint32_t class_waves(const clk_batch *b, uint32_t c, uint32_t kmax, uint32_t dmax)
+{
+    for (uint32_t w = clk::kBatchMaxWaves; w >= 1; --w)
+        if (class_lds(c, w, kmax, dmax, nullptr) + 4096 <= b->ctx->lds_per_block)  // + static LDS
+            return w;
+    return 0;
+}
+
 int rebuild(clk_batch *b, hipStream_t stream)
 {
     std::vector<BTile> tiles;
-    for (uint32_t s = 0; s < b->jobs.size(); ++s) {
-        BJob &J = b->jobs[s];
-        if (J.state != CLK_JOB_ACTIVE)
+    std::vector<uint32_t> active;
+    std::vector<BCtl> ctl(clk::kClasses, BCtl{0, 0, 0, 0});
+    for (uint32_t c = 0; c < clk::kClasses; ++c) {
+        BClassState &C = b->cls[c];
+        C = BClassState{};
+        C.tile0 = (uint32_t)tiles.size();
+        uint32_t kmax = 0, dmax = 0;
+        for (uint32_t s = 0; s < b->jobs.size(); ++s) {
+            BJob &J = b->jobs[s];
+            if (J.state != CLK_JOB_ACTIVE || class_of(J.kind, J.d, J.k) != c)
+                continue;
+            active.push_back(s);
+            kmax = std::max(kmax, J.k);
+            dmax = std::max(dmax, J.d);
+            J.tile0 = (uint32_t)tiles.size();
+            for (uint32_t t = 0; t < J.ntiles; ++t)
+                tiles.push_back({s, t});
+        }
+        C.ntiles = (uint32_t)tiles.size() - C.tile0;
+        ctl[c].ntiles = C.ntiles;
+        if (C.ntiles == 0)
             continue;
-        J.tile0 = (uint32_t)tiles.size();
-        for (uint32_t t = 0; t < J.ntiles; ++t)
-            tiles.push_back({s, t});
+        C.waves = class_waves(b, c, kmax, dmax);
+        if (C.waves == 0) {  // admitted jobs fit by construction (clk_batch_create checks max_k x max_d)
+            clk::set_error("clk_batch: a size class does not fit the work-group's LDS");
+            return CLK_ERR_UNSUPPORTED;
+        }
+        C.lds = class_lds(c, C.waves, kmax, dmax, &C.layout);
+        const void *kern = class_kernel(b, c);
+        if (C.lds > 65536)
+            CLK_HIP_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)C.lds));
+        C.grid = (uint32_t)clk::auto_grid(b->ctx, kern, (int)(64 * C.waves), C.lds,
+                                          (uint64_t)C.ntiles * 64 * C.waves);
     }
     if (tiles.size() > b->tile_cap) {
         if (b->d_tiles)
@@ -665,15 +801,12 @@ int rebuild(clk_batch *b, hipStream_t stream)
     if (!tiles.empty())
         CLK_HIP_CHECK(hipMemcpyAsync(b->d_tiles, tiles.data(), tiles.size() * sizeof(BTile),
                                      hipMemcpyHostToDevice, stream));
-    BCtl ctl{0, 0, (uint32_t)tiles.size(), 0};
-    CLK_HIP_CHECK(hipMemcpyAsync(b->d_ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice, stream));
+    if (!active.empty())
+        CLK_HIP_CHECK(hipMemcpyAsync(b->d_active, active.data(), active.size() * sizeof(uint32_t),
+                                     hipMemcpyHostToDevice, stream));
+    b->nactive = (uint32_t)active.size();
+    CLK_HIP_CHECK(hipMemcpyAsync(b->d_ctl, ctl.data(), ctl.size() * sizeof(BCtl), hipMemcpyHostToDevice, stream));
     CLK_HIP_CHECK(hipStreamSynchronize(stream));  // tiles / ctl are host temporaries
-    const void *kern = !b->mfma ? (const void *)clk::batch_step_kernel
-                     : b->fp16  ? (const void *)clk::batch_step_mfma_kernel<true>
-                                : (const void *)clk::batch_step_mfma_kernel<false>;
-    b->grid = tiles.empty() ? 0
-                            : (uint32_t)clk::auto_grid(b->ctx, kern, (int)(64 * b->waves), b->lds,
-                                                       (uint64_t)tiles.size() * 64 * b->waves);
     if (b->exec) {
         (void)hipGraphExecDestroy(b->exec);
         b->exec = nullptr;
@@ -686,17 +819,47 @@ int rebuild(clk_batch *b, hipStream_t stream)
     return CLK_OK;
 }
 
+// The tile kernel of every non-empty class, then the merge kernel (which
+// resets the tile counters).
 int launch_step(clk_batch *b, hipStream_t stream)
 {
-    if (!b->mfma)
-        hipLaunchKernelGGL(clk::batch_step_kernel, dim3(b->grid), dim3(64 * b->waves), b->lds, stream,
-                           b->d_jobs, b->d_tiles, b->d_ctl);
-    else if (b->fp16)
-        hipLaunchKernelGGL(clk::batch_step_mfma_kernel<true>, dim3(b->grid), dim3(64 * b->waves), b->lds,
-                           stream, b->d_jobs, b->d_tiles, b->d_ctl, b->layout);
+    for (uint32_t c = 0; c < clk::kClasses; ++c) {
+        const BClassState &C = b->cls[c];
+        if (C.ntiles == 0)
+            continue;
+        const BTile *tl = b->d_tiles + C.tile0;
+        BCtl *ctl = b->d_ctl + c;
+        const dim3 g(C.grid), t(64 * C.waves);
+        if (c < 10) {
+            switch (c / 2) {
+            case 0: hipLaunchKernelGGL(clk::batch_tiles_kernel<4>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
+            case 1: hipLaunchKernelGGL(clk::batch_tiles_kernel<8>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
+            case 2: hipLaunchKernelGGL(clk::batch_tiles_kernel<16>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
+            case 3: hipLaunchKernelGGL(clk::batch_tiles_kernel<32>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
+            default: hipLaunchKernelGGL(clk::batch_tiles_kernel<64>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
+            }
+        } else if (b->fp16) {
+            if (c >= 12)
+                hipLaunchKernelGGL((clk::batch_tiles_mfma_kernel<true, 2>), g, t, C.lds, stream, b->d_jobs, tl,
+                                   ctl, C.layout);
+            else
+                hipLaunchKernelGGL((clk::batch_tiles_mfma_kernel<true, 1>), g, t, C.lds, stream, b->d_jobs, tl,
+                                   ctl, C.layout);
+        } else {
+            if (c >= 12)
+                hipLaunchKernelGGL((clk::batch_tiles_mfma_kernel<false, 2>), g, t, C.lds, stream, b->d_jobs, tl,
+                                   ctl, C.layout);
+            else
+                hipLaunchKernelGGL((clk::batch_tiles_mfma_kernel<false, 1>), g, t, C.lds, stream, b->d_jobs, tl,
+                                   ctl, C.layout);
+        }
+        CLK_HIP_CHECK(hipGetLastError());
+    }
+    const dim3 mg(std::max<uint32_t>(b->nactive, 1)), mb(256);
+    if (b->fp16)
+        hipLaunchKernelGGL(clk::batch_merge_kernel<true>, mg, mb, 0, stream, b->d_jobs, b->d_active, b->d_ctl);
     else
-        hipLaunchKernelGGL(clk::batch_step_mfma_kernel<false>, dim3(b->grid), dim3(64 * b->waves), b->lds,
-                           stream, b->d_jobs, b->d_tiles, b->d_ctl, b->layout);
+        hipLaunchKernelGGL(clk::batch_merge_kernel<false>, mg, mb, 0, stream, b->d_jobs, b->d_active, b->d_ctl);
     CLK_HIP_CHECK(hipGetLastError());
     return CLK_OK;
 }
@@ -714,8 +877,8 @@ extern "C" int clk_batch_create(clk_context *ctx, const clk_batch_config *cfg, c
     CLK_TRY(clk::enter(ctx));
     CLK_REQUIRE(cfg && out, CLK_ERR_INVALID_ARGUMENT, "clk_batch_create: null argument");
     CLK_REQUIRE(cfg->max_jobs >= 1 && cfg->arena_bytes > 0, CLK_ERR_SIZE, "clk_batch_create: empty batch");
-    CLK_REQUIRE(cfg->max_d >= 1 && cfg->max_d <= 64 && cfg->max_k >= 1, CLK_ERR_SIZE,
-                "clk_batch_create: max_d must be 1..64, max_k >= 1");
+    CLK_REQUIRE(cfg->max_d >= 1 && cfg->max_d <= 64 && cfg->max_k >= 1 && cfg->max_k <= (1u << 20),
+                CLK_ERR_SIZE, "clk_batch_create: max_d must be 1..64, max_k 1..2^20");
     *out = nullptr;
     clk_batch *b = new clk_batch;
     b->ctx = ctx;
@@ -726,57 +889,41 @@ extern "C" int clk_batch_create(clk_context *ctx, const clk_batch_config *cfg, c
                 CLK_ERR_INVALID_ARGUMENT, "clk_batch_create: unknown labeling strategy id");
     b->mfma = cfg->labeling != CLK_LABELING_UNROLL_VECTOR;
     b->fp16 = cfg->labeling == CLK_LABELING_MFMA_FP16;
-    const uint32_t dm = clk::bucket_of(cfg->max_d);
-    b->waves = 0;
-    for (uint32_t w = clk::kBatchMaxWaves; w >= 1; --w) {
-        const size_t need = b->mfma ? clk::mfma_layout_for(w, cfg->max_k, cfg->max_d).bytes
-                                    : clk::batch_lds_bytes(w, cfg->max_k, dm);
-        if (need + 4096 <= ctx->lds_per_block) {  // + the kernel's static LDS
-            b->waves = w;
-            break;
+    // every class a job up to max_k x max_d can land in must fit one wave
+    {
+        bool fits = true;
+        for (uint32_t kind = 0; kind <= (b->mfma ? 1u : 0u); ++kind)
+            for (uint32_t k : {std::min(cfg->max_k, kSmallK), cfg->max_k}) {
+                const uint32_t c = class_of(kind, cfg->max_d, k);
+                fits = fits && class_waves(b, c, k, cfg->max_d) > 0;
+            }
+        if (!fits) {
+            delete b;
+            clk::set_error("clk_batch_create: max_k x max_d too large for the work-group's LDS");
+            return CLK_ERR_UNSUPPORTED;
         }
     }
-    if (b->waves == 0) {
-        delete b;
-        clk::set_error("clk_batch_create: max_k x max_d too large for the work-group's LDS");
-        return CLK_ERR_UNSUPPORTED;
-    }
     if (b->mfma) {
-        b->layout = clk::mfma_layout_for(b->waves, cfg->max_k, cfg->max_d);
-        b->lds = b->layout.bytes;
         const int rc = clk::mfma_subnormal_flags(ctx, nullptr);
         if (rc != CLK_OK) {
             delete b;
             return rc;
-        }
-    } else {
-        b->lds = clk::batch_lds_bytes(b->waves, cfg->max_k, dm);
-    }
-    if (b->lds > 65536) {
-        const void *kern = !b->mfma ? (const void *)clk::batch_step_kernel
-                         : b->fp16  ? (const void *)clk::batch_step_mfma_kernel<true>
-                                    : (const void *)clk::batch_step_mfma_kernel<false>;
-        hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds);
-        if (e != hipSuccess) {
-            delete b;
-            clk::set_error(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-            return CLK_ERR_HIP;
         }
     }
     b->jobs.assign(cfg->max_jobs, BJob{});
     b->off.assign(cfg->max_jobs, 0);
     b->bytes.assign(cfg->max_jobs, 0);
     hipError_t e1 = hipMalloc(&b->d_jobs, cfg->max_jobs * sizeof(BJob));
-    hipError_t e2 = hipMalloc(&b->d_ctl, sizeof(BCtl));
+    hipError_t e2 = hipMalloc(&b->d_ctl, clk::kClasses * sizeof(BCtl));
     hipError_t e3 = hipMalloc(&b->arena, cfg->arena_bytes);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+    hipError_t e4 = hipMalloc(&b->d_active, cfg->max_jobs * sizeof(uint32_t));
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
         clk_batch_destroy(b);
         clk::set_error("clk_batch_create: device allocation failed");
         return CLK_ERR_NOMEM;
     }
     CLK_HIP_CHECK(hipMemset(b->d_jobs, 0, cfg->max_jobs * sizeof(BJob)));
-    BCtl ctl{0, 0, 0, 0};
-    CLK_HIP_CHECK(hipMemcpy(b->d_ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice));
+    CLK_HIP_CHECK(hipMemset(b->d_ctl, 0, clk::kClasses * sizeof(BCtl)));
     *out = b;
     return CLK_OK;
 }
@@ -797,6 +944,8 @@ extern "C" int clk_batch_destroy(clk_batch *b)
         (void)hipFree(b->d_ctl);
     if (b->d_tiles)
         (void)hipFree(b->d_tiles);
+    if (b->d_active)
+        (void)hipFree(b->d_active);
     if (b->arena)
         (void)hipFree(b->arena);
     delete b;
@@ -843,16 +992,6 @@ extern "C" int clk_batch_add(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, c
     J.stats = reinterpret_cast<float *>(p);
     p += 256;
     J.prm = clk::mfma::make_params(b->fp16, d, k, (b->ctx->mfma_subnormals & (b->fp16 ? 1 : 2)) != 0);
-    {
-        // replicas: R*k rows must fit the rows and cells sized for max_k x max_d
-        const uint64_t p_job = b->mfma ? (uint64_t)J.nkb * clk::mfma::kKBlock : clk::bucket_of(d);
-        const uint64_t p_max = b->mfma ? (uint64_t)b->layout.pmax : clk::bucket_of(b->cfg.max_d);
-        const uint64_t lim = std::min<uint64_t>(std::min<uint64_t>(16, b->cfg.max_k / k),
-                                                (uint64_t)b->cfg.max_k * p_max / ((uint64_t)k * p_job));
-        J.rep = 1;
-        while (J.rep * 2 <= lim)
-            J.rep *= 2;
-    }
     J.pts = reinterpret_cast<const float *>(p);
     p += clk::align_up(n * d * 4, 256);
     J.labels = reinterpret_cast<uint32_t *>(p);
@@ -871,9 +1010,9 @@ extern "C" int clk_batch_add(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, c
     J.ntiles = (uint32_t)clk::div_up(n, tp);
     J.max_iter = max_iterations;
     J.iter = 0;
-    J.done = 0;
     J.state = max_iterations > 0 ? CLK_JOB_ACTIVE : CLK_JOB_DONE;
     J.dmax = clk::bucket_of(d);
+    J.kind = use_mfma(b, d, k) ? 1u : 0u;
     J.tol = tolerance;
     J.shift = 0.0;
     CLK_HIP_CHECK(hipMemcpyAsync(const_cast<float *>(J.pts), host_points, n * d * 4, hipMemcpyHostToDevice, st));
@@ -881,7 +1020,7 @@ extern "C" int clk_batch_add(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, c
     CLK_HIP_CHECK(hipMemsetAsync(J.masses, 0, (size_t)k * 4, st));
     CLK_HIP_CHECK(hipMemsetAsync(J.labels, 0, n * 4, st));
     CLK_HIP_CHECK(hipMemcpyAsync(b->d_jobs + s, &J, sizeof(BJob), hipMemcpyHostToDevice, st));
-    if (b->mfma) {
+    if (J.kind == 1) {
         if (b->fp16)
             hipLaunchKernelGGL(clk::batch_build_table_kernel<true>, dim3(1), dim3(256), 0, st, b->d_jobs,
                                (uint32_t)s);
@@ -906,7 +1045,7 @@ extern "C" int clk_batch_step(clk_batch *b, void *stream)
     hipStream_t st = clk::as_stream(stream);
     if (b->dirty)
         CLK_TRY(rebuild(b, st));
-    if (b->grid == 0)
+    if (b->nactive == 0)
         return CLK_OK;
     if (!b->exec || b->graph_stream != st) {
         if (b->exec) {
