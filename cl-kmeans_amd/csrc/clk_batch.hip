@@ -86,8 +86,8 @@ constexpr uint32_t kBatchPairs = 256;  // candidate pairs per wave and flush (MF
 __host__ __device__ inline uint32_t table_stride(uint32_t d) { return (d + 1) | 1u; }
 
 // LDS of the MFMA batch kernel for the largest admitted job (byte offsets):
-// table | norms | W tables [kmax][ts(dmax)] | W pair lists | W running
-// minima [64] | W pair counters
+// table | norms | the work-group's table [kmax][ts(dmax)] | W pair lists |
+// W running minima [64] | W pair counters
 struct BLayout {
     uint32_t kmax, waves;
     uint32_t o_norm, o_tabs, o_pairs, o_min, o_count, bytes;
@@ -104,7 +104,7 @@ inline BLayout mfma_layout_for(uint32_t waves, uint32_t kmax, uint32_t dmax_feat
     L.o_norm = (uint32_t)o;
     o += (size_t)nt16 * mfma::kTile * 4;
     L.o_tabs = (uint32_t)o;
-    o += (size_t)waves * kmax * table_stride(dmax_feat) * 4;
+    o += (size_t)kmax * table_stride(dmax_feat) * 4;
     L.o_pairs = (uint32_t)o;
     o += (size_t)waves * kBatchPairs * 4;
     o = (o + 15) / 16 * 16;
@@ -126,30 +126,64 @@ __host__ __device__ inline size_t batch_lds_bytes(uint32_t waves, uint32_t k, ui
 
 constexpr uint32_t kNoLabel = 0x3FFFFFFu;  // sorts after every label (< 2^26)
 
-// Bitonic sort of one 32-bit key per lane, ascending in lane order.
-__device__ __forceinline__ uint32_t wave_sort(uint32_t key, uint32_t lane)
-{
-#pragma unroll
-    for (uint32_t size = 2; size <= 64; size *= 2)
-#pragma unroll
-        for (uint32_t j = size / 2; j > 0; j /= 2) {
-            const uint32_t other = (uint32_t)__shfl_xor((int)key, (int)j);
-            const bool desc = (lane & size) != 0;
-            const bool lower = (lane & j) == 0;
-            key = (lower != desc) ? min(key, other) : max(key, other);
-        }
-    return key;
-}
-
-// DPP controls: row_shr:n (n = 1..15) within rows of 16 lanes; row_bcast:15
-// (the last lane of each row to the next row) and row_bcast:31 (lane 31 to
-// rows 2 and 3).
-constexpr int kDppRowShr = 0x110, kDppBcast15 = 0x142, kDppBcast31 = 0x143;
+// DPP controls: quad_perm [1,0,3,2] and [2,3,0,1] (lane xor 1, xor 2);
+// row_shl:n / row_shr:n (lane i reads lane i+n / i-n within its row of 16);
+// row_ror:8 (lane xor 8 within the row); row_bcast:15 (the last lane of each
+// row to the next row) and row_bcast:31 (lane 31 to rows 2 and 3).
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppRowShl = 0x100, kDppRowShr = 0x110, kDppXor8 = 0x128,
+              kDppBcast15 = 0x142, kDppBcast31 = 0x143;
 
 template <int CTRL, int ROWS>
 __device__ __forceinline__ uint32_t dpp_move(uint32_t v)
 {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, false);
+}
+
+// The value of lane (lane xor J): DPP within rows of 16, LDS permutes across.
+template <uint32_t J>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v, uint32_t lane)
+{
+    if constexpr (J == 1)
+        return dpp_move<kDppXor1, 0xF>(v);
+    else if constexpr (J == 2)
+        return dpp_move<kDppXor2, 0xF>(v);
+    else if constexpr (J == 4) {
+        const uint32_t up = dpp_move<kDppRowShl + 4, 0xF>(v), down = dpp_move<kDppRowShr + 4, 0xF>(v);
+        return (lane & 4u) ? down : up;
+    } else if constexpr (J == 8)
+        return dpp_move<kDppXor8, 0xF>(v);
+    else
+        return (uint32_t)__shfl_xor((int)v, (int)J);
+}
+
+// One compare-exchange step of the bitonic network (block SIZE, stride J).
+template <uint32_t SIZE, uint32_t J>
+__device__ __forceinline__ uint32_t bitonic_step(uint32_t key, uint32_t lane)
+{
+    const uint32_t other = lane_xor<J>(key, lane);
+    const bool desc = (lane & SIZE) != 0;
+    const bool lower = (lane & J) == 0;
+    return (lower != desc) ? min(key, other) : max(key, other);
+}
+
+template <uint32_t SIZE, uint32_t J>
+__device__ __forceinline__ uint32_t bitonic_merge(uint32_t key, uint32_t lane)
+{
+    key = bitonic_step<SIZE, J>(key, lane);
+    if constexpr (J > 1)
+        key = bitonic_merge<SIZE, J / 2>(key, lane);
+    return key;
+}
+
+// Bitonic sort of one 32-bit key per lane, ascending in lane order.
+__device__ __forceinline__ uint32_t wave_sort(uint32_t key, uint32_t lane)
+{
+    key = bitonic_merge<2, 1>(key, lane);
+    key = bitonic_merge<4, 2>(key, lane);
+    key = bitonic_merge<8, 4>(key, lane);
+    key = bitonic_merge<16, 8>(key, lane);
+    key = bitonic_merge<32, 16>(key, lane);
+    return bitonic_merge<64, 32>(key, lane);
 }
 
 // The wave's points sorted by label.  Lane i holds the point of lane src;
@@ -161,6 +195,7 @@ struct WaveRuns {
     uint32_t label, src, count;
     bool last;
     bool step[6];
+    uint32_t steps;  // wave-uniform: bit s set if some run needs scan step s
 };
 
 __device__ __forceinline__ WaveRuns wave_runs(uint32_t label, bool valid, uint32_t lane)
@@ -169,12 +204,6 @@ __device__ __forceinline__ WaveRuns wave_runs(uint32_t label, bool valid, uint32
     WaveRuns r;
     r.label = key >> 6;
     r.src = key & 63u;
-    const uint32_t prev = (uint32_t)__shfl((int)r.label, (int)((lane + 63) & 63));
-    const uint64_t heads = __ballot(lane == 0 || prev != r.label);
-    const bool run_end = lane == 63 || (((heads >> lane) >> 1) & 1ull) != 0;
-    r.last = run_end && r.label != kNoLabel;
-    const uint64_t upto = heads & ((2ull << lane) - 1ull);  // lanes 0..lane
-    r.count = lane - (63u - (uint32_t)__builtin_clzll(upto)) + 1u;
     // every lane takes part in every move (a lane that sits out would feed
     // the old value to its neighbours)
     const uint32_t l1 = dpp_move<kDppRowShr + 1, 0xF>(r.label);
@@ -184,6 +213,20 @@ __device__ __forceinline__ WaveRuns wave_runs(uint32_t label, bool valid, uint32
     const uint32_t l15 = dpp_move<kDppBcast15, 0xA>(r.label);
     const uint32_t l31 = dpp_move<kDppBcast31, 0xC>(r.label);
     const uint32_t row_lane = lane & 15u, row = lane >> 4;
+    // label of lane - 1: in the row, else the previous row's last lane
+    const uint32_t prev = row_lane != 0 ? l1 : (row & 1u) != 0 ? l15 : l31;
+    const uint64_t heads = __ballot(lane == 0 || prev != r.label);
+    const bool run_end = lane == 63 || (((heads >> lane) >> 1) & 1ull) != 0;
+    r.last = run_end && r.label != kNoLabel;
+    const uint64_t upto = heads & ((2ull << lane) - 1ull);  // lanes 0..lane
+    r.count = lane - (63u - (uint32_t)__builtin_clzll(upto)) + 1u;
+    // in-row step s (shift 2^s) matters only if a run is longer than 2^s;
+    // the row broadcasts only if a run crosses a row boundary
+    const uint32_t len = r.last ? r.count : 0u;
+    const bool cross = r.last && (lane & 15u) + 1u < r.count;
+    r.steps = (__ballot(len > 1) != 0 ? 1u : 0u) | (__ballot(len > 2) != 0 ? 2u : 0u) |
+              (__ballot(len > 4) != 0 ? 4u : 0u) | (__ballot(len > 8) != 0 ? 8u : 0u) |
+              (__ballot(cross) != 0 ? 48u : 0u);
     r.step[0] = (row_lane >= 1) & (l1 == r.label);
     r.step[1] = (row_lane >= 2) & (l2 == r.label);
     r.step[2] = (row_lane >= 4) & (l4 == r.label);
@@ -195,21 +238,34 @@ __device__ __forceinline__ WaveRuns wave_runs(uint32_t label, bool valid, uint32
 
 // Inclusive scan of v along the sorted lanes within runs: the last lane of a
 // run ends with the run's sum (a fixed summation tree).
+// Steps that no run needs are skipped (wave-uniform branches): a step adds
+// only within a run, and a run of length L is complete after the in-row
+// shifts below L plus, if it crosses a row boundary, the broadcasts.
 __device__ __forceinline__ float run_sum(float v, const WaveRuns &r)
 {
     float t;
-    t = __uint_as_float(dpp_move<kDppRowShr + 1, 0xF>(__float_as_uint(v)));
-    v = r.step[0] ? v + t : v;
-    t = __uint_as_float(dpp_move<kDppRowShr + 2, 0xF>(__float_as_uint(v)));
-    v = r.step[1] ? v + t : v;
-    t = __uint_as_float(dpp_move<kDppRowShr + 4, 0xF>(__float_as_uint(v)));
-    v = r.step[2] ? v + t : v;
-    t = __uint_as_float(dpp_move<kDppRowShr + 8, 0xF>(__float_as_uint(v)));
-    v = r.step[3] ? v + t : v;
-    t = __uint_as_float(dpp_move<kDppBcast15, 0xA>(__float_as_uint(v)));
-    v = r.step[4] ? v + t : v;
-    t = __uint_as_float(dpp_move<kDppBcast31, 0xC>(__float_as_uint(v)));
-    v = r.step[5] ? v + t : v;
+    if (r.steps & 1u) {
+        t = __uint_as_float(dpp_move<kDppRowShr + 1, 0xF>(__float_as_uint(v)));
+        v = r.step[0] ? v + t : v;
+    }
+    if (r.steps & 2u) {
+        t = __uint_as_float(dpp_move<kDppRowShr + 2, 0xF>(__float_as_uint(v)));
+        v = r.step[1] ? v + t : v;
+    }
+    if (r.steps & 4u) {
+        t = __uint_as_float(dpp_move<kDppRowShr + 4, 0xF>(__float_as_uint(v)));
+        v = r.step[2] ? v + t : v;
+    }
+    if (r.steps & 8u) {
+        t = __uint_as_float(dpp_move<kDppRowShr + 8, 0xF>(__float_as_uint(v)));
+        v = r.step[3] ? v + t : v;
+    }
+    if (r.steps & 16u) {
+        t = __uint_as_float(dpp_move<kDppBcast15, 0xA>(__float_as_uint(v)));
+        v = r.step[4] ? v + t : v;
+        t = __uint_as_float(dpp_move<kDppBcast31, 0xC>(__float_as_uint(v)));
+        v = r.step[5] ? v + t : v;
+    }
     return v;
 }
 
@@ -229,7 +285,7 @@ __device__ __forceinline__ void write_partial_row(const float *tabs, uint32_t wa
 }
 
 template <int DMAX>
-__device__ void batch_tile(const BJob &J, uint32_t tindex, float *lds, uint32_t waves)
+__device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float *lds, uint32_t waves)
 {
     const uint32_t tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     const uint32_t d = J.d, k = J.k, ts = table_stride(d);
@@ -245,14 +301,26 @@ __device__ void batch_tile(const BJob &J, uint32_t tindex, float *lds, uint32_t 
     const uint64_t first = (uint64_t)tindex * J.tile_pts;
     const uint64_t end = std::min<uint64_t>(first + J.tile_pts, J.n);
     const uint64_t n = J.n;
+    // the next pass's points are loaded while this pass is labeled
+    float xn[DMAX];
+    auto fetch = [&](uint64_t b0) {
+        const uint64_t p = b0 + tid;
+        const uint64_t pc = p < end ? p : first;
+#pragma unroll
+        for (int f = 0; f < DMAX; ++f)
+            if (f < (int)d)
+                xn[f] = J.pts[(uint64_t)f * n + pc];
+    };
+    fetch(first);
     for (uint64_t base = first; base < end; base += blockDim.x) {
         const uint64_t p = base + tid;
         const bool valid = p < end;
-        const uint64_t pc = valid ? p : first;
         float x[DMAX][1];
 #pragma unroll
         for (int f = 0; f < DMAX; ++f)
-            x[f][0] = (f < (int)d) ? J.pts[(uint64_t)f * n + pc] : 0.0f;
+            x[f][0] = (f < (int)d) ? xn[f] : 0.0f;
+        if (base + blockDim.x < end)
+            fetch(base + blockDim.x);
         uint32_t lab[1];
         argmin_lds<DMAX, 1>(x, cen_s, k, lab);
         if (valid)
@@ -401,8 +469,21 @@ batch_tiles_kernel(const BJob *__restrict__ jobs, const BTile *__restrict__ tile
     });
 }
 
+// Columns S, S+4, S+8, ... of a table row: feature sums v[f] for f < d and
+// the count in column d.
+template <int P, int S>
+__device__ __forceinline__ void add_columns(float *row, const float (&v)[P], uint32_t d, uint32_t count)
+{
+#pragma unroll
+    for (int f = S; f < P; f += 4)
+        if (f < (int)d)
+            row[f] += v[f];
+    if (d % 4 == S)
+        row[d] += (float)count;
+}
+
 template <bool FP16, int NKB>
-__device__ void batch_tile_mfma(const BJob &J, uint32_t tindex, char *lds, const BLayout &Lo)
+__device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, char *lds, const BLayout &Lo)
 {
     using namespace mfma;
     constexpr uint32_t P = NKB * kKBlock;
@@ -414,8 +495,7 @@ __device__ void batch_tile_mfma(const BJob &J, uint32_t tindex, char *lds, const
     u16x8 *ltab = reinterpret_cast<u16x8 *>(lds);
     float *lnorm = reinterpret_cast<float *>(lds + Lo.o_norm);
     const uint32_t ts = table_stride(d);
-    float *tabs = reinterpret_cast<float *>(lds + Lo.o_tabs);
-    float *T = tabs + (size_t)wave * k * ts;
+    float *T = reinterpret_cast<float *>(lds + Lo.o_tabs);  // the work-group's table
     uint32_t *pl = reinterpret_cast<uint32_t *>(lds + Lo.o_pairs) + (size_t)wave * kBatchPairs;
     unsigned long long *wmin = reinterpret_cast<unsigned long long *>(lds + Lo.o_min) + (size_t)wave * 64;
     uint32_t *pc = reinterpret_cast<uint32_t *>(lds + Lo.o_count) + wave;
@@ -425,25 +505,61 @@ __device__ void batch_tile_mfma(const BJob &J, uint32_t tindex, char *lds, const
         ltab[e] = J.table[e];
     for (uint32_t c = tid; c < nt16 * kTile; c += blockDim.x)
         lnorm[c] = J.norm[c];
-    for (uint32_t i = tid; i < waves * k * ts; i += blockDim.x)
-        tabs[i] = 0.0f;
+    for (uint32_t i = tid; i < k * ts; i += blockDim.x)
+        T[i] = 0.0f;
     __syncthreads();
     const bool table_ok = J.stats[2] == 0.0f;
     const uint32_t kmask = (1u << J.prm.key_bits) - 1u;
     const uint64_t first = (uint64_t)tindex * J.tile_pts;
     const uint64_t end = std::min<uint64_t>(first + J.tile_pts, n);
     const uint64_t nb = (end - first + 63) / 64;
-    for (uint64_t bt = wave; bt < nb; bt += waves) {
-        const uint64_t base = first + bt * 64;
-        uint64_t pid[kGroups];
+    // the next batch's points are loaded (MFMA lane layout) while this batch
+    // is labeled; this batch's points are loaded again one point per lane
+    // (cache hits, for the accumulation) at the start of its labeling
+    auto batch_ids = [&](uint64_t b0, uint64_t (&pid)[kGroups]) {
 #pragma unroll
         for (int g = 0; g < kGroups; ++g) {
-            const uint64_t p = base + g * 16 + q;
+            const uint64_t p = b0 + g * 16 + q;
             pid[g] = p < end ? p : ~0ull;
         }
+    };
+    float raw[kGroups][NKB * 8];
+    auto fetch = [&](uint64_t b0) {
+        uint64_t pid[kGroups];
+        batch_ids(b0, pid);
+        fetch_points<NKB, false>(raw, J.pts, n, d, pid, h);
+    };
+    if (wave < nb)
+        fetch(first + (uint64_t)wave * 64);
+    // Rounds: in round rd wave w labels batch rd*W + w; then the waves add
+    // their run sums to the shared table in W phases, wave w owning column
+    // slice (w + phase) mod W in each phase (a fixed order per column; with
+    // 4 waves slice s holds the columns s, s+4, s+8, ...).
+    const uint32_t rounds = (uint32_t)((nb + waves - 1) / waves);
+    const uint32_t cols = d + 1, fs = (cols + waves - 1) / waves;
+    for (uint32_t rd = 0; rd < rounds; ++rd) {
+        const uint64_t bt = (uint64_t)rd * waves + wave;
+        WaveRuns r;
+        r.last = false;
+        r.label = 0;
+        r.count = 0;
+        float xc[P];
+        if (bt < nb) {
+        const uint64_t base = first + bt * 64;
+        uint64_t pid[kGroups];
+        batch_ids(base, pid);
         PointOps<NKB> x;
         float X2[kGroups], RX2[kGroups];
-        load_points<FP16, NKB, false>(x, X2, RX2, J.pts, n, d, pid, q, h);
+        build_operands<FP16, NKB, false>(raw, x, X2, RX2, n, d, pid, h);
+        {
+            const uint64_t pc = base + lane < end ? base + lane : first;
+#pragma unroll
+            for (int f = 0; f < (int)P; ++f)
+                if (f < (int)d)
+                    xc[f] = J.pts[(uint64_t)f * n + pc];
+        }
+        if (bt + waves < nb)
+            fetch(first + (bt + waves) * 64);
         const WaveBounds w = wave_bounds<NKB>(X2, RX2, J.stats, J.prm);
         uint32_t best[kGroups], second[kGroups];
         rank_all<FP16, NKB>(ltab, lnorm, nt16, lane, h, w.K, kmask, x, best, second);
@@ -525,28 +641,40 @@ __device__ void batch_tile_mfma(const BJob &J, uint32_t tindex, char *lds, const
             }
         if (mp < n)
             J.labels[mp] = mine;
-        // accumulate: lane l holds point base + l; the points are read again
-        // in the sorted order (cache hits) and summed along their runs
-        const WaveRuns r = wave_runs(mine, mp < n, lane);
-        float *row = T + (size_t)(r.last ? r.label : 0u) * ts;
-        const float *src = J.pts + (r.label != kNoLabel ? base + r.src : first);
-        float xv[P];
+        // lane l holds point base + l (xc); sorted by label, the points are
+        // summed along their runs (the sums replace xc)
+        r = wave_runs(mine, mp < n, lane);
 #pragma unroll
         for (int f = 0; f < (int)P; ++f)
-            xv[f] = (f < (int)d) ? src[(uint64_t)f * n] : 0.0f;
+            if (f < (int)d)
+                xc[f] = run_sum(__shfl(xc[f], (int)r.src), r);
+        }
+        float *row = T + (size_t)r.label * ts;
+        for (uint32_t ph = 0; ph < waves; ++ph) {
+            __syncthreads();
+            const uint32_t sl = (wave + ph) % waves;
+            if (!r.last)
+                continue;
+            if (waves == kBatchMaxWaves) {  // slice sl: columns sl, sl+4, ...
+                switch (sl) {
+                case 0: add_columns<P, 0>(row, xc, d, r.count); break;
+                case 1: add_columns<P, 1>(row, xc, d, r.count); break;
+                case 2: add_columns<P, 2>(row, xc, d, r.count); break;
+                default: add_columns<P, 3>(row, xc, d, r.count); break;
+                }
+            } else {  // contiguous slices of fs columns
+                const uint32_t lo = sl * fs, hi = min(cols, lo + fs);
 #pragma unroll
-        for (int f = 0; f < (int)P; ++f)
-            if (f < (int)d) {
-                const float v = run_sum(xv[f], r);
-                if (r.last)
-                    row[f] += v;
+                for (int f = 0; f < (int)P; ++f)
+                    if (f < (int)d && (uint32_t)f >= lo && (uint32_t)f < hi)
+                        row[f] += xc[f];
+                if (d >= lo && d < hi)
+                    row[d] += (float)r.count;
             }
-        if (r.last)
-            row[d] += (float)r.count;
-        wave_lds_order();
+        }
     }
     __syncthreads();
-    write_partial_row(tabs, waves, k, d, k, J.part + (size_t)tindex * ((size_t)k * (d + 1)));
+    write_partial_row(T, 1, k, d, k, J.part + (size_t)tindex * ((size_t)k * (d + 1)));
 }
 
 // Tiles of the jobs of one size class (MFMA labeling, NKB K-blocks).
@@ -694,15 +822,16 @@ bool arena_alloc(clk_batch *b, uint64_t need, uint64_t *where)
     return true;
 }
 
-// Jobs with few clusters label faster with the FP32 scan than on the matrix
+constexpr uint32_t kSmallK = 64;
+
+// Jobs with a small k*d label faster with the FP32 scan than on the matrix
 // cores (the 16-bit path has a fixed cost per point: conversion, bounds,
-// margin test); the labels are the same either way.
+// margin test, and d is padded to 32; measured: d = 3 or 8 with k up to 256
+// are faster on the scan); the labels are the same either way.
 bool use_mfma(const clk_batch *b, uint32_t d, uint32_t k)
 {
     return b->mfma && k >= 16 && (uint64_t)k * d >= 1024;
 }
-
-constexpr uint32_t kSmallK = 64;
 
 uint32_t class_of(uint32_t kind, uint32_t d, uint32_t k)
 {

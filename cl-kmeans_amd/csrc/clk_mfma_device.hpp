@@ -284,6 +284,66 @@ __device__ __forceinline__ void load_points_keep(PointOps<NKB> &x, float (&X2)[k
     load_points_core<FP16, NKB, FULL, true>(x, X2, RX2, pts, n, d, pid, q, h, nullptr, keep);
 }
 
+// The loads of load_points alone, for software pipelining: raw[g][kb*8 + j]
+// = feature kb*32 + 8h + j of point pid[g], not yet masked (a missing point
+// reads point n-1, a missing feature feature 0; build_operands zeroes them).
+template <int NKB, bool FULL>
+__device__ __forceinline__ void fetch_points(float (&raw)[kGroups][NKB * 8], const float *__restrict__ pts,
+                                             uint64_t n, uint32_t d, const uint64_t (&pid)[kGroups],
+                                             uint32_t h)
+{
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g) {
+        const uint64_t pc = pid[g] < n ? pid[g] : n - 1;
+        const float *row = pts + pc + (uint64_t)(8 * h) * n;  // feature 8h
+#pragma unroll
+        for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint64_t uoff = (uint64_t)(kb * kKBlock + j) * n;
+                if constexpr (FULL) {
+                    raw[g][kb * 8 + j] = row[uoff];
+                } else {
+                    const bool fv = (uint32_t)(kb * kKBlock + 8 * h + j) < d;
+                    raw[g][kb * 8 + j] = *(fv ? row + uoff : pts + pc);
+                }
+            }
+    }
+}
+
+// The operands, X2 and RX2 of load_points from fetch_points' values.
+template <bool FP16, int NKB, bool FULL>
+__device__ __forceinline__ void build_operands(const float (&raw)[kGroups][NKB * 8], PointOps<NKB> &x,
+                                               float (&X2)[kGroups], float (&RX2)[kGroups], uint64_t n,
+                                               uint32_t d, const uint64_t (&pid)[kGroups], uint32_t h)
+{
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g) {
+        const bool valid = pid[g] < n;
+        float s = 0.0f, r = 0.0f;
+#pragma unroll
+        for (int kb = 0; kb < NKB; ++kb) {
+            u16x8 vb;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const bool fv = FULL || (uint32_t)(kb * kKBlock + 8 * h + j) < d;
+                const float v = (valid && fv) ? raw[g][kb * 8 + j] : 0.0f;
+                s = __builtin_fmaf(v, v, s);
+                vb[j] = to16<FP16>(v);
+                const float res = v - from16<FP16>(vb[j]);
+                r = __builtin_fmaf(res, res, r);
+            }
+            x.b[g][kb] = vb;
+        }
+        s += __shfl_xor(s, 16);
+        s += __shfl_xor(s, 32);
+        r += __shfl_xor(r, 16);
+        r += __shfl_xor(r, 32);
+        X2[g] = s * (1.0f + (float)(2 * NKB * kKBlock + 8) * kU) + 1e-37f;
+        RX2[g] = r * (1.0f + (float)(2 * NKB * kKBlock + 8) * kU) + 1e-37f;
+    }
+}
+
 // Everything the margin test needs about one wave's points.
 struct WaveBounds {
     float K;               // shared score offset
