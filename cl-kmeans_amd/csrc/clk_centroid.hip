@@ -210,16 +210,26 @@ cu_slice_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t 
 
 using CmKernel = void (*)(const float *, uint64_t, uint32_t, uint32_t, const uint32_t *, double *);
 
+// Points per work-item at most (VEC x DMAX values in registers, no spills).
+constexpr int cm_vec_cap(int dmax) { return dmax >= 64 ? 1 : dmax >= 32 ? 2 : 4; }
+
+template <int DMAX, int BLOCK>
+CmKernel pick_cm_block(int vec)
+{
+    constexpr int cap = cm_vec_cap(DMAX);
+    if constexpr (cap >= 4)
+        if (vec >= 4)
+            return cu_cluster_merge_kernel<DMAX, 4, BLOCK>;
+    if constexpr (cap >= 2)
+        if (vec >= 2)
+            return cu_cluster_merge_kernel<DMAX, 2, BLOCK>;
+    return cu_cluster_merge_kernel<DMAX, 1, BLOCK>;
+}
+
 template <int DMAX>
 CmKernel pick_cm_vec(int vec, bool b256)
 {
-    if (b256)
-        return vec == 4 ? cu_cluster_merge_kernel<DMAX, 4, 256>
-             : vec == 2 ? cu_cluster_merge_kernel<DMAX, 2, 256>
-                        : cu_cluster_merge_kernel<DMAX, 1, 256>;
-    return vec == 4 ? cu_cluster_merge_kernel<DMAX, 4, 0>
-         : vec == 2 ? cu_cluster_merge_kernel<DMAX, 2, 0>
-                    : cu_cluster_merge_kernel<DMAX, 1, 0>;
+    return b256 ? pick_cm_block<DMAX, 256>(vec) : pick_cm_block<DMAX, 0>(vec);
 }
 
 CmKernel pick_cm(int dmax, int vec, bool b256)
@@ -265,7 +275,8 @@ int run_centroid_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t
     uint64_t gx = 0;
 
     if (strategy == CLK_CENTROID_CLUSTER_MERGE && dmax && dk * B * 4 <= ctx->lds_per_block) {
-        const int vec = pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels);
+        const int vec = std::min(pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels),
+                                 cm_vec_cap(dmax));
         CmKernel kc = pick_cm(dmax, vec, B == 256);
         lds = dk * B * 4;
         gx = user_grid ? user_grid : auto_grid(ctx, (const void *)kc, (int)B, lds, n / vec);

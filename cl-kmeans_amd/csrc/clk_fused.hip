@@ -26,7 +26,7 @@ namespace clk {
 namespace {
 
 template <int DMAX, int VEC, int BLOCK, bool PRIVATE, bool EXACT>
-__global__ void __launch_bounds__(BLOCK > 0 ? BLOCK : 1024)
+__global__ void __launch_bounds__(BLOCK > 0 ? BLOCK : (DMAX >= 64 ? 512 : 1024))
 fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
                   const float *__restrict__ cen, uint32_t *__restrict__ labels,
                   double *__restrict__ part_sums, uint32_t *__restrict__ part_masses)
@@ -114,21 +114,28 @@ fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
 using FusedKernel = void (*)(const float *, uint64_t, uint32_t, uint32_t, const float *,
                              uint32_t *, double *, uint32_t *);
 
+// Points per work-item at most (VEC x DMAX values in registers, no spills).
+constexpr int fused_vec_cap(int dmax) { return dmax >= 32 ? 1 : dmax >= 16 ? 2 : 4; }
+
+template <int DMAX, int BLOCK, bool PRIVATE, bool EXACT>
+FusedKernel pick_vec_block(int vec)
+{
+    constexpr int cap = fused_vec_cap(DMAX);
+    if constexpr (cap >= 4)
+        if (vec >= 4)
+            return fused_tile_kernel<DMAX, 4, BLOCK, PRIVATE, EXACT>;
+    if constexpr (cap >= 2)
+        if (vec >= 2)
+            return fused_tile_kernel<DMAX, 2, BLOCK, PRIVATE, EXACT>;
+    return fused_tile_kernel<DMAX, 1, BLOCK, PRIVATE, EXACT>;
+}
+
 template <int DMAX, bool PRIVATE, bool EXACT>
 FusedKernel pick_vec(int vec, bool block256)
 {
-    if (PRIVATE && block256) {
-        switch (vec) {
-        case 4: return fused_tile_kernel<DMAX, 4, 256, PRIVATE, EXACT>;
-        case 2: return fused_tile_kernel<DMAX, 2, 256, PRIVATE, EXACT>;
-        default: return fused_tile_kernel<DMAX, 1, 256, PRIVATE, EXACT>;
-        }
-    }
-    switch (vec) {
-    case 4: return fused_tile_kernel<DMAX, 4, 0, PRIVATE, EXACT>;
-    case 2: return fused_tile_kernel<DMAX, 2, 0, PRIVATE, EXACT>;
-    default: return fused_tile_kernel<DMAX, 1, 0, PRIVATE, EXACT>;
-    }
+    if (PRIVATE && block256)
+        return pick_vec_block<DMAX, 256, PRIVATE, EXACT>(vec);
+    return pick_vec_block<DMAX, 0, PRIVATE, EXACT>(vec);
 }
 
 template <int DMAX, bool PRIVATE>
@@ -191,7 +198,10 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
     const int B = block_size(ctx, cfg);
     CLK_REQUIRE(B > 0, CLK_ERR_INVALID_ARGUMENT, "fused: local_size exceeds the device limit");
     const int dmax = dmax_bucket(d);
-    const int vec = pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels);
+    const int vec = std::min(pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels),
+                             fused_vec_cap(dmax));
+    CLK_REQUIRE(dmax < 64 || B <= 512, CLK_ERR_INVALID_ARGUMENT,
+                "fused: local_size above 512 with more than 32 features");
 
     // Variant by LDS fit (the reference picks its GLOBAL_MEM variant the same
     // way, fused_feature_sum.hpp:315-337): private tile, shared tile, or the

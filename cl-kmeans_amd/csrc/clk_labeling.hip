@@ -9,9 +9,14 @@ namespace clk {
 namespace {
 
 // Centroids staged in LDS ([k][DMAX]); each work-item labels VEC
-// consecutive points per step with 128-bit row loads.
+// consecutive points per step with 128-bit row loads.  VEC * DMAX point
+// values live in registers: with 64 features the work-group is capped at 512
+// work-items so that the kernel may use 256 registers.
+template <int DMAX>
+constexpr int label_lds_threads() { return DMAX >= 64 ? 512 : 1024; }
+
 template <int DMAX, int VEC>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(label_lds_threads<DMAX>())
 label_lds_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
                  const float *__restrict__ cen, uint32_t *__restrict__ labels)
 {
@@ -97,14 +102,20 @@ accumulate_global_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, 
 using LabelKernel = void (*)(const float *, uint64_t, uint32_t, uint32_t, const float *,
                              uint32_t *);
 
+// Points per work-item at most (VEC x DMAX values in registers, no spills).
+constexpr int label_vec_cap(int dmax) { return dmax >= 64 ? 1 : dmax >= 32 ? 2 : 4; }
+
 template <int DMAX>
 LabelKernel pick_label_vec(int vec)
 {
-    switch (vec) {
-    case 4: return label_lds_kernel<DMAX, 4>;
-    case 2: return label_lds_kernel<DMAX, 2>;
-    default: return label_lds_kernel<DMAX, 1>;
-    }
+    constexpr int cap = label_vec_cap(DMAX);
+    if constexpr (cap >= 4)
+        if (vec >= 4)
+            return label_lds_kernel<DMAX, 4>;
+    if constexpr (cap >= 2)
+        if (vec >= 2)
+            return label_lds_kernel<DMAX, 2>;
+    return label_lds_kernel<DMAX, 1>;
 }
 
 LabelKernel pick_label_lds(int dmax, int vec)
@@ -149,8 +160,11 @@ int launch_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, 
     const int dmax = dmax_bucket(d);
     const uint64_t cen_bytes = dmax ? k * dmax * 4 : 0;
     if (dmax && cen_bytes <= ctx->lds_per_block) {
-        const int vec = pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels);
+        const int vec = std::min(pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels),
+                                 label_vec_cap(dmax));
         LabelKernel kern = pick_label_lds(dmax, vec);
+        CLK_REQUIRE(dmax < 64 || B <= 512, CLK_ERR_INVALID_ARGUMENT,
+                    "labeling: local_size above 512 with more than 32 features");
         uint64_t grid = (cfg && cfg->global_size[0])
                             ? div_up(cfg->global_size[0], B)
                             : auto_grid(ctx, (const void *)kern, (int)B, cen_bytes, n / vec);
