@@ -57,6 +57,7 @@ EXPORTS = [
     "clk_kmeans_run", "clk_conf_describe", "clk_lloyd_step", "clk_labeling_refined_points",
     "clk_batch_job_bytes", "clk_batch_create", "clk_batch_destroy", "clk_batch_add", "clk_batch_step",
     "clk_batch_poll", "clk_batch_take", "clk_batch_job_buffers", "clk_batch_autotune", "clk_batch_run_serial",
+    "clk_serve_run", "clk_serve_serial",
 ]
 
 JOB_FREE, JOB_ACTIVE, JOB_DONE = 0, 1, 2
@@ -70,6 +71,40 @@ class BatchConfig(ctypes.Structure):
         ("max_d", ctypes.c_uint32),
         ("max_k", ctypes.c_uint32),
         ("labeling", ctypes.c_int32),
+    ]
+
+
+class ServeJob(ctypes.Structure):
+    """clk_serve_job (clkmeans_batch.h)."""
+    _fields_ = [
+        ("n", ctypes.c_uint64),
+        ("d", ctypes.c_uint32),
+        ("k", ctypes.c_uint32),
+        ("points", ctypes.c_void_p),
+        ("centroids", ctypes.c_void_p),
+        ("arrival", ctypes.c_double),
+        ("max_iterations", ctypes.c_uint32),
+        ("tolerance", ctypes.c_double),
+        ("out_centroids", ctypes.c_void_p),
+        ("out_labels", ctypes.c_void_p),
+        ("out_masses", ctypes.c_void_p),
+        ("iterations", ctypes.c_uint32),
+        ("shift", ctypes.c_double),
+        ("admitted", ctypes.c_double),
+        ("finished", ctypes.c_double),
+    ]
+
+
+class ServeStats(ctypes.Structure):
+    """clk_serve_stats (clkmeans_batch.h)."""
+    _fields_ = [
+        ("makespan", ctypes.c_double),
+        ("solves_per_s", ctypes.c_double),
+        ("p50", ctypes.c_double),
+        ("p99", ctypes.c_double),
+        ("mean_latency", ctypes.c_double),
+        ("steps", ctypes.c_uint64),
+        ("max_active", ctypes.c_uint32),
     ]
 
 
@@ -127,6 +162,8 @@ def load(path=None):
                                         ctypes.POINTER(vp)]),
         "clk_batch_autotune": (i32, [vp, ctypes.POINTER(ctypes.c_uint32), vp]),
         "clk_batch_run_serial": (i32, [vp, vp]),
+        "clk_serve_run": (i32, [vp, ctypes.POINTER(ServeJob), ctypes.c_uint32, ctypes.POINTER(ServeStats), vp]),
+        "clk_serve_serial": (i32, [vp, ctypes.POINTER(ServeJob), ctypes.c_uint32, ctypes.POINTER(ServeStats), vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -388,6 +425,70 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class ServeRequest:
+    """One k-means job of a serving run: host points [d][n] and initial
+    centroids [d][k] (float32; pinned arrays let the copy-in overlap the GPU
+    work), its arrival time in seconds and stopping rule; after the run, the
+    result arrays and timings."""
+
+    def __init__(self, points, centroids, arrival, max_iterations=50, tolerance=1e-4, want_labels=True):
+        self.points = points
+        self.centroids = centroids
+        self.d, self.n = points.shape
+        self.k = centroids.shape[1]
+        assert points.dtype == np.float32 and centroids.dtype == np.float32
+        assert points.flags.c_contiguous and centroids.flags.c_contiguous and centroids.shape[0] == self.d
+        self.arrival = float(arrival)
+        self.max_iterations = int(max_iterations)
+        self.tolerance = float(tolerance)
+        self.out_centroids = np.empty((self.d, self.k), np.float32)
+        self.out_labels = np.empty(self.n, np.uint32) if want_labels else None
+        self.out_masses = np.empty(self.k, np.uint32)
+        self.iterations = 0
+        self.shift = 0.0
+        self.admitted = self.finished = 0.0
+
+    def _struct(self):
+        return ServeJob(self.n, self.d, self.k, self.points.ctypes.data, self.centroids.ctypes.data, self.arrival,
+                        self.max_iterations, self.tolerance, self.out_centroids.ctypes.data,
+                        self.out_labels.ctypes.data if self.out_labels is not None else None,
+                        self.out_masses.ctypes.data, 0, 0.0, 0.0, 0.0)
+
+    def _read(self, js):
+        self.iterations, self.shift = js.iterations, js.shift
+        self.admitted, self.finished = js.admitted, js.finished
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def _serve(fn, first, requests, stream):
+    arr = (ServeJob * len(requests))(*[r._struct() for r in requests])
+    st = ServeStats()
+    check(fn(first, arr, len(requests), ctypes.byref(st), _s(stream)))
+    for r, js in zip(requests, arr):
+        r._read(js)
+    return {f: getattr(st, f) for f, _ in ServeStats._fields_}
+
+
+def serve(batch, requests, stream):
+    """Continuous batching of the requests (arrival order) through an empty
+    Batch (clk_serve_run); returns the run's statistics as a dict."""
+    return _serve(batch.L.clk_serve_run, batch.handle, requests, stream)
+
+
+def serve_serial(ctx, requests, stream):
+    """The same requests one at a time through clk_lloyd_step (clk_serve_serial)."""
+    return _serve(load().clk_serve_serial, ctx.handle, requests, stream)
+
+
+def poisson_arrivals(count, rate, seed=0):
+    """Arrival times (seconds) of a Poisson process of the given rate (jobs/s)."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=count))
 
 
 def job_mix(count, seed=0, log2n=(10, 18), ds=(3, 8, 16, 32), k_range=(4, 256), k_cap_div=16):

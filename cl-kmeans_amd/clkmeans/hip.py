@@ -120,6 +120,32 @@ class DeviceArray:
             pass
 
 
+class PinnedArray:
+    """A numpy array in page-locked host memory (hipHostMalloc), so copies
+    from it run asynchronously at full link speed."""
+
+    def __init__(self, shape, dtype):
+        dtype = np.dtype(dtype)
+        shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        nbytes = max(int(np.prod(shape, dtype=np.int64)) * dtype.itemsize, 1)
+        self.ptr = ctypes.c_void_p()
+        check(lib().hipHostMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(nbytes), 0), "hipHostMalloc")
+        buf = (ctypes.c_char * nbytes).from_address(self.ptr.value)
+        self.array = np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape, dtype=np.int64))).reshape(shape)
+
+    def free(self):
+        if self.ptr and self.ptr.value:
+            self.array = None
+            lib().hipHostFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Stream:
     def __init__(self):
         self.handle = ctypes.c_void_p()

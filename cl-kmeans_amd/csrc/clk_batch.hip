@@ -38,8 +38,13 @@
 #include "clkmeans_batch.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
+#include <deque>
+#include <string>
+#include <thread>
 #include <vector>
 
 namespace clk {
@@ -764,13 +769,16 @@ struct clk_batch {
     // host mirror of the job table (device-side fields iter/state/shift are
     // read back by poll / take)
     std::vector<BJob> jobs;
+    std::vector<uint32_t> staged_state;  // state to activate a staged slot with
     std::vector<uint64_t> off, bytes;  // arena ranges per slot
     uint32_t tile_work = 0;
     bool dirty = true;  // tile list / graph must be rebuilt
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
+    uint32_t stable_steps = 0;  // steps since the job set last changed
     BClassState cls[clk::kClasses];
+    uint32_t class_waves[clk::kClasses] = {};
 };
 
 namespace {
@@ -909,7 +917,9 @@ int rebuild(clk_batch *b, hipStream_t stream)
         ctl[c].ntiles = C.ntiles;
         if (C.ntiles == 0)
             continue;
-        C.waves = class_waves(b, c, kmax, dmax);
+        // the waves per work-group are the class's (fixed at creation): the
+        // FP32 summation order of a tile depends on them, never on the mix
+        C.waves = b->class_waves[c];
         if (C.waves == 0) {  // admitted jobs fit by construction (clk_batch_create checks max_k x max_d)
             clk::set_error("clk_batch: a size class does not fit the work-group's LDS");
             return CLK_ERR_UNSUPPORTED;
@@ -1018,14 +1028,19 @@ extern "C" int clk_batch_create(clk_context *ctx, const clk_batch_config *cfg, c
                 CLK_ERR_INVALID_ARGUMENT, "clk_batch_create: unknown labeling strategy id");
     b->mfma = cfg->labeling != CLK_LABELING_UNROLL_VECTOR;
     b->fp16 = cfg->labeling == CLK_LABELING_MFMA_FP16;
+    // waves per work-group of each class, for its largest admissible job;
     // every class a job up to max_k x max_d can land in must fit one wave
     {
+        for (uint32_t c = 0; c < clk::kClasses; ++c) {
+            const bool big = (c & 1u) != 0;
+            const uint32_t kmax = big ? cfg->max_k : std::min(cfg->max_k, kSmallK);
+            const uint32_t dcap = c < 10 ? (4u << (c / 2)) : (c >= 12 ? 64u : 32u);
+            b->class_waves[c] = class_waves(b, c, kmax, std::min(cfg->max_d, dcap));
+        }
         bool fits = true;
         for (uint32_t kind = 0; kind <= (b->mfma ? 1u : 0u); ++kind)
-            for (uint32_t k : {std::min(cfg->max_k, kSmallK), cfg->max_k}) {
-                const uint32_t c = class_of(kind, cfg->max_d, k);
-                fits = fits && class_waves(b, c, k, cfg->max_d) > 0;
-            }
+            for (uint32_t k : {std::min(cfg->max_k, kSmallK), cfg->max_k})
+                fits = fits && b->class_waves[class_of(kind, cfg->max_d, k)] > 0;
         if (!fits) {
             delete b;
             clk::set_error("clk_batch_create: max_k x max_d too large for the work-group's LDS");
@@ -1040,6 +1055,7 @@ extern "C" int clk_batch_create(clk_context *ctx, const clk_batch_config *cfg, c
         }
     }
     b->jobs.assign(cfg->max_jobs, BJob{});
+    b->staged_state.assign(cfg->max_jobs, 0);
     b->off.assign(cfg->max_jobs, 0);
     b->bytes.assign(cfg->max_jobs, 0);
     hipError_t e1 = hipMalloc(&b->d_jobs, cfg->max_jobs * sizeof(BJob));
@@ -1081,12 +1097,20 @@ extern "C" int clk_batch_destroy(clk_batch *b)
     return CLK_OK;
 }
 
-extern "C" int clk_batch_add(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *host_points,
-                             const float *host_centroids, uint32_t max_iterations, double tolerance,
-                             int32_t *slot, void *stream)
+namespace {
+
+// Host-side slot state while a job's data is copied in but the job is not
+// yet in the device job table (the serving loop stages copies while the GPU
+// works).
+constexpr uint32_t kJobStaged = 3;
+
+// Stage a job: a free slot and arena range, the job record, and the copies of
+// its points and initial centroids (async on stream).  The slot is not yet
+// visible to the kernels.
+int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *host_points,
+              const float *host_centroids, uint32_t max_iterations, double tolerance, int32_t *slot,
+              hipStream_t st)
 {
-    CLK_REQUIRE(b && slot, CLK_ERR_INVALID_ARGUMENT, "clk_batch_add: null argument");
-    CLK_TRY(clk::enter(b->ctx));
     CLK_REQUIRE(n >= 1 && n < (1ull << 32), CLK_ERR_SIZE, "clk_batch_add: n must be 1..2^32-1");
     CLK_REQUIRE(d >= 1 && d <= b->cfg.max_d && k >= 1 && k <= b->cfg.max_k, CLK_ERR_SIZE,
                 "clk_batch_add: d or k beyond the batch's max_d / max_k");
@@ -1109,7 +1133,6 @@ extern "C" int clk_batch_add(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, c
         clk::set_error("clk_batch_add: arena full");
         return CLK_ERR_NOMEM;
     }
-    hipStream_t st = clk::as_stream(stream);
     char *p = b->arena + at;
     BJob J{};
     J.nt16 = (uint32_t)clk::div_up(k, clk::mfma::kTile);
@@ -1148,6 +1171,23 @@ extern "C" int clk_batch_add(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, c
     CLK_HIP_CHECK(hipMemcpyAsync(J.cen[0], host_centroids, (size_t)d * k * 4, hipMemcpyHostToDevice, st));
     CLK_HIP_CHECK(hipMemsetAsync(J.masses, 0, (size_t)k * 4, st));
     CLK_HIP_CHECK(hipMemsetAsync(J.labels, 0, n * 4, st));
+    b->jobs[s] = J;
+    b->jobs[s].state = kJobStaged;
+    b->staged_state[s] = J.state;
+    b->off[s] = at;
+    b->bytes[s] = need;
+    *slot = s;
+    return CLK_OK;
+}
+
+// Make a staged job visible: its record in the device job table and, for the
+// MFMA labeling, its first 16-bit table (stream-ordered after the copies when
+// st is the staging stream; otherwise the caller orders the two streams).
+// No step may be in flight on another stream.
+int activate_job(clk_batch *b, int32_t s, hipStream_t st)
+{
+    BJob &J = b->jobs[s];
+    J.state = b->staged_state[s];
     CLK_HIP_CHECK(hipMemcpyAsync(b->d_jobs + s, &J, sizeof(BJob), hipMemcpyHostToDevice, st));
     if (J.kind == 1) {
         if (b->fp16)
@@ -1158,11 +1198,28 @@ extern "C" int clk_batch_add(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, c
                                (uint32_t)s);
         CLK_HIP_CHECK(hipGetLastError());
     }
-    CLK_HIP_CHECK(hipStreamSynchronize(st));  // host_points may be reused at once
-    b->jobs[s] = J;
-    b->off[s] = at;
-    b->bytes[s] = need;
+    // the record is copied from host memory: keep it unchanged until the copy ends
+    CLK_HIP_CHECK(hipStreamSynchronize(st));
     b->dirty = true;
+    return CLK_OK;
+}
+
+}  // namespace
+
+extern "C" int clk_batch_add(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *host_points,
+                             const float *host_centroids, uint32_t max_iterations, double tolerance,
+                             int32_t *slot, void *stream)
+{
+    CLK_REQUIRE(b && slot, CLK_ERR_INVALID_ARGUMENT, "clk_batch_add: null argument");
+    CLK_TRY(clk::enter(b->ctx));
+    hipStream_t st = clk::as_stream(stream);
+    int32_t s = -1;
+    CLK_TRY(stage_job(b, n, d, k, host_points, host_centroids, max_iterations, tolerance, &s, st));
+    const int rc = activate_job(b, s, st);  // synchronizes: host_points may be reused at once
+    if (rc != CLK_OK) {
+        b->jobs[s].state = CLK_JOB_FREE;
+        return rc;
+    }
     *slot = s;
     return CLK_OK;
 }
@@ -1172,10 +1229,18 @@ extern "C" int clk_batch_step(clk_batch *b, void *stream)
     CLK_REQUIRE(b, CLK_ERR_INVALID_ARGUMENT, "clk_batch_step: null batch");
     CLK_TRY(clk::enter(b->ctx));
     hipStream_t st = clk::as_stream(stream);
-    if (b->dirty)
+    if (b->dirty) {
         CLK_TRY(rebuild(b, st));
+        b->stable_steps = 0;
+    }
     if (b->nactive == 0)
         return CLK_OK;
+    // a graph pays off once the job set repeats: while jobs come and go every
+    // iteration (serving), launch directly
+    if (++b->stable_steps < 2) {
+        clk::KernelTimer t(b->ctx, "BatchStep", st);
+        return launch_step(b, st);
+    }
     if (!b->exec || b->graph_stream != st) {
         if (b->exec) {
             (void)hipGraphExecDestroy(b->exec);
@@ -1319,6 +1384,336 @@ extern "C" int clk_batch_run_serial(clk_batch *b, void *stream)
     (void)hipFree(shift_d);
     CLK_TRY(rc);
     CLK_HIP_CHECK(hipMemcpy(b->d_jobs, dev.data(), dev.size() * sizeof(BJob), hipMemcpyHostToDevice));
+    return CLK_OK;
+}
+
+// ---------------------------------------------------------------- serving ---
+
+namespace {
+
+double seconds_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void latency_stats(const clk_serve_job *jobs, uint32_t count, clk_serve_stats *st)
+{
+    std::vector<double> lat(count);
+    double first = jobs[0].arrival, last = 0.0, sum = 0.0;
+    for (uint32_t i = 0; i < count; ++i) {
+        lat[i] = jobs[i].finished - jobs[i].arrival;
+        sum += lat[i];
+        first = std::min(first, jobs[i].arrival);
+        last = std::max(last, jobs[i].finished);
+    }
+    std::sort(lat.begin(), lat.end());
+    auto pct = [&](double q) {  // nearest rank
+        const size_t r = (size_t)std::ceil(q * (double)count);
+        return lat[std::min<size_t>(count - 1, r > 0 ? r - 1 : 0)];
+    };
+    st->makespan = last - first;
+    st->solves_per_s = st->makespan > 0.0 ? (double)count / st->makespan : 0.0;
+    st->p50 = pct(0.50);
+    st->p99 = pct(0.99);
+    st->mean_latency = sum / (double)count;
+}
+
+int check_serve_jobs(const clk_serve_job *jobs, uint32_t count)
+{
+    CLK_REQUIRE(jobs && count > 0, CLK_ERR_INVALID_ARGUMENT, "clk_serve: no jobs");
+    for (uint32_t i = 0; i < count; ++i) {
+        CLK_REQUIRE(jobs[i].points && jobs[i].centroids, CLK_ERR_INVALID_ARGUMENT, "clk_serve: null job data");
+        CLK_REQUIRE(i == 0 || jobs[i].arrival >= jobs[i - 1].arrival, CLK_ERR_INVALID_ARGUMENT,
+                    "clk_serve: jobs must be in arrival order");
+    }
+    return CLK_OK;
+}
+
+struct Staged {
+    int32_t slot;
+    uint32_t job;
+    hipEvent_t copied;
+};
+
+}  // namespace
+
+extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, clk_serve_stats *stats,
+                             void *stream)
+{
+    CLK_REQUIRE(b, CLK_ERR_INVALID_ARGUMENT, "clk_serve_run: null batch");
+    CLK_TRY(clk::enter(b->ctx));
+    CLK_TRY(check_serve_jobs(jobs, count));
+    hipStream_t st = clk::as_stream(stream);
+    CLK_REQUIRE(st != nullptr, CLK_ERR_INVALID_ARGUMENT, "clk_serve_run: needs a created stream");
+    for (uint32_t s = 0; s < b->jobs.size(); ++s)
+        CLK_REQUIRE(b->jobs[s].state == CLK_JOB_FREE, CLK_ERR_INVALID_ARGUMENT, "clk_serve_run: batch not empty");
+    for (uint32_t i = 0; i < count; ++i) {  // every job fits an empty batch, so admission cannot stall
+        const clk_serve_job &j = jobs[i];
+        CLK_REQUIRE(j.n >= 1 && j.d >= 1 && j.d <= b->cfg.max_d && j.k >= 1 && j.k <= b->cfg.max_k, CLK_ERR_SIZE,
+                    "clk_serve_run: a job's n, d or k is outside the batch's limits");
+        CLK_REQUIRE(job_bytes(j.n, j.d, j.k, tile_points(b->tile_work, j.d, j.k)) <= b->cfg.arena_bytes,
+                    CLK_ERR_NOMEM, "clk_serve_run: a job is larger than the arena");
+    }
+    const uint32_t slots = (uint32_t)b->jobs.size();
+    hipStream_t cs = nullptr;  // admission copies, concurrent with the iterations
+    hipEvent_t step_done = nullptr;
+    uint32_t *h_state = nullptr;
+    CLK_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    int rc = CLK_OK;
+    std::vector<int64_t> slot_job(slots, -1);
+    std::deque<Staged> staged;
+    auto fail = [&](int code) {
+        rc = code;
+        return code;
+    };
+    if (hipEventCreateWithFlags(&step_done, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void **)&h_state, slots * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+        clk::set_error("clk_serve_run: event / pinned buffer allocation failed");
+        fail(CLK_ERR_HIP);
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    uint32_t next = 0, finished = 0, active = 0;
+    bool inflight = false;
+    uint64_t steps = 0;
+    uint32_t max_active = 0;
+    while (rc == CLK_OK && finished < count) {
+        const double now = seconds_since(t0);
+        // 1. stage the arrived jobs in arrival order while slots and arena allow
+        //    (their copies overlap the iteration in flight)
+        while (next < count && jobs[next].arrival <= now) {
+            clk_serve_job &j = jobs[next];
+            int32_t s = -1;
+            const int r = stage_job(b, j.n, j.d, j.k, j.points, j.centroids, j.max_iterations, j.tolerance, &s, cs);
+            if (r == CLK_ERR_NOMEM)
+                break;
+            if (r != CLK_OK) {
+                fail(r);
+                break;
+            }
+            hipEvent_t ev = nullptr;
+            if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+                hipEventRecord(ev, cs) != hipSuccess) {
+                clk::set_error("clk_serve_run: event record failed");
+                fail(CLK_ERR_HIP);
+                break;
+            }
+            j.admitted = seconds_since(t0);
+            slot_job[s] = next;
+            staged.push_back({s, next, ev});
+            ++next;
+        }
+        if (rc != CLK_OK)
+            break;
+        // 2. end of the iteration in flight: collect finished jobs
+        if (inflight) {
+            if (hipEventSynchronize(step_done) != hipSuccess) {
+                clk::set_error("clk_serve_run: iteration failed");
+                fail(CLK_ERR_HIP);
+                break;
+            }
+            inflight = false;
+            if (hipMemcpy2DAsync(h_state, sizeof(uint32_t), reinterpret_cast<char *>(b->d_jobs) + offsetof(BJob, state),
+                                 sizeof(BJob), sizeof(uint32_t), slots, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess) {
+                clk::set_error("clk_serve_run: state read failed");
+                fail(CLK_ERR_HIP);
+                break;
+            }
+            for (uint32_t s = 0; s < slots && rc == CLK_OK; ++s) {
+                if (b->jobs[s].state != CLK_JOB_ACTIVE || h_state[s] != CLK_JOB_DONE)
+                    continue;
+                b->jobs[s].state = CLK_JOB_DONE;  // leaves the tile list
+                b->dirty = true;
+                clk_serve_job &j = jobs[slot_job[s]];
+                const int r = clk_batch_take(b, (int32_t)s, j.out_centroids, j.out_labels, j.out_masses,
+                                             &j.iterations, &j.shift, st);
+                if (r != CLK_OK) {
+                    fail(r);
+                    break;
+                }
+                j.finished = seconds_since(t0);
+                slot_job[s] = -1;
+                ++finished;
+                --active;
+            }
+            if (rc != CLK_OK)
+                break;
+        }
+        // 3. activate staged jobs whose copies are done (all of them when the
+        //    GPU would otherwise idle)
+        while (!staged.empty()) {
+            Staged &g = staged.front();
+            hipError_t q = hipEventQuery(g.copied);
+            if (q == hipErrorNotReady && active == 0)
+                q = hipEventSynchronize(g.copied);
+            if (q == hipErrorNotReady)
+                break;
+            if (q != hipSuccess) {
+                clk::set_error("clk_serve_run: admission copy failed");
+                fail(CLK_ERR_HIP);
+                break;
+            }
+            (void)hipEventDestroy(g.copied);
+            const int r = activate_job(b, g.slot, st);
+            if (r != CLK_OK) {
+                fail(r);
+                break;
+            }
+            if (b->jobs[g.slot].state == CLK_JOB_ACTIVE) {
+                ++active;
+            } else {  // zero iterations: done at once
+                clk_serve_job &j = jobs[g.job];
+                const int r2 = clk_batch_take(b, g.slot, j.out_centroids, j.out_labels, j.out_masses,
+                                              &j.iterations, &j.shift, st);
+                if (r2 != CLK_OK) {
+                    fail(r2);
+                    break;
+                }
+                j.finished = seconds_since(t0);
+                slot_job[g.slot] = -1;
+                ++finished;
+            }
+            staged.pop_front();
+        }
+        if (rc != CLK_OK)
+            break;
+        // 4. the next iteration of every resident job
+        if (active > 0) {
+            max_active = std::max(max_active, active);
+            const int r = clk_batch_step(b, st);
+            if (r != CLK_OK || hipEventRecord(step_done, st) != hipSuccess) {
+                fail(r != CLK_OK ? r : CLK_ERR_HIP);
+                break;
+            }
+            inflight = true;
+            ++steps;
+        } else if (staged.empty() && next < count) {
+            // idle until the next arrival
+            const double wait = jobs[next].arrival - seconds_since(t0);
+            if (wait > 0.0)
+                std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.001)));
+        }
+    }
+    if (inflight)
+        (void)hipEventSynchronize(step_done);
+    for (auto &g : staged) {
+        (void)hipEventSynchronize(g.copied);
+        (void)hipEventDestroy(g.copied);
+        b->jobs[g.slot].state = CLK_JOB_FREE;
+    }
+    if (rc != CLK_OK)  // leave the batch empty
+        for (uint32_t s = 0; s < slots; ++s)
+            if (b->jobs[s].state != CLK_JOB_FREE)
+                (void)clk_batch_take(b, (int32_t)s, nullptr, nullptr, nullptr, nullptr, nullptr, st);
+    (void)hipStreamSynchronize(cs);
+    (void)hipStreamDestroy(cs);
+    if (step_done)
+        (void)hipEventDestroy(step_done);
+    if (h_state)
+        (void)hipHostFree(h_state);
+    CLK_TRY(rc);
+    if (stats) {
+        latency_stats(jobs, count, stats);
+        stats->steps = steps;
+        stats->max_active = max_active;
+    }
+    return CLK_OK;
+}
+
+extern "C" int clk_serve_serial(clk_context *ctx, clk_serve_job *jobs, uint32_t count, clk_serve_stats *stats,
+                                void *stream)
+{
+    CLK_TRY(clk::enter(ctx));
+    CLK_TRY(check_serve_jobs(jobs, count));
+    hipStream_t st = clk::as_stream(stream);
+    uint64_t maxn = 0, maxdn = 0, maxdk = 0, maxk = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        maxn = std::max<uint64_t>(maxn, jobs[i].n);
+        maxdn = std::max<uint64_t>(maxdn, jobs[i].n * jobs[i].d);
+        maxdk = std::max<uint64_t>(maxdk, (uint64_t)jobs[i].d * jobs[i].k);
+        maxk = std::max<uint64_t>(maxk, jobs[i].k);
+    }
+    float *pts = nullptr, *cen[2] = {nullptr, nullptr};
+    double *sums = nullptr, *shift_d = nullptr;
+    uint32_t *labels = nullptr, *masses = nullptr;
+    int rc = CLK_OK;
+    if (hipMalloc(&pts, maxdn * 4) != hipSuccess || hipMalloc(&cen[0], maxdk * 4) != hipSuccess ||
+        hipMalloc(&cen[1], maxdk * 4) != hipSuccess || hipMalloc(&sums, maxdk * 8) != hipSuccess ||
+        hipMalloc(&labels, maxn * 4) != hipSuccess || hipMalloc(&masses, maxk * 4) != hipSuccess ||
+        hipMalloc(&shift_d, 8) != hipSuccess) {
+        clk::set_error("clk_serve_serial: device allocation failed");
+        rc = CLK_ERR_NOMEM;
+    }
+    clk_kernel_config cfg;
+    if (rc == CLK_OK)
+        rc = clk_default_config(ctx, CLK_STAGE_FUSED, CLK_FUSED_CLUSTER_MERGE, &cfg);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 0; i < count && rc == CLK_OK; ++i) {
+        clk_serve_job &j = jobs[i];
+        while (seconds_since(t0) < j.arrival)
+            std::this_thread::sleep_for(std::chrono::duration<double>(
+                std::min(0.001, j.arrival - seconds_since(t0))));
+        j.admitted = seconds_since(t0);
+        hipError_t e = hipMemcpyAsync(pts, j.points, j.n * j.d * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(cen[0], j.centroids, (size_t)j.d * j.k * 4, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) {
+            clk::set_error(std::string("clk_serve_serial: ") + hipGetErrorString(e));
+            rc = CLK_ERR_HIP;
+            break;
+        }
+        uint32_t it = 0;
+        double shift = 0.0;
+        int cur = 0;
+        while (it < j.max_iterations) {
+            rc = clk_lloyd_step(ctx, &cfg, j.d, j.n, j.k, pts, cen[cur], cen[cur ^ 1], sums, labels, masses,
+                                j.tolerance > 0.0 ? shift_d : nullptr, st);
+            if (rc != CLK_OK)
+                break;
+            cur ^= 1;
+            ++it;
+            if (j.tolerance > 0.0) {  // the convergence test of every iteration
+                e = hipMemcpyAsync(&shift, shift_d, 8, hipMemcpyDeviceToHost, st);
+                if (e == hipSuccess)
+                    e = hipStreamSynchronize(st);
+                if (e != hipSuccess) {
+                    clk::set_error(std::string("clk_serve_serial: ") + hipGetErrorString(e));
+                    rc = CLK_ERR_HIP;
+                    break;
+                }
+                if (shift < j.tolerance)
+                    break;
+            }
+        }
+        if (rc != CLK_OK)
+            break;
+        e = hipStreamSynchronize(st);
+        if (e == hipSuccess && j.out_centroids)
+            e = hipMemcpy(j.out_centroids, cen[cur], (size_t)j.d * j.k * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && j.out_labels)
+            e = hipMemcpy(j.out_labels, labels, j.n * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && j.out_masses)
+            e = hipMemcpy(j.out_masses, masses, (size_t)j.k * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            clk::set_error(std::string("clk_serve_serial: ") + hipGetErrorString(e));
+            rc = CLK_ERR_HIP;
+            break;
+        }
+        j.iterations = it;
+        j.shift = shift;
+        j.finished = seconds_since(t0);
+    }
+    (void)hipStreamSynchronize(st);
+    for (void *p : {(void *)pts, (void *)cen[0], (void *)cen[1], (void *)sums, (void *)labels, (void *)masses,
+                    (void *)shift_d})
+        if (p)
+            (void)hipFree(p);
+    CLK_TRY(rc);
+    if (stats) {
+        latency_stats(jobs, count, stats);
+        stats->steps = 0;
+        stats->max_active = 1;
+    }
     return CLK_OK;
 }
 

@@ -89,6 +89,48 @@ int clk_batch_run_serial(clk_batch *b, void *stream);
  * chosen may be NULL. */
 int clk_batch_autotune(clk_batch *b, uint32_t *chosen, void *stream);
 
+/* ---- serving: jobs arriving over time --------------------------------- */
+
+typedef struct clk_serve_job {
+    uint64_t n;
+    uint32_t d, k;
+    const float *points;      /* host [d][n]; pinned memory lets the copy overlap the GPU work */
+    const float *centroids;   /* host [d][k], initial centroids */
+    double arrival;           /* seconds after the start of the run (non-decreasing over the array) */
+    uint32_t max_iterations;
+    double tolerance;         /* stop when the max centroid shift < tolerance (0: never) */
+    float *out_centroids;     /* [d][k] or NULL */
+    uint32_t *out_labels;     /* [n] or NULL */
+    uint32_t *out_masses;     /* [k] or NULL */
+    uint32_t iterations;      /* out */
+    double shift;             /* out: the last max shift (0 when no tolerance) */
+    double admitted;          /* out: seconds after the start when the copy-in began */
+    double finished;          /* out: seconds after the start when the result was on the host */
+} clk_serve_job;
+
+typedef struct clk_serve_stats {
+    double makespan;          /* first arrival .. last completion, seconds */
+    double solves_per_s;      /* jobs / makespan */
+    double p50, p99;          /* latency (finished - arrival), nearest rank, seconds */
+    double mean_latency;
+    uint64_t steps;           /* batch iterations launched */
+    uint32_t max_active;      /* most jobs iterating at once */
+} clk_serve_stats;
+
+/* Continuous batching over an empty batch: a job is admitted once its
+ * arrival time has passed and a slot and arena space are free (in arrival
+ * order), its points are copied in on a second stream while the resident jobs
+ * iterate, it joins the next iteration's launch, and it leaves (results
+ * copied to its out_* buffers) when it reaches its tolerance or iteration
+ * limit.  A job's results are those it gets alone in a batch. */
+int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, clk_serve_stats *stats, void *stream);
+
+/* The plain per-job loop for the same arrivals: one job at a time, copied in
+ * and iterated with clk_lloyd_step (fused cluster_merge, FP32 labels, FP64
+ * merge), the shift read back after every iteration when it has a
+ * tolerance. */
+int clk_serve_serial(clk_context *ctx, clk_serve_job *jobs, uint32_t count, clk_serve_stats *stats, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
