@@ -81,6 +81,7 @@ class ServeJob(ctypes.Structure):
         ("d", ctypes.c_uint32),
         ("k", ctypes.c_uint32),
         ("points", ctypes.c_void_p),
+        ("device_points", ctypes.c_int32),
         ("centroids", ctypes.c_void_p),
         ("arrival", ctypes.c_double),
         ("max_iterations", ctypes.c_uint32),
@@ -434,12 +435,15 @@ class ServeRequest:
     result arrays and timings."""
 
     def __init__(self, points, centroids, arrival, max_iterations=50, tolerance=1e-4, want_labels=True):
+        # points: host float32 [d][n], or a hip.DeviceArray (read in place)
         self.points = points
         self.centroids = centroids
         self.d, self.n = points.shape
         self.k = centroids.shape[1]
+        self.on_device = not isinstance(points, np.ndarray)
         assert points.dtype == np.float32 and centroids.dtype == np.float32
-        assert points.flags.c_contiguous and centroids.flags.c_contiguous and centroids.shape[0] == self.d
+        assert self.on_device or points.flags.c_contiguous
+        assert centroids.flags.c_contiguous and centroids.shape[0] == self.d
         self.arrival = float(arrival)
         self.max_iterations = int(max_iterations)
         self.tolerance = float(tolerance)
@@ -451,7 +455,8 @@ class ServeRequest:
         self.admitted = self.finished = 0.0
 
     def _struct(self):
-        return ServeJob(self.n, self.d, self.k, self.points.ctypes.data, self.centroids.ctypes.data, self.arrival,
+        pts = self.points.addr if self.on_device else self.points.ctypes.data
+        return ServeJob(self.n, self.d, self.k, pts, int(self.on_device), self.centroids.ctypes.data, self.arrival,
                         self.max_iterations, self.tolerance, self.out_centroids.ctypes.data,
                         self.out_labels.ctypes.data if self.out_labels is not None else None,
                         self.out_masses.ctypes.data, 0, 0.0, 0.0, 0.0)

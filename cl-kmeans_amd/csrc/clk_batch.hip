@@ -3,24 +3,27 @@
 // Device state: a job table (BJob, one per slot), a tile list (each job's
 // points cut into fixed tiles of tile_pts points, contiguous per job), the
 // list of active slots and a control block.  One iteration of every active
-// job is two launches, replayed from one HIP graph.  The tile kernel is
+// job is a few launches, replayed from one HIP graph.  The tile kernel is
 // persistent: work-groups take tiles from a global counter (the next one is
 // requested while the current one is processed); a tile is labeled with the
 // reference arithmetic (centroids staged in LDS, sequential fma distance,
 // strict '<', lowest index) and accumulated into per-wave LDS tables, and
 // the tile's sums and masses go to the job's FP64 partial row for that tile.
-// The merge kernel (one work-group per active job) then sums the job's rows
-// in a fixed order in FP64, divides (one rounding to float, NaN for an empty
-// cluster), computes the FP64 max centroid shift, advances the job and
-// resets the work counter.  The kernel boundary publishes the partial rows:
+// The merge kernel (one work-group per element slice of a job's row) then
+// sums the job's rows in a fixed order in FP64, and the finish kernel (one
+// work-group per active job) divides (one rounding to float, NaN for an
+// empty cluster), computes the FP64 max centroid shift, advances the job and
+// resets the work counters.  The kernel boundaries publish the partial rows:
 // no tile needs a device-scope fence, which on this GPU writes back and
-// invalidates the L2 of the tile's XCD.
+// invalidates the L2 of the tile's XCD.  Jobs fall into size classes (FP32
+// or MFMA labeling, d bucket, k up to 64 or above), one tile-kernel launch
+// each, so that a class's LDS and registers fit its own jobs.
 //
 // With an MFMA labeling strategy each tile is labeled like the dense-distance
 // kernels of clk_mfma.hip (the job's 16-bit centroid table staged in LDS,
 // scores on the matrix cores, the margin test; undecided points of a wave
 // are settled in the wave by exact FP32 distances to their candidate
-// centroids).  The merge kernel also builds the job's table for the next
+// centroids).  The finish kernel also builds the job's table for the next
 // iteration.
 //
 // Accumulation without float atomics: each wave owns an LDS table T[k][ts]
@@ -79,7 +82,7 @@ struct BTile {
 struct BCtl {
     uint32_t next;    // work counter of the tile kernel
     uint32_t ntiles;  // tiles in the list
-    uint32_t dry;     // 1: the merge kernel only resets the counter (no job advances)
+    uint32_t dry;     // 1: the finish kernel only resets the counters (no job advances)
     uint32_t pad;
 };
 
@@ -350,7 +353,7 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
 // Lanes per element in the merge of a job's tile rows: a power of two <= 64
 // that depends only on the row length and the tile count (never on the
 // launch), so a job's merged sums are the same in every batch.
-__device__ inline uint32_t merge_split(uint32_t row, uint32_t nt)
+__host__ __device__ inline uint32_t merge_split(uint32_t row, uint32_t nt)
 {
     uint32_t s = 1;
     while (s < 64 && 2 * s <= nt && (2 * s) * row <= 64 * kBatchMaxWaves)
@@ -358,41 +361,53 @@ __device__ inline uint32_t merge_split(uint32_t row, uint32_t nt)
     return s;
 }
 
-// Merge a job's tile rows, divide, shift, advance the job.  Element e of the
-// row (sums [d][k], then masses [k]) is summed by S lanes of one wave: lane s
-// adds rows s, s+S, s+2S, ... in order, then the S lane sums combine by an
-// xor butterfly (every lane ends with the same bits); the merged row
-// overwrites row 0, from which the divide reads.
+constexpr uint32_t kMergeThreads = 256;
+
+// A merge work-group's share: elements [e0, e0 + kMergeThreads / S) of one
+// job's row.
+struct BMergeUnit {
+    uint32_t job, e0;
+};
+
+// Merge a job's tile rows for one element slice.  Element e of the row (sums
+// [d][k], then masses [k]) is summed by S lanes of one wave: lane s adds rows
+// s, s+S, s+2S, ... in order, then the S lane sums combine by an xor
+// butterfly (every lane ends with the same bits); the merged value overwrites
+// row 0 (read only by the lane that writes it), where the finish reads it.
+__device__ void merge_rows(const BJob &J, uint32_t e0)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nt = J.ntiles, rl = J.k * (J.d + 1);
+    const uint32_t S = merge_split(rl, nt);
+    const uint32_t per = kMergeThreads / S, s = tid % S;
+    const uint32_t e = e0 + tid / S;
+    const bool mine = e < rl && e < e0 + per;
+    double acc = 0.0;
+    if (mine) {
+        const double *col = J.part + e;
+        uint32_t t = s;
+        for (; t + 3 * S < nt; t += 4 * S) {
+            const double a0 = col[(size_t)t * rl], a1 = col[(size_t)(t + S) * rl];
+            const double a2 = col[(size_t)(t + 2 * S) * rl], a3 = col[(size_t)(t + 3 * S) * rl];
+            acc += a0;
+            acc += a1;
+            acc += a2;
+            acc += a3;
+        }
+        for (; t < nt; t += S)
+            acc += col[(size_t)t * rl];
+    }
+    for (uint32_t o = 1; o < S; o *= 2)
+        acc += __shfl_xor(acc, (int)o);
+    if (mine && s == 0)
+        J.part[e] = acc;
+}
+
+// Divide the merged row 0, shift, advance the job.
 __device__ void batch_finalize(BJob &J, double *red)
 {
     const uint32_t tid = threadIdx.x;
-    const uint32_t d = J.d, k = J.k, nt = J.ntiles;
-    const uint32_t rl = k * (d + 1);
-    const uint32_t S = merge_split(rl, nt);
-    const uint32_t per = blockDim.x / S, s = tid % S;
-    for (uint32_t e0 = 0; e0 < rl; e0 += per) {
-        const uint32_t e = e0 + tid / S;
-        double acc = 0.0;
-        if (e < rl) {
-            const double *col = J.part + e;
-            uint32_t t = s;
-            for (; t + 3 * S < nt; t += 4 * S) {
-                const double a0 = col[(size_t)t * rl], a1 = col[(size_t)(t + S) * rl];
-                const double a2 = col[(size_t)(t + 2 * S) * rl], a3 = col[(size_t)(t + 3 * S) * rl];
-                acc += a0;
-                acc += a1;
-                acc += a2;
-                acc += a3;
-            }
-            for (; t < nt; t += S)
-                acc += col[(size_t)t * rl];
-        }
-        for (uint32_t o = 1; o < S; o *= 2)
-            acc += __shfl_xor(acc, (int)o);
-        if (e < rl && s == 0)  // row 0 of element e is read only by this lane
-            J.part[e] = acc;
-    }
-    __syncthreads();
+    const uint32_t d = J.d, k = J.k;
     const double *tot = J.part;
     const float *old = J.cen[J.iter & 1];
     float *nxt = J.cen[(J.iter & 1) ^ 1];
@@ -697,12 +712,26 @@ batch_tiles_mfma_kernel(const BJob *__restrict__ jobs, const BTile *__restrict__
 
 constexpr uint32_t kClasses = 14;  // 5 d buckets x 2 k ranges (FP32) + 2 K-block counts x 2 k ranges (MFMA)
 
-// One work-group per active job: merge, divide, advance, and for a job with
-// the MFMA labeling the next iteration's 16-bit table.  Work-group 0 resets
-// the tile counters of every class (the tile kernels have finished).
+// The merge of every active job's tile rows: one work-group per element
+// slice of a job (large jobs get many).
+__global__ void __launch_bounds__(kMergeThreads)
+batch_merge_kernel(const BJob *__restrict__ jobs, const BMergeUnit *__restrict__ units, const BCtl *__restrict__ ctl)
+{
+    if (ctl[0].dry != 0)
+        return;
+    const BMergeUnit u = units[blockIdx.x];
+    const BJob &J = jobs[u.job];
+    if (J.state != CLK_JOB_ACTIVE)
+        return;
+    merge_rows(J, u.e0);
+}
+
+// One work-group per active job: divide, advance, and for a job with the MFMA
+// labeling the next iteration's 16-bit table.  Work-group 0 resets the tile
+// counters of every class (the tile kernels have finished).
 template <bool FP16>
 __global__ void __launch_bounds__(256)
-batch_merge_kernel(BJob *__restrict__ jobs, const uint32_t *__restrict__ active, BCtl *__restrict__ ctl)
+batch_finish_kernel(BJob *__restrict__ jobs, const uint32_t *__restrict__ active, BCtl *__restrict__ ctl)
 {
     __shared__ double red[256];
     if (blockIdx.x == 0 && threadIdx.x < kClasses)
@@ -745,6 +774,7 @@ size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 using clk::BCtl;
 using clk::BJob;
+using clk::BMergeUnit;
 using clk::BTile;
 
 // A size class: the jobs of one tile-kernel instantiation.  Its LDS and
@@ -763,8 +793,10 @@ struct clk_batch {
     BCtl *d_ctl = nullptr;  // one per class
     BTile *d_tiles = nullptr;
     uint32_t tile_cap = 0;
-    uint32_t *d_active = nullptr;  // slots of the active jobs (merge work-groups)
+    uint32_t *d_active = nullptr;  // slots of the active jobs (finish work-groups)
     uint32_t nactive = 0;
+    clk::BMergeUnit *d_units = nullptr;  // merge work-groups
+    uint32_t nunits = 0, unit_cap = 0;
     char *arena = nullptr;
     // host mirror of the job table (device-side fields iter/state/shift are
     // read back by poll / take)
@@ -783,7 +815,7 @@ struct clk_batch {
 
 namespace {
 
-uint64_t job_bytes(uint64_t n, uint32_t d, uint32_t k, uint32_t tile_pts)
+uint64_t job_bytes(uint64_t n, uint32_t d, uint32_t k, uint32_t tile_pts, bool points = true)
 {
     const uint64_t nt = clk::div_up(std::max<uint64_t>(n, 1), tile_pts);
     const uint64_t nt16 = clk::div_up(k, clk::mfma::kTile), nkb = clk::div_up(d, clk::mfma::kKBlock);
@@ -791,7 +823,7 @@ uint64_t job_bytes(uint64_t n, uint32_t d, uint32_t k, uint32_t tile_pts)
     b += clk::align_up(nt16 * nkb * 64 * 16, 256);        // 16-bit table
     b += clk::align_up(nt16 * clk::mfma::kTile * 4, 256);  // norms
     b += 256;                                             // stats
-    b += clk::align_up(n * d * 4, 256);                   // points
+    b += points ? clk::align_up(n * d * 4, 256) : 0;     // points (unless read in place)
     b += clk::align_up(n * 4, 256);                       // labels
     b += 2 * clk::align_up((uint64_t)d * k * 4, 256);     // centroids x2
     b += clk::align_up((uint64_t)k * 4, 256);             // masses
@@ -944,6 +976,24 @@ int rebuild(clk_batch *b, hipStream_t stream)
         CLK_HIP_CHECK(hipMemcpyAsync(b->d_active, active.data(), active.size() * sizeof(uint32_t),
                                      hipMemcpyHostToDevice, stream));
     b->nactive = (uint32_t)active.size();
+    std::vector<BMergeUnit> units;
+    for (uint32_t s : active) {
+        const BJob &J = b->jobs[s];
+        const uint32_t rl = J.k * (J.d + 1);
+        const uint32_t per = clk::kMergeThreads / clk::merge_split(rl, J.ntiles);
+        for (uint32_t e0 = 0; e0 < rl; e0 += per)
+            units.push_back({s, e0});
+    }
+    if (units.size() > b->unit_cap) {
+        if (b->d_units)
+            CLK_HIP_CHECK(hipFree(b->d_units));
+        b->unit_cap = (uint32_t)std::max<size_t>(units.size(), 1024) * 2;
+        CLK_HIP_CHECK(hipMalloc(&b->d_units, b->unit_cap * sizeof(BMergeUnit)));
+    }
+    if (!units.empty())
+        CLK_HIP_CHECK(hipMemcpyAsync(b->d_units, units.data(), units.size() * sizeof(BMergeUnit),
+                                     hipMemcpyHostToDevice, stream));
+    b->nunits = (uint32_t)units.size();
     CLK_HIP_CHECK(hipMemcpyAsync(b->d_ctl, ctl.data(), ctl.size() * sizeof(BCtl), hipMemcpyHostToDevice, stream));
     CLK_HIP_CHECK(hipStreamSynchronize(stream));  // tiles / ctl are host temporaries
     if (b->exec) {
@@ -958,8 +1008,8 @@ int rebuild(clk_batch *b, hipStream_t stream)
     return CLK_OK;
 }
 
-// The tile kernel of every non-empty class, then the merge kernel (which
-// resets the tile counters).
+// The tile kernel of every non-empty class, then the merge and finish
+// kernels (the finish resets the tile counters).
 int launch_step(clk_batch *b, hipStream_t stream)
 {
     for (uint32_t c = 0; c < clk::kClasses; ++c) {
@@ -994,11 +1044,16 @@ int launch_step(clk_batch *b, hipStream_t stream)
         }
         CLK_HIP_CHECK(hipGetLastError());
     }
-    const dim3 mg(std::max<uint32_t>(b->nactive, 1)), mb(256);
+    if (b->nunits > 0) {
+        hipLaunchKernelGGL(clk::batch_merge_kernel, dim3(b->nunits), dim3(clk::kMergeThreads), 0, stream, b->d_jobs,
+                           b->d_units, b->d_ctl);
+        CLK_HIP_CHECK(hipGetLastError());
+    }
+    const dim3 fg(std::max<uint32_t>(b->nactive, 1)), fb(256);
     if (b->fp16)
-        hipLaunchKernelGGL(clk::batch_merge_kernel<true>, mg, mb, 0, stream, b->d_jobs, b->d_active, b->d_ctl);
+        hipLaunchKernelGGL(clk::batch_finish_kernel<true>, fg, fb, 0, stream, b->d_jobs, b->d_active, b->d_ctl);
     else
-        hipLaunchKernelGGL(clk::batch_merge_kernel<false>, mg, mb, 0, stream, b->d_jobs, b->d_active, b->d_ctl);
+        hipLaunchKernelGGL(clk::batch_finish_kernel<false>, fg, fb, 0, stream, b->d_jobs, b->d_active, b->d_ctl);
     CLK_HIP_CHECK(hipGetLastError());
     return CLK_OK;
 }
@@ -1091,6 +1146,8 @@ extern "C" int clk_batch_destroy(clk_batch *b)
         (void)hipFree(b->d_tiles);
     if (b->d_active)
         (void)hipFree(b->d_active);
+    if (b->d_units)
+        (void)hipFree(b->d_units);
     if (b->arena)
         (void)hipFree(b->arena);
     delete b;
@@ -1109,7 +1166,7 @@ constexpr uint32_t kJobStaged = 3;
 // visible to the kernels.
 int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *host_points,
               const float *host_centroids, uint32_t max_iterations, double tolerance, int32_t *slot,
-              hipStream_t st)
+              hipStream_t st, bool device_points = false)
 {
     CLK_REQUIRE(n >= 1 && n < (1ull << 32), CLK_ERR_SIZE, "clk_batch_add: n must be 1..2^32-1");
     CLK_REQUIRE(d >= 1 && d <= b->cfg.max_d && k >= 1 && k <= b->cfg.max_k, CLK_ERR_SIZE,
@@ -1127,7 +1184,7 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
         return CLK_ERR_NOMEM;
     }
     const uint32_t tp = tile_points(b->tile_work, d, k);
-    const uint64_t need = job_bytes(n, d, k, tp);
+    const uint64_t need = job_bytes(n, d, k, tp, !device_points);
     uint64_t at = 0;
     if (!arena_alloc(b, need, &at)) {
         clk::set_error("clk_batch_add: arena full");
@@ -1144,8 +1201,12 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
     J.stats = reinterpret_cast<float *>(p);
     p += 256;
     J.prm = clk::mfma::make_params(b->fp16, d, k, (b->ctx->mfma_subnormals & (b->fp16 ? 1 : 2)) != 0);
-    J.pts = reinterpret_cast<const float *>(p);
-    p += clk::align_up(n * d * 4, 256);
+    if (device_points) {
+        J.pts = host_points;
+    } else {
+        J.pts = reinterpret_cast<const float *>(p);
+        p += clk::align_up(n * d * 4, 256);
+    }
     J.labels = reinterpret_cast<uint32_t *>(p);
     p += clk::align_up(n * 4, 256);
     J.cen[0] = reinterpret_cast<float *>(p);
@@ -1167,7 +1228,9 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
     J.kind = use_mfma(b, d, k) ? 1u : 0u;
     J.tol = tolerance;
     J.shift = 0.0;
-    CLK_HIP_CHECK(hipMemcpyAsync(const_cast<float *>(J.pts), host_points, n * d * 4, hipMemcpyHostToDevice, st));
+    if (!device_points)
+        CLK_HIP_CHECK(
+            hipMemcpyAsync(const_cast<float *>(J.pts), host_points, n * d * 4, hipMemcpyHostToDevice, st));
     CLK_HIP_CHECK(hipMemcpyAsync(J.cen[0], host_centroids, (size_t)d * k * 4, hipMemcpyHostToDevice, st));
     CLK_HIP_CHECK(hipMemsetAsync(J.masses, 0, (size_t)k * 4, st));
     CLK_HIP_CHECK(hipMemsetAsync(J.labels, 0, n * 4, st));
@@ -1451,7 +1514,8 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
         const clk_serve_job &j = jobs[i];
         CLK_REQUIRE(j.n >= 1 && j.d >= 1 && j.d <= b->cfg.max_d && j.k >= 1 && j.k <= b->cfg.max_k, CLK_ERR_SIZE,
                     "clk_serve_run: a job's n, d or k is outside the batch's limits");
-        CLK_REQUIRE(job_bytes(j.n, j.d, j.k, tile_points(b->tile_work, j.d, j.k)) <= b->cfg.arena_bytes,
+        CLK_REQUIRE(job_bytes(j.n, j.d, j.k, tile_points(b->tile_work, j.d, j.k), j.device_points == 0) <=
+                        b->cfg.arena_bytes,
                     CLK_ERR_NOMEM, "clk_serve_run: a job is larger than the arena");
     }
     const uint32_t slots = (uint32_t)b->jobs.size();
@@ -1483,7 +1547,8 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
         while (next < count && jobs[next].arrival <= now) {
             clk_serve_job &j = jobs[next];
             int32_t s = -1;
-            const int r = stage_job(b, j.n, j.d, j.k, j.points, j.centroids, j.max_iterations, j.tolerance, &s, cs);
+            const int r = stage_job(b, j.n, j.d, j.k, j.points, j.centroids, j.max_iterations, j.tolerance, &s, cs,
+                                    j.device_points != 0);
             if (r == CLK_ERR_NOMEM)
                 break;
             if (r != CLK_OK) {
@@ -1629,7 +1694,8 @@ extern "C" int clk_serve_serial(clk_context *ctx, clk_serve_job *jobs, uint32_t 
     uint64_t maxn = 0, maxdn = 0, maxdk = 0, maxk = 0;
     for (uint32_t i = 0; i < count; ++i) {
         maxn = std::max<uint64_t>(maxn, jobs[i].n);
-        maxdn = std::max<uint64_t>(maxdn, jobs[i].n * jobs[i].d);
+        if (!jobs[i].device_points)
+            maxdn = std::max<uint64_t>(maxdn, jobs[i].n * jobs[i].d);
         maxdk = std::max<uint64_t>(maxdk, (uint64_t)jobs[i].d * jobs[i].k);
         maxk = std::max<uint64_t>(maxk, jobs[i].k);
     }
@@ -1637,7 +1703,7 @@ extern "C" int clk_serve_serial(clk_context *ctx, clk_serve_job *jobs, uint32_t 
     double *sums = nullptr, *shift_d = nullptr;
     uint32_t *labels = nullptr, *masses = nullptr;
     int rc = CLK_OK;
-    if (hipMalloc(&pts, maxdn * 4) != hipSuccess || hipMalloc(&cen[0], maxdk * 4) != hipSuccess ||
+    if (hipMalloc(&pts, std::max<uint64_t>(maxdn, 1) * 4) != hipSuccess || hipMalloc(&cen[0], maxdk * 4) != hipSuccess ||
         hipMalloc(&cen[1], maxdk * 4) != hipSuccess || hipMalloc(&sums, maxdk * 8) != hipSuccess ||
         hipMalloc(&labels, maxn * 4) != hipSuccess || hipMalloc(&masses, maxk * 4) != hipSuccess ||
         hipMalloc(&shift_d, 8) != hipSuccess) {
@@ -1654,7 +1720,9 @@ extern "C" int clk_serve_serial(clk_context *ctx, clk_serve_job *jobs, uint32_t 
             std::this_thread::sleep_for(std::chrono::duration<double>(
                 std::min(0.001, j.arrival - seconds_since(t0))));
         j.admitted = seconds_since(t0);
-        hipError_t e = hipMemcpyAsync(pts, j.points, j.n * j.d * 4, hipMemcpyHostToDevice, st);
+        const float *jp = j.device_points ? j.points : pts;
+        hipError_t e = j.device_points ? hipSuccess
+                                       : hipMemcpyAsync(pts, j.points, j.n * j.d * 4, hipMemcpyHostToDevice, st);
         if (e == hipSuccess)
             e = hipMemcpyAsync(cen[0], j.centroids, (size_t)j.d * j.k * 4, hipMemcpyHostToDevice, st);
         if (e != hipSuccess) {
@@ -1666,7 +1734,7 @@ extern "C" int clk_serve_serial(clk_context *ctx, clk_serve_job *jobs, uint32_t 
         double shift = 0.0;
         int cur = 0;
         while (it < j.max_iterations) {
-            rc = clk_lloyd_step(ctx, &cfg, j.d, j.n, j.k, pts, cen[cur], cen[cur ^ 1], sums, labels, masses,
+            rc = clk_lloyd_step(ctx, &cfg, j.d, j.n, j.k, jp, cen[cur], cen[cur ^ 1], sums, labels, masses,
                                 j.tolerance > 0.0 ? shift_d : nullptr, st);
             if (rc != CLK_OK)
                 break;

@@ -94,7 +94,10 @@ int clk_batch_autotune(clk_batch *b, uint32_t *chosen, void *stream);
 typedef struct clk_serve_job {
     uint64_t n;
     uint32_t d, k;
-    const float *points;      /* host [d][n]; pinned memory lets the copy overlap the GPU work */
+    const float *points;      /* host [d][n]; pinned memory lets the copy overlap the GPU work.
+                                 With device_points set: device memory, read in place (it must
+                                 stay valid until the job has finished) */
+    int32_t device_points;
     const float *centroids;   /* host [d][k], initial centroids */
     double arrival;           /* seconds after the start of the run (non-decreasing over the array) */
     uint32_t max_iterations;
