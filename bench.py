@@ -379,7 +379,10 @@ def batch(args):
     points = sum(n for n, _, _, _ in mix)
     bytes_per_iter = sum(n * (4 * d + 4) for n, d, _, _ in mix)
 
-    # batch engine: graph capture on the first step, then 25 replays timed
+    # tile sizes by the autotuner (dry passes over these jobs), then the
+    # batch engine: graph capture on the first step, 25 replays timed
+    tuned = eng.autotune(s)
+    log("autotuned tile_work = %d" % tuned)
     e0, e1 = hip.Event(), hip.Event()
     stream.synchronize()
     ctx.clear_records()
@@ -455,7 +458,8 @@ def batch(args):
                                "capped at N/16, sigma=1 blobs in [-10,10]^d, per-job seeds, first-x init, "
                                "%d iterations; batch engine (one persistent launch per iteration, HIP graph), "
                                "%s labeling" % (len(mix), iters, args.batch_labeling),
-                   "jobs": len(mix), "points": points, "labeling": args.batch_labeling},
+                   "jobs": len(mix), "points": points, "labeling": args.batch_labeling,
+                   "tile_work": tuned},
         "serial_per_job_loop": {"solves_per_s": len(mix) / (serial_ms * 1e-3), "total_ms": serial_ms},
         "speedup_vs_serial": serial_ms / batch_ms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -809,6 +809,7 @@ struct clk_batch {
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
     uint32_t stable_steps = 0;  // steps since the job set last changed
+    bool dry = false;           // autotuning: iterations label and accumulate only
     BClassState cls[clk::kClasses];
     uint32_t class_waves[clk::kClasses] = {};
 };
@@ -840,6 +841,15 @@ uint32_t tile_points(uint32_t tile_work, uint32_t d, uint32_t k)
     uint64_t t = (uint64_t)tile_work * k;
     t = std::max<uint64_t>(1024, std::min<uint64_t>(t, 16384));
     return (uint32_t)(t / 256 * 256);
+}
+
+// The autotuner's tile_work candidates; a job's partial rows are allocated for
+// the smallest, so the tuner can re-tile admitted jobs.
+constexpr uint32_t kTileWorkCandidates[] = {16, 32, 64, 128, 256};
+
+uint32_t row_tile_points(uint32_t tile_work, uint32_t d, uint32_t k)
+{
+    return tile_points(std::min(tile_work, kTileWorkCandidates[0]), d, k);
 }
 
 // First fit over the arena (slots hold [off, off+bytes)).
@@ -928,7 +938,7 @@ int rebuild(clk_batch *b, hipStream_t stream)
 {
     std::vector<BTile> tiles;
     std::vector<uint32_t> active;
-    std::vector<BCtl> ctl(clk::kClasses, BCtl{0, 0, 0, 0});
+    std::vector<BCtl> ctl(clk::kClasses, BCtl{0, 0, b->dry ? 1u : 0u, 0});
     for (uint32_t c = 0; c < clk::kClasses; ++c) {
         BClassState &C = b->cls[c];
         C = BClassState{};
@@ -1063,7 +1073,7 @@ int launch_step(clk_batch *b, hipStream_t stream)
 extern "C" uint64_t clk_batch_job_bytes(const clk_batch_config *cfg, uint64_t n, uint32_t d, uint32_t k)
 {
     const uint32_t tw = (cfg && cfg->tile_work) ? cfg->tile_work : 64;
-    return job_bytes(n, d, k, tile_points(tw, d, k));
+    return job_bytes(n, d, k, row_tile_points(tw, d, k));
 }
 
 extern "C" int clk_batch_create(clk_context *ctx, const clk_batch_config *cfg, clk_batch **out)
@@ -1184,7 +1194,7 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
         return CLK_ERR_NOMEM;
     }
     const uint32_t tp = tile_points(b->tile_work, d, k);
-    const uint64_t need = job_bytes(n, d, k, tp, !device_points);
+    const uint64_t need = job_bytes(n, d, k, row_tile_points(b->tile_work, d, k), !device_points);
     uint64_t at = 0;
     if (!arena_alloc(b, need, &at)) {
         clk::set_error("clk_batch_add: arena full");
@@ -1514,7 +1524,7 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
         const clk_serve_job &j = jobs[i];
         CLK_REQUIRE(j.n >= 1 && j.d >= 1 && j.d <= b->cfg.max_d && j.k >= 1 && j.k <= b->cfg.max_k, CLK_ERR_SIZE,
                     "clk_serve_run: a job's n, d or k is outside the batch's limits");
-        CLK_REQUIRE(job_bytes(j.n, j.d, j.k, tile_points(b->tile_work, j.d, j.k), j.device_points == 0) <=
+        CLK_REQUIRE(job_bytes(j.n, j.d, j.k, row_tile_points(b->tile_work, j.d, j.k), j.device_points == 0) <=
                         b->cfg.arena_bytes,
                     CLK_ERR_NOMEM, "clk_serve_run: a job is larger than the arena");
     }
@@ -1789,9 +1799,77 @@ extern "C" int clk_batch_autotune(clk_batch *b, uint32_t *chosen, void *stream)
 {
     CLK_REQUIRE(b, CLK_ERR_INVALID_ARGUMENT, "clk_batch_autotune: null batch");
     CLK_TRY(clk::enter(b->ctx));
-    (void)stream;
+    hipStream_t st = clk::as_stream(stream);
+    CLK_HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<BJob> dev(b->jobs.size());
+    CLK_HIP_CHECK(hipMemcpy(dev.data(), b->d_jobs, dev.size() * sizeof(BJob), hipMemcpyDeviceToHost));
+    uint32_t active = 0;
+    for (size_t s = 0; s < dev.size(); ++s) {
+        if (b->jobs[s].state != CLK_JOB_ACTIVE)
+            continue;
+        CLK_REQUIRE(dev[s].iter == 0, CLK_ERR_INVALID_ARGUMENT,
+                    "clk_batch_autotune: tune before the first iteration (the tiling fixes the summation order)");
+        ++active;
+    }
+    CLK_REQUIRE(active > 0, CLK_ERR_INVALID_ARGUMENT, "clk_batch_autotune: no active jobs to time");
+    // re-tile every active job (records on the host and the device)
+    auto retile = [&](uint32_t tw) -> int {
+        for (size_t s = 0; s < b->jobs.size(); ++s) {
+            BJob &J = b->jobs[s];
+            if (J.state != CLK_JOB_ACTIVE)
+                continue;
+            J.tile_pts = tile_points(tw, J.d, J.k);
+            J.ntiles = (uint32_t)clk::div_up(J.n, J.tile_pts);
+            dev[s].tile_pts = J.tile_pts;
+            dev[s].ntiles = J.ntiles;
+        }
+        CLK_HIP_CHECK(hipMemcpy(b->d_jobs, dev.data(), dev.size() * sizeof(BJob), hipMemcpyHostToDevice));
+        b->dirty = true;
+        return CLK_OK;
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    CLK_HIP_CHECK(hipEventCreate(&e0));
+    CLK_HIP_CHECK(hipEventCreate(&e1));
+    b->dry = true;
+    uint32_t best = b->tile_work;
+    float best_ms = 0.0f;
+    int rc = CLK_OK;
+    for (uint32_t tw : kTileWorkCandidates) {
+        rc = retile(tw);
+        if (rc == CLK_OK)
+            rc = rebuild(b, st);
+        if (rc == CLK_OK)
+            rc = launch_step(b, st);  // warm-up
+        if (rc != CLK_OK)
+            break;
+        float ms = 0.0f;
+        if (hipEventRecord(e0, st) != hipSuccess) {
+            rc = CLK_ERR_HIP;
+            break;
+        }
+        for (int i = 0; i < 3 && rc == CLK_OK; ++i)
+            rc = launch_step(b, st);
+        if (rc != CLK_OK || hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = rc != CLK_OK ? rc : CLK_ERR_HIP;
+            break;
+        }
+        if (best_ms == 0.0f || ms < best_ms) {
+            best_ms = ms;
+            best = tw;
+        }
+    }
+    b->dry = false;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == CLK_ERR_HIP)
+        clk::set_error("clk_batch_autotune: timing failed");
+    const int rc2 = retile(best);  // also on failure: a consistent tiling
+    b->tile_work = best;
+    CLK_TRY(rc);
+    CLK_TRY(rc2);
+    CLK_TRY(rebuild(b, st));
     if (chosen)
-        *chosen = b->tile_work;
-    clk::set_error("clk_batch_autotune: not available yet");
-    return CLK_ERR_UNSUPPORTED;
+        *chosen = best;
+    return CLK_OK;
 }

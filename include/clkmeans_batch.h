@@ -35,7 +35,8 @@ typedef struct clk_batch clk_batch;
 typedef struct clk_batch_config {
     uint32_t max_jobs;        /* slots */
     uint64_t arena_bytes;     /* device memory for job data (points, labels, centroids, partials) */
-    uint32_t tile_work;       /* target k*d*points per tile; 0 = default (autotuned value if set) */
+    uint32_t tile_work;       /* points per tile per cluster (tiles of 1024..16384 points); 0 = 64;
+                                 clk_batch_autotune picks one */
     uint32_t max_d;           /* largest d admitted (<= 64) */
     uint32_t max_k;           /* largest k admitted */
     int32_t labeling;         /* CLK_LABELING_UNROLL_VECTOR (plain FP32), CLK_LABELING_MFMA_BF16 or
@@ -84,9 +85,14 @@ int clk_batch_job_buffers(clk_batch *b, int32_t slot, const float **points, floa
  * iteration.  Jobs end in the same states as with clk_batch_step. */
 int clk_batch_run_serial(clk_batch *b, void *stream);
 
-/* Autotune tile_work on the current jobs: times label+accumulate passes
- * (without advancing any job) for a few tile sizes and keeps the fastest.
- * chosen may be NULL. */
+/* Autotune tile_work on the current jobs, before their first iteration:
+ * times label + accumulate passes (no job advances) for tile_work in
+ * {16, 32, 64, 128, 256}, re-tiles the jobs with the fastest and keeps it for
+ * jobs admitted later.  The tiling fixes each job's FP32 summation order, so
+ * a job's bits depend on tile_work: pass the chosen value as
+ * clk_batch_config.tile_work to reproduce them (the tuned value takes the
+ * place of a hand-edited .conf entry).  chosen may be NULL.
+ * CLK_ERR_INVALID_ARGUMENT if no job is active or one has iterated. */
 int clk_batch_autotune(clk_batch *b, uint32_t *chosen, void *stream);
 
 /* ---- serving: jobs arriving over time --------------------------------- */

@@ -182,3 +182,31 @@ def test_batch_mfma_near_ties_and_special_values(gpu_ctx, stream, labeling):
         np.testing.assert_array_equal(l, ref)
         np.testing.assert_array_equal(m, O.mass(ref, c_.shape[1]))
     b.close()
+
+
+def test_batch_autotune_picks_a_tiling_and_reproduces(gpu_ctx, stream):
+    jobs = [make_job(n, d, k, seed=200 + i) for i, (n, d, k) in enumerate(MIX[:8])]
+    b = clkmeans.Batch(gpu_ctx, max_jobs=16, arena_bytes=256 << 20, max_d=64, max_k=256, labeling="mfma_fp16")
+    slots = [b.add(p, c, 5, stream=stream.handle) for p, c in jobs]
+    chosen = b.autotune(stream.handle)
+    assert chosen in (16, 32, 64, 128, 256)
+    for _ in range(5):
+        b.step(stream.handle)
+    tuned = [b.take(s, stream.handle) for s in slots]
+    with pytest.raises(clkmeans.InvalidArgument):  # only before the first iteration
+        s2 = b.add(*jobs[0], 5, stream=stream.handle)
+        b.step(stream.handle)
+        b.autotune(stream.handle)
+    b.close()
+    # the same tile_work in a fresh batch: the same bits; the iteration is the reference's
+    again = clkmeans.Batch(gpu_ctx, max_jobs=16, arena_bytes=256 << 20, max_d=64, max_k=256, tile_work=chosen,
+                           labeling="mfma_fp16")
+    slots = [again.add(p, c, 5, stream=stream.handle) for p, c in jobs]
+    for _ in range(5):
+        again.step(stream.handle)
+    for s, (p, c), r in zip(slots, jobs, tuned):
+        r2 = again.take(s, stream.handle)
+        for a, b_ in zip(r[:3], r2[:3]):
+            np.testing.assert_array_equal(a.view(np.uint32), b_.view(np.uint32))
+        check_job(p, c, 5, r)
+    again.close()
