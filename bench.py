@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--workload", default="fused", choices=["fused", "dense", "batch", "serve"])
     ap.add_argument("--serve-jobs", type=int, default=1000, help="serve workload: jobs in the arrival stream")
     ap.add_argument("--rate", type=float, default=300.0, help="serve workload: Poisson arrival rate (jobs/s)")
+    ap.add_argument("--no-serial", action="store_true",
+                    help="batch workload: skip the serial per-job loop (for profiling the batch kernels)")
     ap.add_argument("--jobs", type=int, default=4096, help="batch workload: number of jobs")
     ap.add_argument("--batch-labeling", default="mfma_bf16",
                     help="batch workload: labeling strategy of the batch engine")
@@ -362,7 +364,7 @@ def batch(args):
     probe.close()
     arena = int(need * 1.02) + (64 << 20)
     eng = clkmeans.Batch(ctx, len(mix), arena, 32, 256, labeling=args.batch_labeling)
-    ser = clkmeans.Batch(ctx, len(mix), arena, 32, 256)
+    ser = clkmeans.Batch(ctx, len(mix), arena, 32, 256) if not args.no_serial else None
     t0 = time.time()
     slots = []
     checks = {}
@@ -371,7 +373,7 @@ def batch(args):
     for j, (n, d, k, seed) in enumerate(mix):
         pts, _ = clkmeans.generate_blobs(n, d, k, sigma=1.0, box=10.0, seed=seed, threads=threads)
         init = np.ascontiguousarray(pts[:, :k])
-        slots.append((eng.add(pts, init, iters, stream=s), ser.add(pts, init, iters, stream=s)))
+        slots.append((eng.add(pts, init, iters, stream=s), ser.add(pts, init, iters, stream=s) if ser else None))
         if j in check_ids:
             checks[j] = (pts, init)
     log("generated and loaded %d jobs (%.2f GB of points) in %.1fs" %
@@ -399,9 +401,10 @@ def batch(args):
     # serial per-job loop on the same GPU
     stream.synchronize()
     w0 = time.perf_counter()
-    ser.run_serial(s)
+    if ser:
+        ser.run_serial(s)
     stream.synchronize()
-    serial_ms = (time.perf_counter() - w0) * 1e3
+    serial_ms = (time.perf_counter() - w0) * 1e3 if ser else None
 
     # parity: every job batch == serial (labels, masses exact; centroids), a
     # sample against the oracle
@@ -409,6 +412,12 @@ def batch(args):
     oracle_ok = True
     for j, (a, b_) in enumerate(slots):
         ca, la, ma, ita, _ = eng.take(a, s)
+        if j in checks:
+            pts, init = checks[j]
+            rc, rl, rm = O.lloyd(pts, init, iters)
+            oracle_ok &= bool(np.array_equal(la, rl)) and bool(np.array_equal(ma, rm))
+        if not ser:
+            continue
         cb, lb, mb, itb, _ = ser.take(b_, s)
         lab_equal += int(bool(np.array_equal(la, lb)) and ita == itb == iters)
         mass_equal += int(np.array_equal(ma, mb))
@@ -416,12 +425,9 @@ def batch(args):
         if live.any():
             worst = max(worst, float(np.max(np.abs(ca[:, live].astype(np.float64) - cb[:, live]) /
                                             np.maximum(np.abs(cb[:, live]), 1e-3))))
-        if j in checks:
-            pts, init = checks[j]
-            rc, rl, rm = O.lloyd(pts, init, iters)
-            oracle_ok &= bool(np.array_equal(la, rl)) and bool(np.array_equal(ma, rm))
     eng.close()
-    ser.close()
+    if ser:
+        ser.close()
 
     cpu = None
     if not args.no_cpu:
@@ -460,8 +466,9 @@ def batch(args):
                                "%s labeling" % (len(mix), iters, args.batch_labeling),
                    "jobs": len(mix), "points": points, "labeling": args.batch_labeling,
                    "tile_work": tuned},
-        "serial_per_job_loop": {"solves_per_s": len(mix) / (serial_ms * 1e-3), "total_ms": serial_ms},
-        "speedup_vs_serial": serial_ms / batch_ms,
+        "serial_per_job_loop": {"solves_per_s": len(mix) / (serial_ms * 1e-3), "total_ms": serial_ms}
+                               if serial_ms else None,
+        "speedup_vs_serial": serial_ms / batch_ms if serial_ms else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
                      "kernel": "one batch iteration (size-class tile kernels + merge kernel)",
