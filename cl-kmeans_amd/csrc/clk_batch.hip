@@ -94,25 +94,28 @@ constexpr uint32_t kBatchPairs = 256;  // candidate pairs per wave and flush (MF
 __host__ __device__ inline uint32_t table_stride(uint32_t d) { return (d + 1) | 1u; }
 
 // LDS of the MFMA batch kernel for the largest admitted job (byte offsets):
-// table | norms | the work-group's table [kmax][ts(dmax)] | W pair lists |
-// W running minima [64] | W pair counters
+// table | norms | the accumulation tables [kmax][ts(dmax)] (one per wave, or
+// one for the work-group) | W pair lists | W running minima [64] | W pair
+// counters
 struct BLayout {
     uint32_t kmax, waves;
+    uint32_t priv;  // 1: one table per wave (no phases); 0: one per work-group
     uint32_t o_norm, o_tabs, o_pairs, o_min, o_count, bytes;
 };
 
-inline BLayout mfma_layout_for(uint32_t waves, uint32_t kmax, uint32_t dmax_feat)
+inline BLayout mfma_layout_for(uint32_t waves, uint32_t kmax, uint32_t dmax_feat, bool priv)
 {
     BLayout L{};
     const uint32_t nt16 = (kmax + mfma::kTile - 1) / mfma::kTile;
     const uint32_t nkb = (dmax_feat + mfma::kKBlock - 1) / mfma::kKBlock;
     L.kmax = kmax;
     L.waves = waves;
+    L.priv = priv ? 1u : 0u;
     size_t o = (size_t)nt16 * nkb * 64 * 16;
     L.o_norm = (uint32_t)o;
     o += (size_t)nt16 * mfma::kTile * 4;
     L.o_tabs = (uint32_t)o;
-    o += (size_t)kmax * table_stride(dmax_feat) * 4;
+    o += (size_t)(priv ? waves : 1) * kmax * table_stride(dmax_feat) * 4;
     L.o_pairs = (uint32_t)o;
     o += (size_t)waves * kBatchPairs * 4;
     o = (o + 15) / 16 * 16;
@@ -489,6 +492,9 @@ batch_tiles_kernel(const BJob *__restrict__ jobs, const BTile *__restrict__ tile
     });
 }
 
+// Read in place of a missing point or feature (the loads need no masks).
+__device__ const float kZeroWord = 0.0f;
+
 // Columns S, S+4, S+8, ... of a table row: feature sums v[f] for f < d and
 // the count in column d.
 template <int P, int S>
@@ -515,7 +521,9 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
     u16x8 *ltab = reinterpret_cast<u16x8 *>(lds);
     float *lnorm = reinterpret_cast<float *>(lds + Lo.o_norm);
     const uint32_t ts = table_stride(d);
-    float *T = reinterpret_cast<float *>(lds + Lo.o_tabs);  // the work-group's table
+    const bool priv = Lo.priv != 0;
+    float *tabs = reinterpret_cast<float *>(lds + Lo.o_tabs);
+    float *T = priv ? tabs + (size_t)wave * k * ts : tabs;  // this wave's or the work-group's table
     uint32_t *pl = reinterpret_cast<uint32_t *>(lds + Lo.o_pairs) + (size_t)wave * kBatchPairs;
     unsigned long long *wmin = reinterpret_cast<unsigned long long *>(lds + Lo.o_min) + (size_t)wave * 64;
     uint32_t *pc = reinterpret_cast<uint32_t *>(lds + Lo.o_count) + wave;
@@ -525,8 +533,8 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         ltab[e] = J.table[e];
     for (uint32_t c = tid; c < nt16 * kTile; c += blockDim.x)
         lnorm[c] = J.norm[c];
-    for (uint32_t i = tid; i < k * ts; i += blockDim.x)
-        T[i] = 0.0f;
+    for (uint32_t i = tid; i < (priv ? waves : 1u) * k * ts; i += blockDim.x)
+        tabs[i] = 0.0f;
     __syncthreads();
     const bool table_ok = J.stats[2] == 0.0f;
     const uint32_t kmask = (1u << J.prm.key_bits) - 1u;
@@ -547,14 +555,15 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
     auto fetch = [&](uint64_t b0) {
         uint64_t pid[kGroups];
         batch_ids(b0, pid);
-        fetch_points<NKB, false>(raw, J.pts, n, d, pid, h);
+        fetch_points<NKB>(raw, J.pts, n, d, pid, h, &kZeroWord);
     };
     if (wave < nb)
         fetch(first + (uint64_t)wave * 64);
-    // Rounds: in round rd wave w labels batch rd*W + w; then the waves add
-    // their run sums to the shared table in W phases, wave w owning column
-    // slice (w + phase) mod W in each phase (a fixed order per column; with
-    // 4 waves slice s holds the columns s, s+4, s+8, ...).
+    // Rounds: in round rd wave w labels batch rd*W + w.  With one table per
+    // wave the run sums go straight to it; with one table per work-group the
+    // waves add them in W phases, wave w owning column slice (w + phase) mod
+    // W in each phase (a fixed order per column; with 4 waves slice s holds
+    // the columns s, s+4, s+8, ...).
     const uint32_t rounds = (uint32_t)((nb + waves - 1) / waves);
     const uint32_t cols = d + 1, fs = (cols + waves - 1) / waves;
     for (uint32_t rd = 0; rd < rounds; ++rd) {
@@ -570,7 +579,7 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         batch_ids(base, pid);
         PointOps<NKB> x;
         float X2[kGroups], RX2[kGroups];
-        build_operands<FP16, NKB, false>(raw, x, X2, RX2, n, d, pid, h);
+        build_operands<FP16, NKB>(raw, x, X2, RX2);
         {
             const uint64_t pc = base + lane < end ? base + lane : first;
 #pragma unroll
@@ -670,6 +679,17 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
                 xc[f] = run_sum(__shfl(xc[f], (int)r.src), r);
         }
         float *row = T + (size_t)r.label * ts;
+        if (priv) {  // the wave's own table: no other wave touches it
+            if (r.last) {
+#pragma unroll
+                for (int f = 0; f < (int)P; ++f)
+                    if (f < (int)d)
+                        row[f] += xc[f];
+                row[d] += (float)r.count;
+            }
+            wave_lds_order();
+            continue;
+        }
         for (uint32_t ph = 0; ph < waves; ++ph) {
             __syncthreads();
             const uint32_t sl = (wave + ph) % waves;
@@ -694,7 +714,7 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         }
     }
     __syncthreads();
-    write_partial_row(T, 1, k, d, k, J.part + (size_t)tindex * ((size_t)k * (d + 1)));
+    write_partial_row(tabs, priv ? waves : 1u, k, d, k, J.part + (size_t)tindex * ((size_t)k * (d + 1)));
 }
 
 // Tiles of the jobs of one size class (MFMA labeling, NKB K-blocks).
@@ -710,7 +730,10 @@ batch_tiles_mfma_kernel(const BJob *__restrict__ jobs, const BTile *__restrict__
     });
 }
 
-constexpr uint32_t kClasses = 14;  // 5 d buckets x 2 k ranges (FP32) + 2 K-block counts x 2 k ranges (MFMA)
+// Size classes: FP32 scan, d bucket 4 / 8 / 16 / 32 / 64 (classes 0-9); MFMA,
+// d <= 16 / d <= 32 / d <= 64 (10-15); each with k <= 64 (even) or k > 64 (odd).
+constexpr uint32_t kClasses = 16;
+constexpr uint32_t kMfmaClass0 = 10, kMfmaNkb2Class = 14;
 
 // The merge of every active job's tile rows: one work-group per element
 // slice of a job (large jobs get many).
@@ -811,7 +834,8 @@ struct clk_batch {
     uint32_t stable_steps = 0;  // steps since the job set last changed
     bool dry = false;           // autotuning: iterations label and accumulate only
     BClassState cls[clk::kClasses];
-    uint32_t class_waves[clk::kClasses] = {};
+    uint32_t class_waves[clk::kClasses] = {};  // fixed at creation (they fix the summation order)
+    bool class_priv[clk::kClasses] = {};
 };
 
 namespace {
@@ -891,12 +915,12 @@ uint32_t class_of(uint32_t kind, uint32_t d, uint32_t k)
         const uint32_t i = dm == 4 ? 0 : dm == 8 ? 1 : dm == 16 ? 2 : dm == 32 ? 3 : 4;
         return 2 * i + big;
     }
-    return 10 + 2 * (clk::div_up(d, clk::mfma::kKBlock) - 1) + big;
+    return clk::kMfmaClass0 + 2 * (d <= 16 ? 0u : d <= 32 ? 1u : 2u) + big;
 }
 
 const void *class_kernel(const clk_batch *b, uint32_t c)
 {
-    if (c < 10) {
+    if (c < clk::kMfmaClass0) {
         switch (c / 2) {
         case 0: return (const void *)clk::batch_tiles_kernel<4>;
         case 1: return (const void *)clk::batch_tiles_kernel<8>;
@@ -905,7 +929,7 @@ const void *class_kernel(const clk_batch *b, uint32_t c)
         default: return (const void *)clk::batch_tiles_kernel<64>;
         }
     }
-    const bool nkb2 = c >= 12;
+    const bool nkb2 = c >= clk::kMfmaNkb2Class;
     if (b->fp16)
         return nkb2 ? (const void *)clk::batch_tiles_mfma_kernel<true, 2>
                     : (const void *)clk::batch_tiles_mfma_kernel<true, 1>;
@@ -915,23 +939,31 @@ const void *class_kernel(const clk_batch *b, uint32_t c)
 
 // LDS of class c for jobs up to kmax clusters and dmax features with the
 // given waves per work-group.
-size_t class_lds(uint32_t c, uint32_t waves, uint32_t kmax, uint32_t dmax, clk::BLayout *layout)
+size_t class_lds(uint32_t c, uint32_t waves, uint32_t kmax, uint32_t dmax, bool priv, clk::BLayout *layout)
 {
-    if (c < 10)
+    if (c < clk::kMfmaClass0)
         return clk::batch_lds_bytes(waves, kmax, 4u << (c / 2), dmax);
-    const clk::BLayout L = clk::mfma_layout_for(waves, kmax, dmax);
+    const clk::BLayout L = clk::mfma_layout_for(waves, kmax, dmax, priv);
     if (layout)
         *layout = L;
     return L.bytes;
 }
 
 // The most waves per work-group (<= 4) whose LDS fits; 0 if none does.
-uint32_t class_waves(const clk_batch *b, uint32_t c, uint32_t kmax, uint32_t dmax)
+uint32_t class_waves(const clk_batch *b, uint32_t c, uint32_t kmax, uint32_t dmax, bool priv)
 {
     for (uint32_t w = clk::kBatchMaxWaves; w >= 1; --w)
-        if (class_lds(c, w, kmax, dmax, nullptr) + 4096 <= b->ctx->lds_per_block)  // + static LDS
+        if (class_lds(c, w, kmax, dmax, priv, nullptr) + 4096 <= b->ctx->lds_per_block)  // + static LDS
             return w;
     return 0;
+}
+
+// MFMA classes: one table per wave (no barrier phases) when two 4-wave
+// work-groups still fit a CU's LDS, else one shared table per work-group.
+bool class_private(const clk_batch *b, uint32_t c, uint32_t kmax, uint32_t dmax)
+{
+    return c >= clk::kMfmaClass0 &&
+           2 * (class_lds(c, clk::kBatchMaxWaves, kmax, dmax, true, nullptr) + 4096) <= b->ctx->lds_per_block;
 }
 
 int rebuild(clk_batch *b, hipStream_t stream)
@@ -966,7 +998,7 @@ int rebuild(clk_batch *b, hipStream_t stream)
             clk::set_error("clk_batch: a size class does not fit the work-group's LDS");
             return CLK_ERR_UNSUPPORTED;
         }
-        C.lds = class_lds(c, C.waves, kmax, dmax, &C.layout);
+        C.lds = class_lds(c, C.waves, kmax, dmax, b->class_priv[c], &C.layout);
         const void *kern = class_kernel(b, c);
         if (C.lds > 65536)
             CLK_HIP_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)C.lds));
@@ -1029,7 +1061,7 @@ int launch_step(clk_batch *b, hipStream_t stream)
         const BTile *tl = b->d_tiles + C.tile0;
         BCtl *ctl = b->d_ctl + c;
         const dim3 g(C.grid), t(64 * C.waves);
-        if (c < 10) {
+        if (c < clk::kMfmaClass0) {
             switch (c / 2) {
             case 0: hipLaunchKernelGGL(clk::batch_tiles_kernel<4>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
             case 1: hipLaunchKernelGGL(clk::batch_tiles_kernel<8>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
@@ -1038,14 +1070,14 @@ int launch_step(clk_batch *b, hipStream_t stream)
             default: hipLaunchKernelGGL(clk::batch_tiles_kernel<64>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
             }
         } else if (b->fp16) {
-            if (c >= 12)
+            if (c >= clk::kMfmaNkb2Class)
                 hipLaunchKernelGGL((clk::batch_tiles_mfma_kernel<true, 2>), g, t, C.lds, stream, b->d_jobs, tl,
                                    ctl, C.layout);
             else
                 hipLaunchKernelGGL((clk::batch_tiles_mfma_kernel<true, 1>), g, t, C.lds, stream, b->d_jobs, tl,
                                    ctl, C.layout);
         } else {
-            if (c >= 12)
+            if (c >= clk::kMfmaNkb2Class)
                 hipLaunchKernelGGL((clk::batch_tiles_mfma_kernel<false, 2>), g, t, C.lds, stream, b->d_jobs, tl,
                                    ctl, C.layout);
             else
@@ -1099,8 +1131,10 @@ extern "C" int clk_batch_create(clk_context *ctx, const clk_batch_config *cfg, c
         for (uint32_t c = 0; c < clk::kClasses; ++c) {
             const bool big = (c & 1u) != 0;
             const uint32_t kmax = big ? cfg->max_k : std::min(cfg->max_k, kSmallK);
-            const uint32_t dcap = c < 10 ? (4u << (c / 2)) : (c >= 12 ? 64u : 32u);
-            b->class_waves[c] = class_waves(b, c, kmax, std::min(cfg->max_d, dcap));
+            const uint32_t dcap = c < clk::kMfmaClass0 ? (4u << (c / 2)) : (16u << ((c - clk::kMfmaClass0) / 2));
+            const uint32_t dmax = std::min(cfg->max_d, dcap);
+            b->class_priv[c] = class_private(b, c, kmax, dmax);
+            b->class_waves[c] = class_waves(b, c, kmax, dmax, b->class_priv[c]);
         }
         bool fits = true;
         for (uint32_t kind = 0; kind <= (b->mfma ? 1u : 0u); ++kind)

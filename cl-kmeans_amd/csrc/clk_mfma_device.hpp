@@ -285,49 +285,42 @@ __device__ __forceinline__ void load_points_keep(PointOps<NKB> &x, float (&X2)[k
 }
 
 // The loads of load_points alone, for software pipelining: raw[g][kb*8 + j]
-// = feature kb*32 + 8h + j of point pid[g], not yet masked (a missing point
-// reads point n-1, a missing feature feature 0; build_operands zeroes them).
-template <int NKB, bool FULL>
+// = feature kb*32 + 8h + j of point pid[g], already 0 for a missing point or
+// feature (read from the zero word *zero instead: the address select is paid
+// anyway, and build_operands then needs no masks).
+template <int NKB>
 __device__ __forceinline__ void fetch_points(float (&raw)[kGroups][NKB * 8], const float *__restrict__ pts,
                                              uint64_t n, uint32_t d, const uint64_t (&pid)[kGroups],
-                                             uint32_t h)
+                                             uint32_t h, const float *zero)
 {
 #pragma unroll
     for (int g = 0; g < kGroups; ++g) {
-        const uint64_t pc = pid[g] < n ? pid[g] : n - 1;
-        const float *row = pts + pc + (uint64_t)(8 * h) * n;  // feature 8h
+        const bool valid = pid[g] < n;
+        const float *row = pts + (valid ? pid[g] : 0) + (uint64_t)(8 * h) * n;  // feature 8h
 #pragma unroll
         for (int kb = 0; kb < NKB; ++kb)
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const uint64_t uoff = (uint64_t)(kb * kKBlock + j) * n;
-                if constexpr (FULL) {
-                    raw[g][kb * 8 + j] = row[uoff];
-                } else {
-                    const bool fv = (uint32_t)(kb * kKBlock + 8 * h + j) < d;
-                    raw[g][kb * 8 + j] = *(fv ? row + uoff : pts + pc);
-                }
+                const bool ok = valid && (uint32_t)(kb * kKBlock + 8 * h + j) < d;
+                raw[g][kb * 8 + j] = *(ok ? row + (uint64_t)(kb * kKBlock + j) * n : zero);
             }
     }
 }
 
 // The operands, X2 and RX2 of load_points from fetch_points' values.
-template <bool FP16, int NKB, bool FULL>
+template <bool FP16, int NKB>
 __device__ __forceinline__ void build_operands(const float (&raw)[kGroups][NKB * 8], PointOps<NKB> &x,
-                                               float (&X2)[kGroups], float (&RX2)[kGroups], uint64_t n,
-                                               uint32_t d, const uint64_t (&pid)[kGroups], uint32_t h)
+                                               float (&X2)[kGroups], float (&RX2)[kGroups])
 {
 #pragma unroll
     for (int g = 0; g < kGroups; ++g) {
-        const bool valid = pid[g] < n;
         float s = 0.0f, r = 0.0f;
 #pragma unroll
         for (int kb = 0; kb < NKB; ++kb) {
             u16x8 vb;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const bool fv = FULL || (uint32_t)(kb * kKBlock + 8 * h + j) < d;
-                const float v = (valid && fv) ? raw[g][kb * 8 + j] : 0.0f;
+                const float v = raw[g][kb * 8 + j];
                 s = __builtin_fmaf(v, v, s);
                 vb[j] = to16<FP16>(v);
                 const float res = v - from16<FP16>(vb[j]);
