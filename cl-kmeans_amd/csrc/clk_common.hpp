@@ -162,6 +162,37 @@ __device__ __forceinline__ void argmin_lds(const float (&x)[DMAX][VEC],
     }
 }
 
+// As argmin_lds over centroids c0 .. c0+cn-1 staged in LDS as
+// cen[(c - c0) * DMAX + f], continuing a running minimum (best, lab) over the
+// centroids before c0: visiting chunks in increasing order with strict '<'
+// gives the same label as one pass.
+template <int DMAX, int VEC>
+__device__ __forceinline__ void argmin_lds_chunk(const float (&x)[DMAX][VEC],
+                                                 const float *__restrict__ cen, uint32_t c0,
+                                                 uint32_t cn, float (&best)[VEC],
+                                                 uint32_t (&lab)[VEC])
+{
+    for (uint32_t c = 0; c < cn; ++c) {
+        float cc[DMAX];
+#pragma unroll
+        for (int f = 0; f < DMAX; ++f)
+            cc[f] = cen[c * DMAX + f];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            float t = x[0][v] - cc[0];
+            float dist = t * t;
+#pragma unroll
+            for (int f = 1; f < DMAX; ++f) {
+                t = x[f][v] - cc[f];
+                dist = __builtin_fmaf(t, t, dist);
+            }
+            bool lt = dist < best[v];
+            best[v] = lt ? dist : best[v];
+            lab[v] = lt ? c0 + c : lab[v];
+        }
+    }
+}
+
 // Load VEC consecutive points (p % VEC == 0, rows aligned) of DMAX features;
 // features >= d are 0.  EXACT: d == DMAX (no per-feature guards).
 template <int DMAX, int VEC, bool EXACT = false>

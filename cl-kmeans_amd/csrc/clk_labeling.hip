@@ -35,6 +35,52 @@ label_lds_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t
     }
 }
 
+// k*d beyond LDS: the centroids pass through LDS in chunks of `chunk`
+// (feature-padded rows [chunk][DMAX]) while each work-item keeps its VEC
+// points and their running minima in registers.  The loop over point groups
+// is work-group uniform (every work-item takes part in every staging).
+template <int DMAX, int VEC>
+__global__ void __launch_bounds__(256)
+label_chunked_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
+                     const float *__restrict__ cen, uint32_t *__restrict__ labels, uint32_t chunk)
+{
+    extern __shared__ __align__(16) float cen_lds[];
+    const uint64_t per = (uint64_t)blockDim.x * VEC;
+    for (uint64_t base = (uint64_t)blockIdx.x * per; base < n; base += (uint64_t)gridDim.x * per) {
+        const uint64_t p = base + (uint64_t)threadIdx.x * VEC;
+        const bool valid = p < n;  // n % VEC == 0
+        float x[DMAX][VEC];
+        if (valid) {
+            load_points<DMAX, VEC>(pts, n, d, p, x);
+        } else {
+#pragma unroll
+            for (int f = 0; f < DMAX; ++f)
+#pragma unroll
+                for (int v = 0; v < VEC; ++v)
+                    x[f][v] = 0.0f;
+        }
+        float best[VEC];
+        uint32_t lab[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            best[v] = FLT_MAX;
+            lab[v] = 0;
+        }
+        for (uint32_t c0 = 0; c0 < k; c0 += chunk) {
+            const uint32_t cn = min(chunk, k - c0);
+            __syncthreads();  // the previous chunk has been read
+            for (uint32_t i = threadIdx.x; i < cn * DMAX; i += blockDim.x) {
+                const uint32_t c = i / DMAX, f = i % DMAX;
+                cen_lds[i] = f < d ? cen[(uint64_t)f * k + c0 + c] : 0.0f;
+            }
+            __syncthreads();
+            argmin_lds_chunk<DMAX, VEC>(x, cen_lds, c0, cn, best, lab);
+        }
+        if (valid)
+            store_labels<VEC>(labels, p, lab);
+    }
+}
+
 // Centroids read from global memory with work-group-uniform addresses (any
 // k).  DMAX > 0 keeps the point in registers; DMAX == 0 re-reads it (d > 64).
 template <int DMAX>
@@ -132,6 +178,36 @@ LabelKernel pick_label_lds(int dmax, int vec)
     }
 }
 
+using ChunkKernel = void (*)(const float *, uint64_t, uint32_t, uint32_t, const float *, uint32_t *,
+                             uint32_t);
+
+template <int DMAX>
+ChunkKernel pick_chunk_vec(int vec)
+{
+    constexpr int cap = label_vec_cap(DMAX);
+    if constexpr (cap >= 4)
+        if (vec >= 4)
+            return label_chunked_kernel<DMAX, 4>;
+    if constexpr (cap >= 2)
+        if (vec >= 2)
+            return label_chunked_kernel<DMAX, 2>;
+    return label_chunked_kernel<DMAX, 1>;
+}
+
+ChunkKernel pick_label_chunked(int dmax, int vec)
+{
+    switch (dmax) {
+    case 1: return pick_chunk_vec<1>(vec);
+    case 2: return pick_chunk_vec<2>(vec);
+    case 4: return pick_chunk_vec<4>(vec);
+    case 8: return pick_chunk_vec<8>(vec);
+    case 16: return pick_chunk_vec<16>(vec);
+    case 32: return pick_chunk_vec<32>(vec);
+    case 64: return pick_chunk_vec<64>(vec);
+    default: return nullptr;
+    }
+}
+
 LabelKernel pick_label_global(int dmax)
 {
     switch (dmax) {
@@ -172,6 +248,20 @@ int launch_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, 
         KernelTimer t(ctx, "LabelingUnrollVector", stream);
         hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3((unsigned)B), cen_bytes, stream,
                            points, n, (uint32_t)d, (uint32_t)k, centroids, labels);
+        CLK_HIP_CHECK(hipGetLastError());
+        return CLK_OK;
+    }
+    if (dmax) {  // centroid chunks through LDS (64 KiB: two work-groups per CU)
+        const int vec = std::min(pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels),
+                                 label_vec_cap(dmax));
+        ChunkKernel kern = pick_label_chunked(dmax, vec);
+        const uint32_t chunk = (uint32_t)std::min<uint64_t>(k, (64u << 10) / (4u * dmax));
+        const size_t lds = (size_t)chunk * dmax * 4;
+        uint64_t grid = auto_grid(ctx, (const void *)kern, 256, lds, n / vec);
+        grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, 1u << 20));
+        KernelTimer t(ctx, "LabelingChunked", stream);
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), lds, stream, points, n, (uint32_t)d,
+                           (uint32_t)k, centroids, labels, chunk);
         CLK_HIP_CHECK(hipGetLastError());
         return CLK_OK;
     }

@@ -122,6 +122,27 @@ def test_labeling_matches_oracle(gpu_ctx, n, d, k, vec):
     np.testing.assert_array_equal(dl.to_host(), O.label(pts, cen))
 
 
+@pytest.mark.parametrize("n,d,k,vec", [(4096, 64, 1024, 4), (3001, 30, 2000, 4), (2048, 8, 6000, 2),
+                                       (1000, 3, 20000, 4)])
+def test_labeling_centroid_chunks_match_oracle(gpu_ctx, n, d, k, vec):
+    """k*d beyond LDS: centroids pass through LDS in chunks (64 KiB); a tie
+    across a chunk boundary keeps the lower index, NaN centroids never win."""
+    pts = blobs(n, d, 32, seed=5)
+    rng = np.random.default_rng(d + k)
+    cen = (rng.standard_normal((d, k)) * 5).astype(np.float32)
+    chunk = (64 << 10) // (4 * {3: 4, 8: 8, 30: 32, 64: 64}[d])
+    cen[:, chunk + 3] = cen[:, 3]            # duplicate in the next chunk: label 3 wins
+    cen[:, 1] = pts[:, 0]                    # point 0 sits on centroid 1 (and on its copy below)
+    cen[:, chunk + 1] = pts[:, 0]
+    cen[:, chunk - 1] = np.nan               # an empty cluster at a chunk's end
+    dl = zeros(n, np.uint32)
+    gpu_ctx.labeling(gpu_ctx.config(LAB, "unroll_vector", vector_length=vec), d, n, k, dev(pts), dev(cen), dl)
+    hip.synchronize()
+    got = dl.to_host()
+    np.testing.assert_array_equal(got, O.label(pts, cen))
+    assert got[0] == 1 and not np.any(got == chunk - 1) and not np.any(got == chunk + 3)
+
+
 @pytest.mark.parametrize("strategy", ["global_atomic", "part_global", "part_local", "part_private"])
 @pytest.mark.parametrize("n,k,kw", [(1 << 18, 10, {}), (100003, 7, {}), (5000, 1000, {}),
                                     (64, 3, {"global_size": 128, "local_size": 1, "vector_length": 8}),

@@ -63,7 +63,7 @@ def test_strategy_matches_oracle(data, pipeline, stages):
 def test_dense_three_stage_matches_oracle(labeling):
     # d=64, k=1024: MFMA labeling with refinement and the feature-slice
     # centroid update, three iterations against the oracle's FP64-merge loop.
-    n, d, k = 1 << 14, 64, 1024
+    n, d, k = 1 << 13, 64, 1024  # the single-threaded oracle: ~1.6e9 fma
     pts, _ = clkmeans.generate_blobs(n, d, k, sigma=0.5, box=1.0, seed=3)
     init = O.first_x(pts, k)
     rc, rl, rm = O.lloyd(pts, init, 3)
