@@ -205,7 +205,7 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
 
     // Variant by LDS fit (the reference picks its GLOBAL_MEM variant the same
     // way, fused_feature_sum.hpp:315-337): private tile, shared tile, or the
-    // global-accumulation pass for k*d beyond LDS.
+    // separate labeling, mass and centroid-update stages for k*d beyond LDS.
     const uint64_t cen_bytes = dmax ? ((k * dmax + 3) & ~3ull) * 4 : 0;
     const uint64_t rows = k * (d + 1);
     const uint64_t priv_bytes = cen_bytes + rows * (uint64_t)B * 4;
@@ -219,8 +219,11 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
             CLK_TRY(clk_fill_zero(ctx, masses, k * sizeof(uint32_t), stream));
         }
         if (n) {
+            // labels, then the fixed-order mass histogram and centroid sums
+            // (feature slices for large k*d; global atomics only past those)
             CLK_TRY(launch_labeling(ctx, cfg, d, n, k, points, old_centroids, labels, stream));
-            CLK_TRY(launch_accumulate_global(ctx, d, n, k, points, labels, sums, masses, stream));
+            CLK_TRY(run_mass_update(ctx, nullptr, n, k, labels, masses, stream));
+            CLK_TRY(run_centroid_update(ctx, nullptr, d, n, k, points, sums, labels, stream));
         }
         if (mode == 2)
             return clk_matrix_divide_row(ctx, d, k, sums, masses, new_centroids,

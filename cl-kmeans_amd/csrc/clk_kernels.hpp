@@ -50,6 +50,14 @@ int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, ui
 
 // sums[f,c] += x_f and masses[c] += 1 for every point, with FP64 / u32
 // global atomics (any k*d; summation order not fixed).  masses may be null.
+// The mass and centroid-update stages (accumulate into masses / sums); a
+// null cfg selects part_private / cluster_merge.
+int run_mass_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t n, uint64_t k,
+                    const uint32_t *labels, uint32_t *masses, hipStream_t stream);
+int run_centroid_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64_t n,
+                        uint64_t k, const float *points, double *sums, const uint32_t *labels,
+                        hipStream_t stream);
+
 int launch_accumulate_global(clk_context *ctx, uint64_t d, uint64_t n, uint64_t k,
                              const float *points, const uint32_t *labels, double *sums,
                              uint32_t *masses, hipStream_t stream);

@@ -62,6 +62,7 @@ FUSED_CASES = [
     (2000, 5, 7, {"local_size": 64, "vector_length": 1}),
     (2048, 64, 8, {}),
     (1500, 100, 6, {}),                   # d > 64: global path
+    (4000, 64, 700, {}),                  # k*d beyond LDS: labeling + mass + slice stages
     (20000, 4, 10, {"global_size": 4096, "local_size": 32, "vector_length": 4}),  # GPU .conf geometry
     (7, 4, 10, {}),                       # n < k
 ]
