@@ -6,6 +6,7 @@
 // Box-Muller transform in double precision, rounded once to float.
 #include "clkmeans.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <thread>
@@ -37,29 +38,34 @@ inline double normal(uint64_t seed, uint64_t ctr)
 
 }  // namespace
 
-extern "C" int clk_generate_blobs(uint64_t n, uint64_t d, uint64_t k, double sigma, double box,
-                                  uint64_t seed, float *host_points, float *host_centres,
-                                  int threads)
+namespace {
+
+// Blob centres uniform in [lo, hi]^d; point p of blob b(p) = centre + sigma *
+// N(0, 1) per feature; b(p) = p mod k (interleaved) or p / ceil(n/k)
+// (grouped, the reference generator's layout, cluster_generator.cpp:178-195).
+int generate(uint64_t n, uint64_t d, uint64_t k, double sigma, double lo, double hi, uint64_t seed, bool grouped,
+             float *host_points, float *host_centres, int threads)
 {
-    if (d == 0 || k == 0 || (n > 0 && host_points == nullptr))
+    if (d == 0 || k == 0 || (n > 0 && host_points == nullptr) || !(hi >= lo))
         return CLK_ERR_INVALID_ARGUMENT;
     std::vector<double> centre(d * k);
     for (uint64_t f = 0; f < d; ++f)
         for (uint64_t c = 0; c < k; ++c) {
             double u = uniform(seed, 0, f * k + c);
-            centre[f * k + c] = -box + 2.0 * box * u;
+            centre[f * k + c] = lo + (hi - lo) * u;
         }
     if (host_centres)
         for (uint64_t i = 0; i < d * k; ++i)
             host_centres[i] = (float)centre[i];
     if (threads < 1)
         threads = 1;
+    const uint64_t per_blob = (n + k - 1) / k;
     const uint64_t total = n * d;
     const uint64_t per = (total + threads - 1) / threads;
     auto work = [&](uint64_t e0, uint64_t e1) {
         for (uint64_t e = e0; e < e1; ++e) {
             const uint64_t f = e / n, p = e - f * n;
-            const uint64_t c = p % k;
+            const uint64_t c = grouped ? std::min(k - 1, p / per_blob) : p % k;
             host_points[e] = (float)(centre[f * k + c] + sigma * normal(seed, e));
         }
     };
@@ -73,4 +79,21 @@ extern "C" int clk_generate_blobs(uint64_t n, uint64_t d, uint64_t k, double sig
     for (auto &th : pool)
         th.join();
     return CLK_OK;
+}
+
+}  // namespace
+
+extern "C" int clk_generate_blobs(uint64_t n, uint64_t d, uint64_t k, double sigma, double box,
+                                  uint64_t seed, float *host_points, float *host_centres,
+                                  int threads)
+{
+    return generate(n, d, k, sigma, -box, box, seed, false, host_points, host_centres, threads);
+}
+
+extern "C" int clk_generate_points(uint64_t n, uint64_t d, uint64_t k, double radius, double domain_min,
+                                   double domain_max, uint64_t seed, int grouped, float *host_points,
+                                   float *host_centres, int threads)
+{
+    return generate(n, d, k, radius, domain_min, domain_max, seed, grouped != 0, host_points, host_centres,
+                    threads);
 }

@@ -255,6 +255,17 @@ int clk_generate_blobs(uint64_t n, uint64_t d, uint64_t k, double sigma,
                        double box, uint64_t seed, float *host_points,
                        float *host_centres, int threads);
 
+/* The generator CLI's form (replaces generator.cpp:36-53 with
+ * cluster_generator.cpp:39-100): centres uniform in [domain_min,
+ * domain_max]^d, noise N(0, radius) per feature (the reference draws
+ * N(-radius, radius), cluster_generator.cpp:69); grouped != 0 puts the
+ * points of a blob next to each other (blob of p = p / ceil(n/k)) as the
+ * reference does, else interleaved as clk_generate_blobs. */
+int clk_generate_points(uint64_t n, uint64_t d, uint64_t k, double radius,
+                        double domain_min, double domain_max, uint64_t seed,
+                        int grouped, float *host_points, float *host_centres,
+                        int threads);
+
 #ifdef __cplusplus
 }
 #endif

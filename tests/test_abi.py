@@ -73,3 +73,39 @@ def test_generator_statistics():
     assert abs(resid.std() - 1.0) < 0.01
     # equal cluster sizes, interleaved: point p belongs to blob p mod k
     assert np.ptp(np.bincount(blob, minlength=k)) <= 1
+
+
+def test_generator_cli_writes_the_reference_format(tmp_path):
+    """clk_generate: generator.cpp's options and .bin layout (u64 d, u64 0,
+    u64 n, float32 [d][n]); n = size/4/d rounded down to a multiple of k,
+    then of the divisor (cluster_generator.cpp:55-62); grouped blobs by
+    default; deterministic."""
+    exe = os.path.join(REPO, "cl-kmeans_amd", "clk_generate")
+    if not os.path.exists(exe):
+        pytest.skip("clk_generate not built")
+    out = [str(tmp_path / ("g%d.bin" % i)) for i in range(3)]
+    args = ["--size", "1", "--features", "3", "--clusters", "4", "--radius", "0.5",
+            "--domain_min", "-10", "--domain_max", "10"]
+    subprocess.run([exe] + args + [out[0]], check=True)
+    subprocess.run([exe] + args + [out[1]], check=True)
+    subprocess.run([exe] + args + ["--interleaved", out[2]], check=True)
+    head = np.fromfile(out[0], dtype=np.uint64, count=3)
+    n = (1 << 20) // 4 // 3 // 4 * 4
+    n -= n % 8
+    assert head.tolist() == [3, 0, n]
+    assert os.path.getsize(out[0]) == 24 + 4 * 3 * n
+    x = np.fromfile(out[0], dtype=np.float32, offset=24).reshape(3, n)
+    assert open(out[0], "rb").read() == open(out[1], "rb").read()
+    assert np.all(np.abs(x) < 10 + 0.5 * 8)
+    per = -(-n // 4)
+    means = np.stack([x[:, b * per:(b + 1) * per].mean(axis=1) for b in range(4)], axis=1)
+    spread = np.stack([x[:, b * per:(b + 1) * per].std(axis=1) for b in range(4)], axis=1)
+    assert np.all(spread < 0.6) and np.all(spread > 0.4)  # each group is one blob (sigma 0.5)
+    y = np.fromfile(out[2], dtype=np.float32, offset=24).reshape(3, n)
+    np.testing.assert_allclose(y[:, :4].mean(axis=1) * 0, 0)
+    assert np.all(np.abs(y[:, 0:4] - means) < 3)  # interleaved: point p in blob p mod 4
+    csv = str(tmp_path / "g.csv")
+    subprocess.run([exe, "--csv", "--size", "1", "--features", "2", "--clusters", "3", csv], check=True)
+    rows = open(csv).read().splitlines()
+    m = (1 << 20) // 4 // 2 // 3 * 3
+    assert len(rows) == m - m % 8 and len(rows[0].split(",")) == 2
