@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--workload", default="fused", choices=["fused", "dense", "batch", "serve"])
     ap.add_argument("--serve-jobs", type=int, default=1000, help="serve workload: jobs in the arrival stream")
     ap.add_argument("--rate", type=float, default=300.0, help="serve workload: Poisson arrival rate (jobs/s)")
+    ap.add_argument("--large-frac", type=float, default=0.02,
+                    help="serve workload: share of large jobs (N=2^24, d=32, k=256)")
     ap.add_argument("--no-serial", action="store_true",
                     help="batch workload: skip the serial per-job loop (for profiling the batch kernels)")
     ap.add_argument("--jobs", type=int, default=4096, help="batch workload: number of jobs")
@@ -496,7 +498,7 @@ def serve(args):
     count = args.serve_jobs
     rng = np.random.default_rng(7)
     mix = clkmeans.job_mix(count, seed=1)
-    large = set(rng.choice(count, size=max(1, round(0.02 * count)), replace=False).tolist())
+    large = set(rng.choice(count, size=int(round(args.large_frac * count)), replace=False).tolist())
     ctx = clkmeans.Context(0)
     stream = hip.Stream()
     t0 = time.time()
@@ -526,7 +528,7 @@ def serve(args):
     log("generated %d jobs (%d large, %.2f GB of points) in %.1fs" %
         (count, len(large), sum(n * d * 4 for n, d, _ in specs) / 1e9, time.time() - t0))
     probe = clkmeans.Batch(ctx, 1, 1 << 20, 32, 256)
-    big_bytes = probe.job_bytes(1 << 24, 32, 256)
+    big_bytes = probe.job_bytes(1 << 24, 32, 256) if large else probe.job_bytes(1 << 18, 32, 256)
     probe.close()
 
     def requests(resident):
@@ -543,7 +545,7 @@ def serve(args):
         ss = clkmeans.serve_serial(ctx, sreqs, stream.handle)
         same_iters = sum(int(a.iterations == b.iterations) for a, b in zip(reqs, sreqs))
         same_mass = sum(int(np.array_equal(a.out_masses, b.out_masses)) for a, b in zip(reqs, sreqs))
-        lat_large = [reqs[j].latency for j in large]
+        lat_large = [reqs[j].latency for j in large] or [0.0]
         return {
             "served": {"solves_per_s": st["solves_per_s"], "makespan_s": st["makespan"], "p50_s": st["p50"],
                        "p99_s": st["p99"], "mean_latency_s": st["mean_latency"],
@@ -573,9 +575,9 @@ def serve(args):
                  args.batch_labeling.split("_")[1] + "-mfma+f32-exact",
         "data": "synthetic",
         "config": {"workload": "%d jobs, Poisson arrivals at %.0f jobs/s: the batched mix (N log-uniform "
-                               "[2^10,2^18], d in {3,8,16,32}, k log-uniform [4,256] capped at N/16) plus %d "
-                               "large jobs (N=2^24, d=32, k=256; 4 distinct data sets); each until max shift "
-                               "< 1e-4 or 50 iterations; points already in HBM (read in place)"
+                               "[2^10,2^18], d in {3,8,16,32}, k log-uniform [4,256] capped at N/16) with %d of "
+                               "them large jobs (N=2^24, d=32, k=256; 4 distinct data sets); each until max "
+                               "shift < 1e-4 or 50 iterations; points already in HBM (read in place)"
                                % (count, args.rate, len(large)),
                    "jobs": count, "rate": args.rate, "labeling": args.batch_labeling},
         **res,

@@ -54,6 +54,10 @@ namespace clk {
 namespace {
 
 struct BJob {
+    // status, first so that a strided copy of 16 bytes per record reads it
+    uint32_t iter;      // iterations done
+    uint32_t state;     // CLK_JOB_*
+    double shift;       // max centroid shift of the last iteration
     const float *pts;   // [d][n]
     float *cen[2];      // [d][k], current = cen[iter & 1]
     uint32_t *labels;   // [n]
@@ -62,11 +66,10 @@ struct BJob {
     uint64_t n;
     uint32_t d, k;
     uint32_t tile0, ntiles, tile_pts;
-    uint32_t max_iter, iter;
-    uint32_t state;     // CLK_JOB_*
+    uint32_t max_iter;
     uint32_t dmax;      // register bucket of d
     uint32_t kind;      // 0: FP32 labeling, 1: MFMA labeling
-    double tol, shift;
+    double tol;
     // MFMA labeling: 16-bit table [nt16][nkb][64], norms [nt16*16], stats[16]
     mfma::u16x8 *table;
     float *norm;
@@ -773,6 +776,21 @@ batch_finish_kernel(BJob *__restrict__ jobs, const uint32_t *__restrict__ active
     }
 }
 
+// Result copies of the serving loop: one work-group per (source, words,
+// destination) triple; destinations in device-mapped pinned host memory.
+struct GatherDesc {
+    const uint32_t *src;
+    uint32_t *dst;
+    uint64_t words;
+};
+
+__global__ void __launch_bounds__(256) gather_kernel(const GatherDesc *__restrict__ descs)
+{
+    const GatherDesc g = descs[blockIdx.x];
+    for (uint64_t i = threadIdx.x; i < g.words; i += blockDim.x)
+        g.dst[i] = g.src[i];
+}
+
 // The table of a newly admitted job (from its initial centroids).
 template <bool FP16>
 __global__ void __launch_bounds__(256) batch_build_table_kernel(BJob *__restrict__ jobs, uint32_t slot)
@@ -834,6 +852,7 @@ struct clk_batch {
     uint32_t stable_steps = 0;  // steps since the job set last changed
     bool dry = false;           // autotuning: iterations label and accumulate only
     BClassState cls[clk::kClasses];
+    BJob *h_rec = nullptr;  // pinned staging of job records for activation
     uint32_t class_waves[clk::kClasses] = {};  // fixed at creation (they fix the summation order)
     bool class_priv[clk::kClasses] = {};
 };
@@ -1192,6 +1211,8 @@ extern "C" int clk_batch_destroy(clk_batch *b)
         (void)hipFree(b->d_active);
     if (b->d_units)
         (void)hipFree(b->d_units);
+    if (b->h_rec)
+        (void)hipHostFree(b->h_rec);
     if (b->arena)
         (void)hipFree(b->arena);
     delete b;
@@ -1210,7 +1231,7 @@ constexpr uint32_t kJobStaged = 3;
 // visible to the kernels.
 int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *host_points,
               const float *host_centroids, uint32_t max_iterations, double tolerance, int32_t *slot,
-              hipStream_t st, bool device_points = false)
+              hipStream_t st, bool device_points = false, bool zero_outputs = true)
 {
     CLK_REQUIRE(n >= 1 && n < (1ull << 32), CLK_ERR_SIZE, "clk_batch_add: n must be 1..2^32-1");
     CLK_REQUIRE(d >= 1 && d <= b->cfg.max_d && k >= 1 && k <= b->cfg.max_k, CLK_ERR_SIZE,
@@ -1276,8 +1297,10 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
         CLK_HIP_CHECK(
             hipMemcpyAsync(const_cast<float *>(J.pts), host_points, n * d * 4, hipMemcpyHostToDevice, st));
     CLK_HIP_CHECK(hipMemcpyAsync(J.cen[0], host_centroids, (size_t)d * k * 4, hipMemcpyHostToDevice, st));
-    CLK_HIP_CHECK(hipMemsetAsync(J.masses, 0, (size_t)k * 4, st));
-    CLK_HIP_CHECK(hipMemsetAsync(J.labels, 0, n * 4, st));
+    if (zero_outputs || max_iterations == 0) {  // else the first iteration writes both
+        CLK_HIP_CHECK(hipMemsetAsync(J.masses, 0, (size_t)k * 4, st));
+        CLK_HIP_CHECK(hipMemsetAsync(J.labels, 0, n * 4, st));
+    }
     b->jobs[s] = J;
     b->jobs[s].state = kJobStaged;
     b->staged_state[s] = J.state;
@@ -1291,11 +1314,15 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
 // MFMA labeling, its first 16-bit table (stream-ordered after the copies when
 // st is the staging stream; otherwise the caller orders the two streams).
 // No step may be in flight on another stream.
-int activate_job(clk_batch *b, int32_t s, hipStream_t st)
+int activate_job(clk_batch *b, int32_t s, hipStream_t st, bool sync = true)
 {
     BJob &J = b->jobs[s];
     J.state = b->staged_state[s];
-    CLK_HIP_CHECK(hipMemcpyAsync(b->d_jobs + s, &J, sizeof(BJob), hipMemcpyHostToDevice, st));
+    if (!b->h_rec) {  // pinned copies of the records: the copy runs after the call returns
+        CLK_HIP_CHECK(hipHostMalloc((void **)&b->h_rec, b->jobs.size() * sizeof(BJob), hipHostMallocDefault));
+    }
+    b->h_rec[s] = J;
+    CLK_HIP_CHECK(hipMemcpyAsync(b->d_jobs + s, b->h_rec + s, sizeof(BJob), hipMemcpyHostToDevice, st));
     if (J.kind == 1) {
         if (b->fp16)
             hipLaunchKernelGGL(clk::batch_build_table_kernel<true>, dim3(1), dim3(256), 0, st, b->d_jobs,
@@ -1305,8 +1332,10 @@ int activate_job(clk_batch *b, int32_t s, hipStream_t st)
                                (uint32_t)s);
         CLK_HIP_CHECK(hipGetLastError());
     }
-    // the record is copied from host memory: keep it unchanged until the copy ends
-    CLK_HIP_CHECK(hipStreamSynchronize(st));
+    // the pinned record is not rewritten before the slot is taken again,
+    // which synchronizes the stream first
+    if (sync)
+        CLK_HIP_CHECK(hipStreamSynchronize(st));
     b->dirty = true;
     return CLK_OK;
 }
@@ -1564,9 +1593,26 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
     }
     const uint32_t slots = (uint32_t)b->jobs.size();
     hipStream_t cs = nullptr;  // admission copies, concurrent with the iterations
+    hipStream_t rs = nullptr;  // result copies, concurrent with the next iteration
     hipEvent_t step_done = nullptr;
-    uint32_t *h_state = nullptr;
+    struct JobStatus {  // the first 16 bytes of a job record
+        uint32_t iter, state;
+        double shift;
+    };
+    static_assert(offsetof(BJob, iter) == 0 && offsetof(BJob, state) == 4 && offsetof(BJob, shift) == 8,
+                  "job status leads the record");
+    JobStatus *h_status = nullptr;  // pinned status of every slot
+    std::vector<JobStatus> done_status(slots);
+    char *res = nullptr, *res_dev = nullptr;  // pinned, device-mapped result staging
+    clk::GatherDesc *descs = nullptr, *descs_dev = nullptr;
+    size_t res_cap = 0, desc_cap = 0;
+    std::vector<uint32_t> done_slots;
     CLK_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    if (hipStreamCreateWithFlags(&rs, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        clk::set_error("clk_serve_run: stream creation failed");
+        return CLK_ERR_HIP;
+    }
     int rc = CLK_OK;
     std::vector<int64_t> slot_job(slots, -1);
     std::deque<Staged> staged;
@@ -1575,7 +1621,7 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
         return code;
     };
     if (hipEventCreateWithFlags(&step_done, hipEventDisableTiming) != hipSuccess ||
-        hipHostMalloc((void **)&h_state, slots * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&h_status, slots * sizeof(JobStatus), hipHostMallocDefault) != hipSuccess) {
         clk::set_error("clk_serve_run: event / pinned buffer allocation failed");
         fail(CLK_ERR_HIP);
     }
@@ -1592,7 +1638,7 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
             clk_serve_job &j = jobs[next];
             int32_t s = -1;
             const int r = stage_job(b, j.n, j.d, j.k, j.points, j.centroids, j.max_iterations, j.tolerance, &s, cs,
-                                    j.device_points != 0);
+                                    j.device_points != 0, false);
             if (r == CLK_ERR_NOMEM)
                 break;
             if (r != CLK_OK) {
@@ -1621,32 +1667,24 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
                 break;
             }
             inflight = false;
-            if (hipMemcpy2DAsync(h_state, sizeof(uint32_t), reinterpret_cast<char *>(b->d_jobs) + offsetof(BJob, state),
-                                 sizeof(BJob), sizeof(uint32_t), slots, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            if (hipMemcpy2DAsync(h_status, sizeof(JobStatus), b->d_jobs, sizeof(BJob), sizeof(JobStatus), slots,
+                                 hipMemcpyDeviceToHost, st) != hipSuccess ||
                 hipStreamSynchronize(st) != hipSuccess) {
-                clk::set_error("clk_serve_run: state read failed");
+                clk::set_error("clk_serve_run: job status read failed");
                 fail(CLK_ERR_HIP);
                 break;
             }
-            for (uint32_t s = 0; s < slots && rc == CLK_OK; ++s) {
-                if (b->jobs[s].state != CLK_JOB_ACTIVE || h_state[s] != CLK_JOB_DONE)
-                    continue;
-                b->jobs[s].state = CLK_JOB_DONE;  // leaves the tile list
-                b->dirty = true;
-                clk_serve_job &j = jobs[slot_job[s]];
-                const int r = clk_batch_take(b, (int32_t)s, j.out_centroids, j.out_labels, j.out_masses,
-                                             &j.iterations, &j.shift, st);
-                if (r != CLK_OK) {
-                    fail(r);
-                    break;
+            // finished jobs leave the tile list now; their results are copied
+            // while the next iteration runs (no kernel touches a DONE job)
+            done_slots.clear();
+            for (uint32_t s = 0; s < slots; ++s)
+                if (b->jobs[s].state == CLK_JOB_ACTIVE && h_status[s].state == CLK_JOB_DONE) {
+                    b->jobs[s].state = CLK_JOB_DONE;
+                    b->dirty = true;
+                    done_slots.push_back(s);
+                    done_status[s] = h_status[s];
+                    --active;
                 }
-                j.finished = seconds_since(t0);
-                slot_job[s] = -1;
-                ++finished;
-                --active;
-            }
-            if (rc != CLK_OK)
-                break;
         }
         // 3. activate staged jobs whose copies are done (all of them when the
         //    GPU would otherwise idle)
@@ -1663,7 +1701,7 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
                 break;
             }
             (void)hipEventDestroy(g.copied);
-            const int r = activate_job(b, g.slot, st);
+            const int r = activate_job(b, g.slot, st, false);  // stream-ordered before the next iteration
             if (r != CLK_OK) {
                 fail(r);
                 break;
@@ -1696,11 +1734,84 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
             }
             inflight = true;
             ++steps;
-        } else if (staged.empty() && next < count) {
+        } else if (staged.empty() && next < count && done_slots.empty()) {
             // idle until the next arrival
             const double wait = jobs[next].arrival - seconds_since(t0);
             if (wait > 0.0)
                 std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.001)));
+        }
+        // 5. results of the jobs that finished in the previous iteration:
+        //    copies into pinned staging on their own stream, one
+        //    synchronization, then to the requests' buffers; only then are
+        //    their slots and arena ranges free again
+        if (!done_slots.empty()) {
+            size_t need = 0;
+            for (uint32_t s : done_slots) {
+                const BJob &J = b->jobs[s];
+                need += (size_t)J.d * J.k * 4 + (size_t)J.k * 4 + (jobs[slot_job[s]].out_labels ? J.n * 4 : 0);
+            }
+            const size_t ndesc = 3 * done_slots.size();
+            if (need > res_cap || ndesc > desc_cap) {
+                if (res)
+                    (void)hipHostFree(res);
+                if (descs)
+                    (void)hipHostFree(descs);
+                res = nullptr;
+                descs = nullptr;
+                res_cap = std::max(need, 2 * res_cap);
+                desc_cap = std::max(ndesc, 2 * desc_cap);
+                if (hipHostMalloc((void **)&res, res_cap, hipHostMallocDefault) != hipSuccess ||
+                    hipHostMalloc((void **)&descs, desc_cap * sizeof(clk::GatherDesc), hipHostMallocDefault) !=
+                        hipSuccess ||
+                    hipHostGetDevicePointer((void **)&res_dev, res, 0) != hipSuccess ||
+                    hipHostGetDevicePointer((void **)&descs_dev, descs, 0) != hipSuccess) {
+                    clk::set_error("clk_serve_run: pinned result buffer allocation failed");
+                    fail(CLK_ERR_NOMEM);
+                    break;
+                }
+            }
+            size_t at = 0, nd = 0;
+            for (uint32_t s : done_slots) {
+                const BJob &J = b->jobs[s];
+                const size_t cb = (size_t)J.d * J.k * 4, mb = (size_t)J.k * 4;
+                const size_t lb = jobs[slot_job[s]].out_labels ? J.n * 4 : 0;
+                const float *cen = J.cen[done_status[s].iter & 1];
+                descs[nd++] = {reinterpret_cast<const uint32_t *>(cen), reinterpret_cast<uint32_t *>(res_dev + at),
+                               cb / 4};
+                descs[nd++] = {J.masses, reinterpret_cast<uint32_t *>(res_dev + at + cb), mb / 4};
+                if (lb)
+                    descs[nd++] = {J.labels, reinterpret_cast<uint32_t *>(res_dev + at + cb + mb), lb / 4};
+                at += cb + mb + lb;
+            }
+            hipLaunchKernelGGL(clk::gather_kernel, dim3((unsigned)nd), dim3(256), 0, rs, descs_dev);
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(rs) != hipSuccess) {
+                clk::set_error("clk_serve_run: result copy failed");
+                fail(CLK_ERR_HIP);
+                break;
+            }
+            at = 0;
+            const double now2 = seconds_since(t0);
+            for (uint32_t s : done_slots) {
+                clk_serve_job &j = jobs[slot_job[s]];
+                const BJob &J = b->jobs[s];
+                const size_t cb = (size_t)J.d * J.k * 4, mb = (size_t)J.k * 4, lb = j.out_labels ? J.n * 4 : 0;
+                if (j.out_centroids)
+                    std::memcpy(j.out_centroids, res + at, cb);
+                if (j.out_masses)
+                    std::memcpy(j.out_masses, res + at + cb, mb);
+                if (lb)
+                    std::memcpy(j.out_labels, res + at + cb + mb, lb);
+                at += cb + mb + lb;
+                j.iterations = done_status[s].iter;
+                j.shift = done_status[s].shift;
+                j.finished = now2;
+                // the device record stays DONE (no kernel touches it) until the
+                // slot is activated again
+                b->jobs[s].state = CLK_JOB_FREE;
+                slot_job[s] = -1;
+                ++finished;
+            }
+            done_slots.clear();
         }
     }
     if (inflight)
@@ -1716,10 +1827,16 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
                 (void)clk_batch_take(b, (int32_t)s, nullptr, nullptr, nullptr, nullptr, nullptr, st);
     (void)hipStreamSynchronize(cs);
     (void)hipStreamDestroy(cs);
+    (void)hipStreamSynchronize(rs);
+    (void)hipStreamDestroy(rs);
     if (step_done)
         (void)hipEventDestroy(step_done);
-    if (h_state)
-        (void)hipHostFree(h_state);
+    if (h_status)
+        (void)hipHostFree(h_status);
+    if (res)
+        (void)hipHostFree(res);
+    if (descs)
+        (void)hipHostFree(descs);
     CLK_TRY(rc);
     if (stats) {
         latency_stats(jobs, count, stats);
