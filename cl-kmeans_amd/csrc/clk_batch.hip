@@ -791,6 +791,24 @@ __global__ void __launch_bounds__(256) gather_kernel(const GatherDesc *__restric
         g.dst[i] = g.src[i];
 }
 
+// Activation of staged jobs (serving): one work-group per slot copies the
+// job's record from device-mapped pinned host memory into the job table and,
+// for the MFMA labeling, builds the first 16-bit table.
+template <bool FP16>
+__global__ void __launch_bounds__(256)
+activate_kernel(const uint32_t *__restrict__ slots, const BJob *__restrict__ recs, BJob *__restrict__ jobs)
+{
+    __shared__ double red[256];
+    const uint32_t s = slots[blockIdx.x];
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(recs + s);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(jobs + s);
+    for (uint32_t i = threadIdx.x; i < sizeof(BJob) / 4; i += blockDim.x)
+        dst[i] = src[i];
+    const BJob &J = recs[s];
+    if (J.kind == 1)
+        mfma::build_table_block<FP16>(J.cen[0], J.d, J.k, J.nt16, J.nkb, J.table, J.norm, J.stats, red);
+}
+
 // The table of a newly admitted job (from its initial centroids).
 template <bool FP16>
 __global__ void __launch_bounds__(256) batch_build_table_kernel(BJob *__restrict__ jobs, uint32_t slot)
@@ -853,6 +871,7 @@ struct clk_batch {
     bool dry = false;           // autotuning: iterations label and accumulate only
     BClassState cls[clk::kClasses];
     BJob *h_rec = nullptr;  // pinned staging of job records for activation
+    BJob *h_rec_dev = nullptr;  // its device address
     uint32_t class_waves[clk::kClasses] = {};  // fixed at creation (they fix the summation order)
     bool class_priv[clk::kClasses] = {};
 };
@@ -1310,6 +1329,17 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
     return CLK_OK;
 }
 
+// Pinned, device-mapped copies of the job records (the copies and the
+// activation kernel read them after the call that wrote them returns).
+int ensure_pinned_records(clk_batch *b)
+{
+    if (b->h_rec)
+        return CLK_OK;
+    CLK_HIP_CHECK(hipHostMalloc((void **)&b->h_rec, b->jobs.size() * sizeof(BJob), hipHostMallocDefault));
+    CLK_HIP_CHECK(hipHostGetDevicePointer((void **)&b->h_rec_dev, b->h_rec, 0));
+    return CLK_OK;
+}
+
 // Make a staged job visible: its record in the device job table and, for the
 // MFMA labeling, its first 16-bit table (stream-ordered after the copies when
 // st is the staging stream; otherwise the caller orders the two streams).
@@ -1318,9 +1348,7 @@ int activate_job(clk_batch *b, int32_t s, hipStream_t st, bool sync = true)
 {
     BJob &J = b->jobs[s];
     J.state = b->staged_state[s];
-    if (!b->h_rec) {  // pinned copies of the records: the copy runs after the call returns
-        CLK_HIP_CHECK(hipHostMalloc((void **)&b->h_rec, b->jobs.size() * sizeof(BJob), hipHostMallocDefault));
-    }
+    CLK_TRY(ensure_pinned_records(b));
     b->h_rec[s] = J;
     CLK_HIP_CHECK(hipMemcpyAsync(b->d_jobs + s, b->h_rec + s, sizeof(BJob), hipMemcpyHostToDevice, st));
     if (J.kind == 1) {
@@ -1606,7 +1634,10 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
     char *res = nullptr, *res_dev = nullptr;  // pinned, device-mapped result staging
     clk::GatherDesc *descs = nullptr, *descs_dev = nullptr;
     size_t res_cap = 0, desc_cap = 0;
-    std::vector<uint32_t> done_slots;
+    std::vector<uint32_t> done_slots, act_slots;
+    std::vector<Staged> zero_iter;
+    uint32_t *h_act = nullptr, *h_act_dev = nullptr;  // pinned slot list of the activation kernel
+    size_t act_cap = 0;
     CLK_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     if (hipStreamCreateWithFlags(&rs, hipStreamNonBlocking) != hipSuccess) {
         (void)hipStreamDestroy(cs);
@@ -1625,6 +1656,8 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
         clk::set_error("clk_serve_run: event / pinned buffer allocation failed");
         fail(CLK_ERR_HIP);
     }
+    if (rc == CLK_OK && ensure_pinned_records(b) != CLK_OK)
+        fail(CLK_ERR_HIP);
     const auto t0 = std::chrono::steady_clock::now();
     uint32_t next = 0, finished = 0, active = 0;
     bool inflight = false;
@@ -1634,6 +1667,7 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
         const double now = seconds_since(t0);
         // 1. stage the arrived jobs in arrival order while slots and arena allow
         //    (their copies overlap the iteration in flight)
+        const size_t round0 = staged.size();
         while (next < count && jobs[next].arrival <= now) {
             clk_serve_job &j = jobs[next];
             int32_t s = -1;
@@ -1645,6 +1679,14 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
                 fail(r);
                 break;
             }
+            j.admitted = seconds_since(t0);
+            slot_job[s] = next;
+            staged.push_back({s, next, nullptr});
+            ++next;
+        }
+        if (rc != CLK_OK)
+            break;
+        if (staged.size() > round0) {  // one event marks this round's copies
             hipEvent_t ev = nullptr;
             if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
                 hipEventRecord(ev, cs) != hipSuccess) {
@@ -1652,13 +1694,9 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
                 fail(CLK_ERR_HIP);
                 break;
             }
-            j.admitted = seconds_since(t0);
-            slot_job[s] = next;
-            staged.push_back({s, next, ev});
-            ++next;
+            for (size_t i = round0; i < staged.size(); ++i)
+                staged[i].copied = ev;
         }
-        if (rc != CLK_OK)
-            break;
         // 2. end of the iteration in flight: collect finished jobs
         if (inflight) {
             if (hipEventSynchronize(step_done) != hipSuccess) {
@@ -1688,10 +1726,12 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
         }
         // 3. activate staged jobs whose copies are done (all of them when the
         //    GPU would otherwise idle)
+        act_slots.clear();
+        zero_iter.clear();
         while (!staged.empty()) {
             Staged &g = staged.front();
             hipError_t q = hipEventQuery(g.copied);
-            if (q == hipErrorNotReady && active == 0)
+            if (q == hipErrorNotReady && active == 0 && act_slots.empty())
                 q = hipEventSynchronize(g.copied);
             if (q == hipErrorNotReady)
                 break;
@@ -1700,15 +1740,47 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
                 fail(CLK_ERR_HIP);
                 break;
             }
-            (void)hipEventDestroy(g.copied);
-            const int r = activate_job(b, g.slot, st, false);  // stream-ordered before the next iteration
-            if (r != CLK_OK) {
-                fail(r);
+            if (staged.size() == 1 || staged[1].copied != g.copied)
+                (void)hipEventDestroy(g.copied);  // the round's last job
+            BJob &J = b->jobs[g.slot];
+            J.state = b->staged_state[g.slot];
+            b->h_rec[g.slot] = J;
+            act_slots.push_back((uint32_t)g.slot);
+            if (J.state == CLK_JOB_ACTIVE)
+                ++active;
+            else  // zero iterations: done at once
+                zero_iter.push_back(g);
+            staged.pop_front();
+        }
+        if (rc != CLK_OK)
+            break;
+        if (!act_slots.empty()) {  // records and first tables, stream-ordered before the next iteration
+            if (act_slots.size() > act_cap) {
+                if (h_act)
+                    (void)hipHostFree(h_act);
+                h_act = nullptr;
+                act_cap = std::max(act_slots.size(), 2 * act_cap);
+                if (hipHostMalloc((void **)&h_act, act_cap * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+                    hipHostGetDevicePointer((void **)&h_act_dev, h_act, 0) != hipSuccess) {
+                    clk::set_error("clk_serve_run: pinned buffer allocation failed");
+                    fail(CLK_ERR_NOMEM);
+                    break;
+                }
+            }
+            std::memcpy(h_act, act_slots.data(), act_slots.size() * sizeof(uint32_t));
+            if (b->fp16)
+                hipLaunchKernelGGL(clk::activate_kernel<true>, dim3((unsigned)act_slots.size()), dim3(256), 0, st,
+                                   h_act_dev, b->h_rec_dev, b->d_jobs);
+            else
+                hipLaunchKernelGGL(clk::activate_kernel<false>, dim3((unsigned)act_slots.size()), dim3(256), 0, st,
+                                   h_act_dev, b->h_rec_dev, b->d_jobs);
+            if (hipGetLastError() != hipSuccess) {
+                clk::set_error("clk_serve_run: activation failed");
+                fail(CLK_ERR_HIP);
                 break;
             }
-            if (b->jobs[g.slot].state == CLK_JOB_ACTIVE) {
-                ++active;
-            } else {  // zero iterations: done at once
+            b->dirty = true;
+            for (const Staged &g : zero_iter) {
                 clk_serve_job &j = jobs[g.job];
                 const int r2 = clk_batch_take(b, g.slot, j.out_centroids, j.out_labels, j.out_masses,
                                               &j.iterations, &j.shift, st);
@@ -1720,10 +1792,9 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
                 slot_job[g.slot] = -1;
                 ++finished;
             }
-            staged.pop_front();
+            if (rc != CLK_OK)
+                break;
         }
-        if (rc != CLK_OK)
-            break;
         // 4. the next iteration of every resident job
         if (active > 0) {
             max_active = std::max(max_active, active);
@@ -1816,10 +1887,11 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
     }
     if (inflight)
         (void)hipEventSynchronize(step_done);
-    for (auto &g : staged) {
-        (void)hipEventSynchronize(g.copied);
-        (void)hipEventDestroy(g.copied);
-        b->jobs[g.slot].state = CLK_JOB_FREE;
+    for (size_t i = 0; i < staged.size(); ++i) {  // a round's jobs share one event
+        (void)hipEventSynchronize(staged[i].copied);
+        if (i + 1 == staged.size() || staged[i + 1].copied != staged[i].copied)
+            (void)hipEventDestroy(staged[i].copied);
+        b->jobs[staged[i].slot].state = CLK_JOB_FREE;
     }
     if (rc != CLK_OK)  // leave the batch empty
         for (uint32_t s = 0; s < slots; ++s)
@@ -1837,6 +1909,8 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
         (void)hipHostFree(res);
     if (descs)
         (void)hipHostFree(descs);
+    if (h_act)
+        (void)hipHostFree(h_act);
     CLK_TRY(rc);
     if (stats) {
         latency_stats(jobs, count, stats);
