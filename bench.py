@@ -55,7 +55,10 @@ def main():
     ap.add_argument("--d", type=int, default=4)
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--pipeline", default="single_stage", choices=["single_stage", "three_stage"])
-    ap.add_argument("--strategy", default="cluster_merge")
+    ap.add_argument("--strategy", default="cluster_merge_bounded",
+                    help="fused strategy; other than cluster_merge, the exhaustive cluster_merge pass is timed "
+                         "in the same run too (--no-compare skips it)")
+    ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="fused", choices=["fused", "dense", "batch", "serve"])
@@ -118,13 +121,12 @@ def main():
     sums = hip.DeviceArray((d, k), np.float64)
     masses = hip.DeviceArray((k,), np.uint32)
     labels = hip.DeviceArray((n,), np.uint32)
-    fcfg = ctx.config(clkmeans.STAGE_FUSED, args.strategy)
     lcfg = ctx.config(clkmeans.STAGE_LABELING, "unroll_vector")
     mcfg = ctx.config(clkmeans.STAGE_MASS_UPDATE, "part_private")
     ccfg = ctx.config(clkmeans.STAGE_CENTROID_UPDATE, "cluster_merge")
     s = stream.handle
 
-    def iteration():
+    def iteration(fcfg):
         old, new = cen[0], cen[1]
         if args.pipeline == "single_stage":
             # zero + fused pass + FP64 divide: one launch
@@ -138,36 +140,55 @@ def main():
             ctx.divide(d, k, sums, masses, new, stream=s)
         cen[0], cen[1] = new, old
 
-    for _ in range(args.warmup):
-        iteration()
-    stream.synchronize()
-    hip.synchronize()
+    def restart():
+        # first-x centroids, zeroed labels: the configuration's start
+        cen[0].copy_from(init)
+        ctx.fill_zero(labels)
+        hip.synchronize()
 
-    # Timed region: K iterations, every kernel bracketed by HIP events on the
-    # stream it runs on.
-    ctx.clear_records()
-    ctx.profiling(True)
-    e0, e1 = hip.Event(), hip.Event()
-    w0 = time.perf_counter()
-    e0.record(stream)
-    for it in range(args.steps):
-        ctx.set_iteration(it)
-        iteration()
-    e1.record(stream)
-    e1.synchronize()
-    hip.synchronize()
-    wall = time.perf_counter() - w0
-    ctx.profiling(False)
-    total_ms = e0.elapsed_ms(e1)
-    recs = ctx.records()
-    ms_per_step = total_ms / args.steps
-    by_name = {}
-    for name, _, ms in recs:
-        by_name.setdefault(name, []).append(ms)
-    hot = "FusedClusterMerge" if args.pipeline == "single_stage" else "LabelingUnrollVector"
-    if args.pipeline == "single_stage" and args.strategy == "feature_sum":
-        hot = "FusedFeatureSum"
-    hot_ms = float(np.mean(by_name[hot])) if hot in by_name else float("nan")
+    def hot_name(strategy):
+        if args.pipeline != "single_stage":
+            return "LabelingUnrollVector"
+        return {"feature_sum": "FusedFeatureSum",
+                "cluster_merge_bounded": "FusedClusterMergeBounded"}.get(strategy, "FusedClusterMerge")
+
+    def measure(strategy):
+        """W warm-up iterations, then K timed iterations from the start again."""
+        fcfg = ctx.config(clkmeans.STAGE_FUSED, strategy)
+        restart()
+        for _ in range(args.warmup):
+            iteration(fcfg)
+        stream.synchronize()
+        restart()
+        # Timed region: K iterations, every kernel bracketed by HIP events on
+        # the stream it runs on.
+        ctx.clear_records()
+        ctx.profiling(True)
+        e0, e1 = hip.Event(), hip.Event()
+        w0 = time.perf_counter()
+        e0.record(stream)
+        for it in range(args.steps):
+            ctx.set_iteration(it)
+            iteration(fcfg)
+        e1.record(stream)
+        e1.synchronize()
+        hip.synchronize()
+        wall = time.perf_counter() - w0
+        ctx.profiling(False)
+        by_name = {}
+        for name, _, ms in ctx.records():
+            by_name.setdefault(name, []).append(ms)
+        hot = hot_name(strategy)
+        return {"cfg": fcfg, "ms_per_step": e0.elapsed_ms(e1) / args.steps, "wall": wall, "by_name": by_name,
+                "hot": hot, "hot_ms": float(np.mean(by_name[hot])) if hot in by_name else float("nan"),
+                "hot_each": by_name.get(hot, [])}
+
+    compare = args.pipeline == "single_stage" and args.strategy != "cluster_merge" and not args.no_compare
+    exhaustive = measure("cluster_merge") if compare else None
+    run = measure(args.strategy)
+    ms_per_step, wall, by_name, hot, hot_ms = (run["ms_per_step"], run["wall"], run["by_name"], run["hot"],
+                                               run["hot_ms"])
+    fcfg = run["cfg"]
     bytes_per_launch = n * (4 * d + 4) if args.pipeline == "single_stage" else n * (4 * d + 4)
     achieved = bytes_per_launch / (hot_ms * 1e-3) / 1e9
     log("kernels (avg ms): " + ", ".join("%s=%.4f x%d" % (kk, np.mean(v), len(v)) for kk, v in by_name.items()))
@@ -175,7 +196,7 @@ def main():
     # Parity spot check of one more step: labels of a sample vs the oracle.
     from oracle import oracle as O
     c_in = cen[0].to_host()
-    iteration()
+    iteration(fcfg)
     stream.synchronize()
     lab = labels.to_host()
     rng = np.random.default_rng(0)
@@ -195,10 +216,16 @@ def main():
                          "restatement (oracle/kmeans_oracle.c ora_lloyd_mt_f32), %d threads" % (iters, n, threads)}
 
     traffic = None
-    pmc = os.path.join(REPO, "profiles", "fused_cluster_merge_n2p27_d4_k10_pmc.json")
+    pmc = os.path.join(REPO, "profiles", "fused_%s_n2p27_d4_k10_pmc.json" % args.strategy)
     if os.path.exists(pmc) and args.pipeline == "single_stage" and n == (1 << 27) and d == 4 and k == 10:
         with open(pmc) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    def summary(r, strategy):
+        ach = bytes_per_launch / (r["hot_ms"] * 1e-3) / 1e9
+        return {"strategy": strategy, "value": 1000.0 / r["ms_per_step"], "unit": "iter/s",
+                "ms_per_step": r["ms_per_step"], "kernel": r["hot"], "kernel_ms": r["hot_ms"],
+                "achieved_GBps": ach, "frac": ach / HBM_PEAK_GBS}
 
     out = {
         "metric": METRIC,
@@ -219,7 +246,10 @@ def main():
                    "unroll_vector/part_private/cluster_merge", "n": n, "d": d, "k": k},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": hot,
-                     "kernel_ms": hot_ms, "bytes_per_launch": bytes_per_launch},
+                     "kernel_ms": hot_ms, "bytes_per_launch": bytes_per_launch,
+                     "kernel_ms_iterations_1_to_3": [round(x, 4) for x in run["hot_each"][:3]],
+                     "kernel_ms_last": round(run["hot_each"][-1], 4) if run["hot_each"] else None},
+        "exhaustive": summary(exhaustive, "cluster_merge") if exhaustive else None,
         "cpu_baseline": cpu,
         "wall_s": wall,
         "parity": {"labels_sample": int(idx.size), "labels_equal": parity, "mass_sum_equal_n": mass_ok},

@@ -269,6 +269,54 @@ __device__ __forceinline__ void load_labels(const uint32_t *__restrict__ labels,
     }
 }
 
+// load_labels with non-temporal loads (labels read once per pass).
+template <int VEC>
+__device__ __forceinline__ void stream_load_labels(const uint32_t *__restrict__ labels,
+                                                   uint64_t p, uint32_t (&lab)[VEC])
+{
+    static_assert(VEC == 1 || VEC == 2 || VEC == 4, "1, 2 or 4 labels");
+    const uint32_t *src = labels + p;
+    if constexpr (VEC == 1) {
+        lab[0] = stream_load(src);
+    } else if constexpr (VEC == 2) {
+        u32x2 a = stream_load(reinterpret_cast<const u32x2 *>(src));
+        lab[0] = a.x; lab[1] = a.y;
+    } else {
+        u32x4 a = stream_load(reinterpret_cast<const u32x4 *>(src));
+        lab[0] = a.x; lab[1] = a.y; lab[2] = a.z; lab[3] = a.w;
+    }
+}
+
+// Keep-label threshold of centroid a for the bounded fused pass.
+//
+// s2 = min over c != a of ||c_a - c||^2 (FP64 from the FP32 centroids;
+// NaN distances skipped).  With m = 2^-10 and thr = s2 (1-m) / (2+m)^2
+// (rounded down to float, capped at 1e30), a point x whose FP32 distance
+// D(x, c_a) -- the reference's fma sequence -- is <= thr gets label a from
+// the full scan too:
+//   the FP32 sum of d squares of rounded differences is within a relative
+//   (d+3)*2^-24 of the exact r^2 = ||x - c_a||^2 (plus at most d*2^-126
+//   absolute from underflow), so r < sqrt(s2) / (2+m); the triangle
+//   inequality gives ||x - c|| >= sqrt(s2) - r > sqrt(s2) (1+m) / (2+m) for
+//   every other centroid, hence D(x, c) > thr (1-(d+3)2^-24)(1+m)^2/(1-m)
+//   - d*2^-126 > thr >= D(x, c_a): c_a is the unique strict minimum and
+//   below FLT_MAX, which is what the strict-'<' scan from FLT_MAX returns
+//   (kmeans_naive.cpp:49-63).  The margin m dwarfs the rounding terms for
+//   d <= 2^12 (the FP64 arithmetic here adds a few 2^-53); thresholds below
+//   1e-20 (duplicate or nearly coincident centroids) return -1: never keep.
+__device__ __forceinline__ float keep_threshold(double s2)
+{
+    constexpr double m = 1.0 / 1024.0;
+    if (!(s2 > 0.0))
+        return -1.0f;
+    double t = s2 * (1.0 - m) / ((2.0 + m) * (2.0 + m));
+    if (t > 1e30)
+        t = 1e30;  // also s2 = inf: k == 1, or every other centroid is non-finite
+    if (t < 1e-20)
+        return -1.0f;
+    return __double2float_rd(t);
+}
+
 // Stage centroids [d][k] (global) into LDS as [c][DMAX], zero-padded.
 template <int DMAX>
 __device__ __forceinline__ void stage_centroids(const float *__restrict__ cen,
@@ -278,6 +326,30 @@ __device__ __forceinline__ void stage_centroids(const float *__restrict__ cen,
     for (uint32_t i = threadIdx.x; i < k * DMAX; i += blockDim.x) {
         uint32_t c = i / DMAX, f = i % DMAX;
         lds[i] = (f < d) ? cen[(uint64_t)f * k + c] : 0.0f;
+    }
+}
+
+// keep_threshold of every centroid staged in LDS as [c][DMAX] (padding
+// features are 0 in every row, so they add nothing).  Call after the
+// centroids are visible to the work-group; thr[] needs a barrier after.
+template <int DMAX>
+__device__ __forceinline__ void stage_keep_thresholds(const float *__restrict__ cen_lds,
+                                                      uint32_t k, float *__restrict__ thr)
+{
+    for (uint32_t a = threadIdx.x; a < k; a += blockDim.x) {
+        double s2 = __builtin_inf();
+        for (uint32_t c = 0; c < k; ++c) {
+            if (c == a)
+                continue;
+            double acc = 0.0;
+#pragma unroll
+            for (int f = 0; f < DMAX; ++f) {
+                const double t = (double)cen_lds[a * DMAX + f] - (double)cen_lds[c * DMAX + f];
+                acc = __builtin_fma(t, t, acc);
+            }
+            s2 = acc < s2 ? acc : s2;  // a NaN distance never lowers it
+        }
+        thr[a] = keep_threshold(s2);
     }
 }
 

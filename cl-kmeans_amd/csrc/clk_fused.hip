@@ -25,7 +25,7 @@
 namespace clk {
 namespace {
 
-template <int DMAX, int VEC, int BLOCK, bool PRIVATE, bool EXACT>
+template <int DMAX, int VEC, int BLOCK, bool PRIVATE, bool EXACT, bool BOUNDED>
 __global__ void __launch_bounds__(BLOCK > 0 ? BLOCK : (DMAX >= 64 ? 512 : 1024))
 fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
                   const float *__restrict__ cen, uint32_t *__restrict__ labels,
@@ -37,12 +37,17 @@ fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
     const uint32_t cols = PRIVATE ? B : 1;
     const uint32_t rows = k * (d + 1);
     float *cen_lds = smem;
-    float *tile = smem + ((k * DMAX + 3u) & ~3u);
+    float *thr_lds = smem + ((k * DMAX + 3u) & ~3u);  // BOUNDED: keep thresholds [k]
+    float *tile = thr_lds + (BOUNDED ? ((k + 3u) & ~3u) : 0u);
 
     stage_centroids<DMAX>(cen, d, k, cen_lds);
     for (uint32_t i = tid; i < rows * cols; i += B)
         tile[i] = 0.0f;
     __syncthreads();
+    if constexpr (BOUNDED) {
+        stage_keep_thresholds<DMAX>(cen_lds, k, thr_lds);
+        __syncthreads();
+    }
 
     const uint64_t groups = n / VEC;
     const uint64_t stride = (uint64_t)gridDim.x * B;
@@ -50,11 +55,41 @@ fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
     const uint32_t row_stride = (d + 1) * cols;  // < 2^24: the tile fits in LDS
     // Software pipeline, two register buffers: one step's points are in
     // flight while the other step's are labelled and accumulated.
-    auto step = [&](const float (&x)[DMAX][VEC], uint64_t g) {
+    auto step = [&](const float (&x)[DMAX][VEC], const uint32_t (&prev)[VEC], uint64_t g) {
         const uint64_t p = g * VEC;
         uint32_t lab[VEC];
-        argmin_lds<DMAX, VEC>(x, cen_lds, k, lab);
-        store_labels<VEC>(labels, p, lab);
+        if constexpr (BOUNDED) {
+            // Keep the previous label where the point is inside its centroid's
+            // keep radius (keep_threshold: the full scan provably returns it);
+            // otherwise scan every centroid.  Only changed labels are stored.
+            bool keep = true;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) {
+                const bool valid = prev[v] < k;
+                const uint32_t a = valid ? prev[v] : 0u;
+                const float *ca = cen_lds + a * DMAX;
+                float t = x[0][v] - ca[0];
+                float dist = t * t;
+#pragma unroll
+                for (int f = 1; f < DMAX; ++f) {
+                    t = x[f][v] - ca[f];
+                    dist = __builtin_fmaf(t, t, dist);
+                }
+                keep = keep && valid && dist <= thr_lds[a];
+                lab[v] = a;
+            }
+            if (!keep)
+                argmin_lds<DMAX, VEC>(x, cen_lds, k, lab);
+            bool changed = false;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v)
+                changed = changed || lab[v] != prev[v];
+            if (changed)
+                store_labels<VEC>(labels, p, lab);
+        } else {
+            argmin_lds<DMAX, VEC>(x, cen_lds, k, lab);
+            store_labels<VEC>(labels, p, lab);
+        }
 #pragma unroll
         for (int v = 0; v < VEC; ++v) {
             float *t = mine + __umul24(lab[v], row_stride);
@@ -78,20 +113,27 @@ fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
     // Loads are unconditional (index clamped to the last group, always in
     // bounds) so the loop body is straight-line and the wait counters can
     // keep the prefetch in flight across the labeling of the other buffer.
+    // BOUNDED also prefetches the previous labels of the same points.
     const uint64_t last = groups ? groups - 1 : 0;
     uint64_t g = (uint64_t)blockIdx.x * B + tid;
     float xa[DMAX][VEC], xb[DMAX][VEC];
+    uint32_t la[VEC] = {}, lb[VEC] = {};
+    auto load = [&](uint64_t gg, float (&x)[DMAX][VEC], uint32_t (&l)[VEC]) {
+        load_points<DMAX, VEC, EXACT>(pts, n, d, gg * VEC, x);
+        if constexpr (BOUNDED)
+            stream_load_labels<VEC>(labels, gg * VEC, l);
+    };
     if (groups)
-        load_points<DMAX, VEC, EXACT>(pts, n, d, min(g, last) * VEC, xa);
+        load(min(g, last), xa, la);
     while (g < groups) {
         const uint64_t g1 = g + stride;
-        load_points<DMAX, VEC, EXACT>(pts, n, d, min(g1, last) * VEC, xb);
-        step(xa, g);
+        load(min(g1, last), xb, lb);
+        step(xa, la, g);
         if (g1 >= groups)
             break;
         const uint64_t g2 = g1 + stride;
-        load_points<DMAX, VEC, EXACT>(pts, n, d, min(g2, last) * VEC, xa);
-        step(xb, g1);
+        load(min(g2, last), xa, la);
+        step(xb, lb, g1);
         g = g2;
     }
     __syncthreads();
@@ -117,45 +159,47 @@ using FusedKernel = void (*)(const float *, uint64_t, uint32_t, uint32_t, const 
 // Points per work-item at most (VEC x DMAX values in registers, no spills).
 constexpr int fused_vec_cap(int dmax) { return dmax >= 32 ? 1 : dmax >= 16 ? 2 : 4; }
 
-template <int DMAX, int BLOCK, bool PRIVATE, bool EXACT>
+template <int DMAX, int BLOCK, bool PRIVATE, bool EXACT, bool BOUNDED>
 FusedKernel pick_vec_block(int vec)
 {
     constexpr int cap = fused_vec_cap(DMAX);
     if constexpr (cap >= 4)
         if (vec >= 4)
-            return fused_tile_kernel<DMAX, 4, BLOCK, PRIVATE, EXACT>;
+            return fused_tile_kernel<DMAX, 4, BLOCK, PRIVATE, EXACT, BOUNDED>;
     if constexpr (cap >= 2)
         if (vec >= 2)
-            return fused_tile_kernel<DMAX, 2, BLOCK, PRIVATE, EXACT>;
-    return fused_tile_kernel<DMAX, 1, BLOCK, PRIVATE, EXACT>;
+            return fused_tile_kernel<DMAX, 2, BLOCK, PRIVATE, EXACT, BOUNDED>;
+    return fused_tile_kernel<DMAX, 1, BLOCK, PRIVATE, EXACT, BOUNDED>;
 }
 
-template <int DMAX, bool PRIVATE, bool EXACT>
+template <int DMAX, bool PRIVATE, bool EXACT, bool BOUNDED>
 FusedKernel pick_vec(int vec, bool block256)
 {
     if (PRIVATE && block256)
-        return pick_vec_block<DMAX, 256, PRIVATE, EXACT>(vec);
-    return pick_vec_block<DMAX, 0, PRIVATE, EXACT>(vec);
+        return pick_vec_block<DMAX, 256, PRIVATE, EXACT, BOUNDED>(vec);
+    return pick_vec_block<DMAX, 0, PRIVATE, EXACT, BOUNDED>(vec);
 }
 
-template <int DMAX, bool PRIVATE>
+template <int DMAX, bool PRIVATE, bool BOUNDED>
 FusedKernel pick_exact(bool exact, int vec, bool block256)
 {
-    return exact ? pick_vec<DMAX, PRIVATE, true>(vec, block256)
-                 : pick_vec<DMAX, PRIVATE, false>(vec, block256);
+    return exact ? pick_vec<DMAX, PRIVATE, true, BOUNDED>(vec, block256)
+                 : pick_vec<DMAX, PRIVATE, false, BOUNDED>(vec, block256);
 }
 
-template <bool PRIVATE>
+// BOUNDED (keep-radius shortcut) exists for the private-tile kernel only.
+template <bool PRIVATE, bool BOUNDED>
 FusedKernel pick_fused(int dmax, bool exact, int vec, bool block256)
 {
+    static_assert(PRIVATE || !BOUNDED, "bounded pass: private tile only");
     switch (dmax) {
-    case 1: return pick_vec<1, PRIVATE, true>(vec, block256);
-    case 2: return pick_exact<2, PRIVATE>(exact, vec, block256);
-    case 4: return pick_exact<4, PRIVATE>(exact, vec, block256);
-    case 8: return pick_exact<8, PRIVATE>(exact, vec, block256);
-    case 16: return pick_exact<16, PRIVATE>(exact, vec, block256);
-    case 32: return pick_exact<32, PRIVATE>(exact, vec, block256);
-    case 64: return pick_exact<64, PRIVATE>(exact, vec, block256);
+    case 1: return pick_vec<1, PRIVATE, true, BOUNDED>(vec, block256);
+    case 2: return pick_exact<2, PRIVATE, BOUNDED>(exact, vec, block256);
+    case 4: return pick_exact<4, PRIVATE, BOUNDED>(exact, vec, block256);
+    case 8: return pick_exact<8, PRIVATE, BOUNDED>(exact, vec, block256);
+    case 16: return pick_exact<16, PRIVATE, BOUNDED>(exact, vec, block256);
+    case 32: return pick_exact<32, PRIVATE, BOUNDED>(exact, vec, block256);
+    case 64: return pick_exact<64, PRIVATE, BOUNDED>(exact, vec, block256);
     default: return nullptr;
     }
 }
@@ -193,8 +237,10 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
     CLK_REQUIRE(mode == 1 || new_centroids != old_centroids, CLK_ERR_INVALID_ARGUMENT,
                 "lloyd_step: new_centroids must not alias old_centroids");
     const int strategy = cfg ? cfg->strategy : CLK_FUSED_CLUSTER_MERGE;
-    CLK_REQUIRE(strategy == CLK_FUSED_CLUSTER_MERGE || strategy == CLK_FUSED_FEATURE_SUM,
+    CLK_REQUIRE(strategy == CLK_FUSED_CLUSTER_MERGE || strategy == CLK_FUSED_FEATURE_SUM ||
+                    strategy == CLK_FUSED_CLUSTER_MERGE_BOUNDED,
                 CLK_ERR_INVALID_ARGUMENT, "fused: unknown strategy id");
+    const bool bounded = strategy == CLK_FUSED_CLUSTER_MERGE_BOUNDED;
     const int B = block_size(ctx, cfg);
     CLK_REQUIRE(B > 0, CLK_ERR_INVALID_ARGUMENT, "fused: local_size exceeds the device limit");
     const int dmax = dmax_bucket(d);
@@ -208,9 +254,14 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
     // separate labeling, mass and centroid-update stages for k*d beyond LDS.
     const uint64_t cen_bytes = dmax ? ((k * dmax + 3) & ~3ull) * 4 : 0;
     const uint64_t rows = k * (d + 1);
-    const uint64_t priv_bytes = cen_bytes + rows * (uint64_t)B * 4;
+    const uint64_t priv_plain = cen_bytes + rows * (uint64_t)B * 4;
     const uint64_t shared_bytes = cen_bytes + rows * 4;
-    bool use_private = dmax && strategy == CLK_FUSED_CLUSTER_MERGE && priv_bytes <= ctx->lds_per_block;
+    bool use_private = dmax && strategy != CLK_FUSED_FEATURE_SUM && priv_plain <= ctx->lds_per_block;
+    // The bounded pass needs k keep thresholds beside the private tile; where
+    // they do not fit it runs as "cluster_merge" (the same results).
+    const uint64_t thr_bytes = ((k + 3) & ~3ull) * 4;
+    const bool run_bounded = bounded && use_private && priv_plain + thr_bytes <= ctx->lds_per_block;
+    const uint64_t priv_bytes = priv_plain + (run_bounded ? thr_bytes : 0);
     bool use_shared = dmax && !use_private && shared_bytes <= ctx->lds_per_block;
 
     if ((!use_private && !use_shared) || n == 0) {
@@ -233,12 +284,18 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
 
     const bool b256 = (B == 256);
     const bool exact = (uint64_t)dmax == d;
-    FusedKernel kern = use_private ? pick_fused<true>(dmax, exact, vec, b256)
-                                   : pick_fused<false>(dmax, exact, vec, b256);
+    FusedKernel kern = !use_private ? pick_fused<false, false>(dmax, exact, vec, b256)
+                       : run_bounded ? pick_fused<true, true>(dmax, exact, vec, b256)
+                                     : pick_fused<true, false>(dmax, exact, vec, b256);
     const size_t lds = use_private ? priv_bytes : shared_bytes;
     const uint64_t groups = n / vec;
+    // The bounded pass takes the grid of the exhaustive private-tile kernel:
+    // the same points per work-item in the same order, so bit-identical sums.
+    const void *grid_kern = run_bounded ? (const void *)pick_fused<true, false>(dmax, exact, vec, b256)
+                                        : (const void *)kern;
+    const size_t grid_lds = run_bounded ? lds - thr_bytes : lds;
     uint64_t grid = (cfg && cfg->global_size[0]) ? div_up(cfg->global_size[0], (uint64_t)B)
-                                                 : auto_grid(ctx, (const void *)kern, B, lds, groups);
+                                                 : auto_grid(ctx, grid_kern, B, grid_lds, groups);
     grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, 1u << 20));
 
     CLK_TRY(ensure_scratch(ctx->part_sums, grid * d * k * sizeof(double)));
@@ -256,7 +313,11 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
     fa.k = (uint32_t)k;
     fa.mode = mode;
     {
-        KernelTimer t(ctx, use_private ? "FusedClusterMerge" : "FusedFeatureSum", stream);
+        KernelTimer t(ctx,
+                      !use_private ? "FusedFeatureSum"
+                      : run_bounded ? "FusedClusterMergeBounded"
+                                   : "FusedClusterMerge",
+                      stream);
         hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(B), lds, stream, points, n, (uint32_t)d,
                            (uint32_t)k, old_centroids, labels, static_cast<double *>(ctx->part_sums.ptr),
                            static_cast<uint32_t *>(ctx->part_masses.ptr));

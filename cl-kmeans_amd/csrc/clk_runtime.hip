@@ -479,6 +479,7 @@ extern "C" int clk_strategy_id(int stage, const char *name)
     case CLK_STAGE_FUSED:
         if (s == "cluster_merge") return CLK_FUSED_CLUSTER_MERGE;
         if (s == "feature_sum") return CLK_FUSED_FEATURE_SUM;
+        if (s == "cluster_merge_bounded") return CLK_FUSED_CLUSTER_MERGE_BOUNDED;
         break;
     case CLK_STAGE_LABELING:
         if (s == "unroll_vector") return CLK_LABELING_UNROLL_VECTOR;

@@ -32,6 +32,14 @@ constexpr void require_supported_types()
                   "the gfx950 kernels implement the float/uint32/uint32 type triple");
 }
 
+// DataPoint name of a fused strategy id (the reference's kernel class names).
+inline std::string fused_kernel_name(int id)
+{
+    return id == CLK_FUSED_FEATURE_SUM           ? "FusedFeatureSum"
+           : id == CLK_FUSED_CLUSTER_MERGE_BOUNDED ? "FusedClusterMergeBounded"
+                                                   : "FusedClusterMerge";
+}
+
 inline void fill_sizes(clk_kernel_config &k, const size_t g[3], const size_t l[3])
 {
     for (int i = 0; i < 3; ++i) {
@@ -90,7 +98,7 @@ public:
         check_status(clk_default_config(context.get(), CLK_STAGE_FUSED, id, &k), "clk_default_config");
         fill_sizes(k, config.global_size, config.local_size);
         k.vector_length = config.vector_length;
-        const std::string name = id == CLK_FUSED_CLUSTER_MERGE ? "FusedClusterMerge" : "FusedFeatureSum";
+        const std::string name = fused_kernel_name(id);
         return [context, k, name](const Queue &q, size_t d, size_t n, size_t kk, const PointT *pts,
                                   const PointT *oc, PointT *nc, AccT *sums, LabelT *labels, MassT *masses,
                                   double *shift, Measurement::DataPoint &dp, const WaitList &events) {
@@ -117,7 +125,7 @@ public:
         check_status(clk_default_config(context.get(), CLK_STAGE_FUSED, id, &k), "clk_default_config");
         fill_sizes(k, config.global_size, config.local_size);
         k.vector_length = config.vector_length;
-        const std::string name = id == CLK_FUSED_CLUSTER_MERGE ? "FusedClusterMerge" : "FusedFeatureSum";
+        const std::string name = fused_kernel_name(id);
         return [context, k, name](const Queue &q, size_t d, size_t n, size_t kk,
                                   BufferIterator<PointT> pb, BufferIterator<PointT> pe,
                                   BufferIterator<PointT> ob, BufferIterator<PointT> oe,

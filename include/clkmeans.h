@@ -57,6 +57,15 @@ typedef struct clk_context clk_context;
 /* Strategy ids (the factories' strategy strings). */
 #define CLK_FUSED_CLUSTER_MERGE 0   /* "cluster_merge"  fused_factory.hpp:73-77 */
 #define CLK_FUSED_FEATURE_SUM 1     /* "feature_sum"    fused_factory.hpp:78-82 */
+/* Extension: "cluster_merge" that reads the labels buffer first.  A point
+ * keeps its previous label, with one distance computed, when that distance
+ * is inside the label's keep radius -- (1-2^-10)/(2+2^-10)^2 of the squared
+ * distance to the nearest other centroid, so the triangle inequality plus
+ * the FP32 rounding bound prove the full scan returns the same label (the
+ * reference's did-changes test, kmeans_naive.cpp:65-68, stores only changed
+ * labels the same way).  Other points scan every centroid.  Labels, masses
+ * and sums are bit-identical to "cluster_merge" for any labels content. */
+#define CLK_FUSED_CLUSTER_MERGE_BOUNDED 2 /* "cluster_merge_bounded" */
 #define CLK_LABELING_UNROLL_VECTOR 0 /* "unroll_vector" labeling_factory.hpp:63-80 */
 /* Extensions (no reference counterpart; the factory would throw for them):
  * x·c^T on MFMA with 16-bit operands and FP32 accumulation, then the exact

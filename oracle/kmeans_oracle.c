@@ -12,8 +12,8 @@
  * cen[f * k + c]  (lloyd_fused_feature_sum.cl:230-232).
  *
  * Pinning status of each function (what checks it besides itself):
- *   ora_label_*          exact-rational brute force on tiny inputs with
- *                        adversarial near-ties (tests/test_oracle.py), hand
+ *   ora_label_*,         exact-rational brute force on tiny inputs with
+ *   ora_dist_f32         adversarial near-ties (tests/test_oracle.py), hand
  *                        ties / NaN / inf cases, sklearn on separated blobs.
  *   ora_mass             integer invariants (sum == n, bincount).
  *   ora_sum_f64          math.fsum per cell, k == 1 closed form.
@@ -76,6 +76,22 @@ void ora_label_f32(const float *pts, uint64_t n, uint64_t d, const float *cen,
 {
     for (uint64_t p = 0; p < n; ++p)
         labels[p] = label_point_f32(pts, n, d, cen, k, p);
+}
+
+/* The distance that label_point_f32 compares, of every point to centroid c
+ * (kmeans_naive.cpp:52-57 with the fma contraction): the checker of the
+ * keep-radius shortcut needs the FP32 value itself. */
+void ora_dist_f32(const float *pts, uint64_t n, uint64_t d, const float *cen,
+                  uint64_t k, uint64_t c, float *out)
+{
+    for (uint64_t p = 0; p < n; ++p) {
+        float dist = 0.0f;
+        for (uint64_t f = 0; f < d; ++f) {
+            float t = pts[f * n + p] - cen[f * k + c];
+            dist = fmaf(t, t, dist);
+        }
+        out[p] = dist;
+    }
 }
 
 /* Same as ora_label_f32 for the double / uint64 type triple

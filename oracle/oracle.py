@@ -29,6 +29,7 @@ def lib():
         sig = {
             "ora_label_f32": [vp, u64, u64, vp, u64, vp],
             "ora_label_f64": [vp, u64, u64, vp, u64, vp],
+            "ora_dist_f32": [vp, u64, u64, vp, u64, u64, vp],
             "ora_mass": [vp, u64, u64, vp],
             "ora_sum_f64": [vp, u64, u64, vp, u64, vp],
             "ora_sum_kahan_f32": [vp, u64, u64, vp, u64, vp],
@@ -64,6 +65,16 @@ def label(points, centroids):
     k = c.shape[1]
     out = np.empty(n, np.uint32)
     lib().ora_label_f32(pp, n, d, cp, k, out.ctypes.data)
+    return out
+
+
+def distance(points, centroids, c):
+    """FP32 distance of every point to centroid c, as label() computes it."""
+    p, pp = _c(points, np.float32)
+    cc, cp = _c(centroids, np.float32)
+    d, n = p.shape
+    out = np.empty(n, np.float32)
+    lib().ora_dist_f32(pp, n, d, cp, cc.shape[1], c, out.ctypes.data)
     return out
 
 

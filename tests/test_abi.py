@@ -40,6 +40,7 @@ def test_strategy_ids_follow_the_factories():
     S = clkmeans
     assert S.strategy_id(S.STAGE_FUSED, "cluster_merge") == 0
     assert S.strategy_id(S.STAGE_FUSED, "feature_sum") == 1
+    assert S.strategy_id(S.STAGE_FUSED, "cluster_merge_bounded") == 2  # extension
     assert S.strategy_id(S.STAGE_LABELING, "unroll_vector") == 0
     assert S.strategy_id(S.STAGE_LABELING, "mfma_bf16") == 1  # extensions
     assert S.strategy_id(S.STAGE_LABELING, "mfma_fp16") == 2

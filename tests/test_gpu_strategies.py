@@ -35,6 +35,8 @@ def data():
 CASES = [
     ("single_stage", {"fused": {"strategy": "cluster_merge"}}),
     ("single_stage", {"fused": {"strategy": "feature_sum"}}),
+    ("single_stage", {"fused": {"strategy": "cluster_merge_bounded"}}),
+    ("single_stage_buffered", {"fused": {"strategy": "cluster_merge_bounded"}}),
     ("three_stage", {"labeling": {"strategy": "unroll_vector"}, "mass_update": {"strategy": "part_private"},
                      "centroid_update": {"strategy": "cluster_merge"}}),
     ("three_stage", {"mass_update": {"strategy": "global_atomic"},

@@ -95,6 +95,22 @@ def test_label_matches_exact_rational_on_near_ties():
     assert total_flips > 0, "near-tie cases do not exercise the rounding sequence"
 
 
+def test_distance_matches_exact_rational():
+    # ora_dist_f32 is the value label() compares: an exact-rational fma chain
+    rng = np.random.default_rng(4)
+    for _ in range(10):
+        d, n, k = int(rng.integers(1, 6)), 8, int(rng.integers(1, 5))
+        pts, cen = near_tie_case(rng, d, n, max(k, 2))
+        for c in range(cen.shape[1]):
+            got = O.distance(pts, cen, c)
+            for p in range(n):
+                dist = Fraction(0)
+                for f in range(d):
+                    t = round_f32(Fraction(float(pts[f, p])) - Fraction(float(cen[f, c])))
+                    dist = round_f32(t * t + dist)
+                assert Fraction(float(got[p])) == dist
+
+
 def test_label_random_small_exact():
     rng = np.random.default_rng(1)
     for _ in range(20):
