@@ -76,6 +76,12 @@ struct BJob {
     float *stats;
     mfma::MfmaParams prm;
     uint32_t nt16, nkb;
+    // FP32-scan jobs: keep thresholds [k] of the current centroids
+    // (keep_threshold); valid when keep_ok (set by the finish kernel, clear
+    // at admission).  (The MFMA tiles do not use them: a whole wave of 64
+    // points seldom passes there, and the test measured slower.)
+    float *keep;
+    uint32_t keep_ok, pad1;
 };
 
 struct BTile {
@@ -130,10 +136,66 @@ inline BLayout mfma_layout_for(uint32_t waves, uint32_t kmax, uint32_t dmax_feat
     return L;
 }
 
-// LDS of the plain kernel: centroids [k][DMAX] | W tables [k][ts(dmax)]
+// LDS of the plain kernel: centroids [k][DMAX] | W tables [k][ts(dmax)] |
+// W rescan lists [64]
 __host__ __device__ inline size_t batch_lds_bytes(uint32_t waves, uint32_t k, uint32_t dm, uint32_t dmax_feat)
 {
-    return (size_t)k * dm * 4 + (size_t)waves * k * table_stride(dmax_feat) * 4;
+    return (size_t)k * dm * 4 + (size_t)waves * k * table_stride(dmax_feat) * 4 + (size_t)waves * 64 * 4;
+}
+
+// Labels of the lanes in `fail` (at most 16 of the wave's lanes: points that
+// failed the keep test), with the wave's lanes split over them: G = 64 /
+// next_pow2(m) lanes per point, lane r of a group scanning centroids r, r+G,
+// r+2G, ... with the reference arithmetic (strict '<' from FLT_MAX), then the
+// group's minima combined as (distance, index) pairs, lowest index on equal
+// distances -- the label the sequential scan returns (0 when no distance is
+// below FLT_MAX).  Returns the label of the calling lane's own point (lanes
+// outside `fail` get a value to ignore).  slot: 64 words of LDS of the wave.
+template <int DMAX>
+__device__ __forceinline__ uint32_t sparse_scan(const float (&x)[DMAX][1], const float *__restrict__ cen_s,
+                                                uint32_t d, uint32_t k, unsigned long long fail, uint32_t lane,
+                                                uint32_t *slot)
+{
+    const uint32_t m = (uint32_t)__builtin_popcountll(fail);
+    const uint32_t G = m <= 1 ? 64u : m <= 2 ? 32u : m <= 4 ? 16u : m <= 8 ? 8u : 4u;
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(fail >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)fail, 0u));
+    if ((fail >> lane) & 1ull)
+        slot[rank] = lane;
+    mfma::wave_lds_order();
+    const uint32_t j = lane / G, r = lane % G;
+    const uint32_t src = j < m ? slot[j] : 0u;
+    float xs[DMAX];
+#pragma unroll
+    for (int f = 0; f < DMAX; ++f)
+        xs[f] = f < (int)d ? __shfl(x[f][0], (int)src) : 0.0f;
+    float best = FLT_MAX;
+    uint32_t bi = 0xFFFFFFFFu;
+    for (uint32_t c = r; c < k; c += G) {
+        const float *cc = cen_s + (size_t)c * DMAX;
+        float t = xs[0] - cc[0];
+        float dist = t * t;
+#pragma unroll
+        for (int f = 1; f < DMAX; ++f) {
+            t = xs[f] - cc[f];
+            dist = __builtin_fmaf(t, t, dist);
+        }
+        if (dist < best) {
+            best = dist;
+            bi = c;
+        }
+    }
+    for (uint32_t off = 1; off < G; off <<= 1) {
+        const float ob = __shfl_xor(best, (int)off);
+        const uint32_t oi = (uint32_t)__shfl_xor((int)bi, (int)off);
+        if (ob < best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
+        }
+    }
+    const uint32_t lab = bi == 0xFFFFFFFFu ? 0u : bi;
+    mfma::wave_lds_order();  // slot is written again by the next call
+    return (uint32_t)__shfl((int)lab, (int)(rank * G));
 }
 
 // ---- runs of equal labels in a wave -------------------------------------
@@ -306,6 +368,7 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
     float *cen_s = lds;
     float *tabs = cen_s + (size_t)k * DMAX;
     float *T = tabs + (size_t)wave * k * ts;
+    uint32_t *slot = reinterpret_cast<uint32_t *>(tabs + (size_t)waves * k * ts) + wave * 64;
 
     stage_centroids<DMAX>(J.cen[J.iter & 1], d, k, cen_s);
     for (uint32_t i = tid; i < waves * k * ts; i += blockDim.x)
@@ -315,8 +378,11 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
     const uint64_t first = (uint64_t)tindex * J.tile_pts;
     const uint64_t end = std::min<uint64_t>(first + J.tile_pts, J.n);
     const uint64_t n = J.n;
-    // the next pass's points are loaded while this pass is labeled
+    // the next pass's points (and previous labels) are loaded while this
+    // pass is labeled
+    const bool keep_ok = J.keep_ok != 0;
     float xn[DMAX];
+    uint32_t ln = 0;
     auto fetch = [&](uint64_t b0) {
         const uint64_t p = b0 + tid;
         const uint64_t pc = p < end ? p : first;
@@ -324,6 +390,8 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
         for (int f = 0; f < DMAX; ++f)
             if (f < (int)d)
                 xn[f] = J.pts[(uint64_t)f * n + pc];
+        if (keep_ok)
+            ln = J.labels[pc];
     };
     fetch(first);
     for (uint64_t base = first; base < end; base += blockDim.x) {
@@ -333,12 +401,40 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
 #pragma unroll
         for (int f = 0; f < DMAX; ++f)
             x[f][0] = (f < (int)d) ? xn[f] : 0.0f;
+        const uint32_t prev = ln;
         if (base + blockDim.x < end)
             fetch(base + blockDim.x);
-        uint32_t lab[1];
-        argmin_lds<DMAX, 1>(x, cen_s, k, lab);
-        if (valid)
-            J.labels[p] = lab[0];
+        // a point inside its previous centroid's keep radius keeps the label
+        // (the scan would return it, keep_threshold); the others scan
+        uint32_t lab[1] = {0u};
+        bool keep = false;
+        if (keep_ok) {
+            const uint32_t a = prev < k ? prev : 0u;
+            const float *ca = cen_s + (size_t)a * DMAX;
+            float t = x[0][0] - ca[0];
+            float dist = t * t;
+#pragma unroll
+            for (int f = 1; f < DMAX; ++f) {
+                t = x[f][0] - ca[f];
+                dist = __builtin_fmaf(t, t, dist);
+            }
+            keep = prev < k && dist <= J.keep[a];
+            lab[0] = a;
+        }
+        const unsigned long long fail = __ballot(valid && !keep);
+        if (fail != 0) {
+            // few failures: the whole wave rescans those points; many: each
+            // failing lane scans its own
+            if (__builtin_popcountll(fail) <= 16) {
+                const uint32_t l = sparse_scan<DMAX>(x, cen_s, d, k, fail, lane, slot);
+                if (valid && !keep)
+                    lab[0] = l;
+            } else if (valid && !keep) {
+                argmin_lds<DMAX, 1>(x, cen_s, k, lab);
+            }
+            if (valid && !keep)
+                J.labels[p] = lab[0];
+        }
         const WaveRuns r = wave_runs(lab[0], valid, lane);
         float *row = T + (size_t)(r.last ? r.label : 0u) * ts;
 #pragma unroll
@@ -410,6 +506,109 @@ __device__ void merge_rows(const BJob &J, uint32_t e0)
 }
 
 // Divide the merged row 0, shift, advance the job.
+// keep_threshold of every centroid of [d][k] (global) into keep[k]: FP64
+// squared distances to the other centroids, NaN distances skipped.  The
+// work-group stages the centroids in LDS as rows [k][DB] zero-padded to the
+// bucket DB >= d (padding adds 0 exactly) when they fit; each wave takes 64
+// centroids a (one per lane, its row in registers) against a slice of the
+// centroids c, unrolled for independent LDS reads, and the slices' minima
+// meet in red.  The minimum does not depend on the split.
+constexpr uint32_t kKeepStage = 12288;  // floats of LDS for the staged centroids
+// Jobs with fewer clusters scan every centroid: their scan is too short to
+// repay the keep test and the previous label read.
+constexpr uint32_t kKeepMinK = 16;
+
+template <int DB>
+__device__ __forceinline__ double keep_min_rows(const float *__restrict__ cs, uint32_t a, uint32_t c0, uint32_t c1)
+{
+    double ca[DB];
+#pragma unroll
+    for (int f = 0; f < DB; ++f)
+        ca[f] = (double)cs[(size_t)a * DB + f];
+    // (c unrolled by U: independent LDS reads in flight, within the registers)
+    constexpr int U = DB <= 8 ? 4 : DB == 16 ? 2 : 1;
+    double s2 = __builtin_inf();
+    uint32_t c = c0;
+    for (; c + U <= c1; c += U) {
+        double acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            acc[u] = 0.0;
+#pragma unroll
+            for (int f = 0; f < DB; ++f) {
+                const double t = ca[f] - (double)cs[(size_t)(c + u) * DB + f];
+                acc[u] = __builtin_fma(t, t, acc[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            s2 = (c + u != a && acc[u] < s2) ? acc[u] : s2;  // a NaN distance never lowers it
+    }
+    for (; c < c1; ++c) {
+        double acc = 0.0;
+#pragma unroll
+        for (int f = 0; f < DB; ++f) {
+            const double t = ca[f] - (double)cs[(size_t)c * DB + f];
+            acc = __builtin_fma(t, t, acc);
+        }
+        s2 = (c != a && acc < s2) ? acc : s2;
+    }
+    return s2;
+}
+
+__device__ __forceinline__ void build_keep_thresholds(const float *__restrict__ cen, uint32_t d, uint32_t k,
+                                      float *__restrict__ keep, float *cs, double *red)
+{
+    const uint32_t tid = threadIdx.x, lane = tid % 64, wave = tid / 64, W = blockDim.x / 64;
+    const uint32_t db = d <= 4 ? 4u : d <= 8 ? 8u : d <= 16 ? 16u : d <= 32 ? 32u : 0u;
+    const bool staged = db != 0 && (uint64_t)k * db <= kKeepStage;
+    if (staged)
+        for (uint32_t i = tid; i < k * db; i += blockDim.x) {
+            const uint32_t c = i / db, f = i % db;
+            cs[i] = f < d ? cen[(size_t)f * k + c] : 0.0f;
+        }
+    __syncthreads();
+    const uint32_t nab = (k + 63) / 64;
+    const uint32_t slices = nab >= W ? 1u : W / nab;
+    for (uint32_t ab0 = 0; ab0 < nab; ab0 += W / slices) {
+        const uint32_t ab = ab0 + wave / slices, sl = wave % slices;
+        const uint32_t a = ab * 64 + lane;
+        const bool mine = ab < nab && a < k;
+        double s2 = __builtin_inf();
+        if (mine) {
+            const uint32_t per = (k + slices - 1) / slices;
+            const uint32_t c0 = sl * per, c1 = min(k, c0 + per);
+            if (staged) {
+                switch (db) {
+                case 4: s2 = keep_min_rows<4>(cs, a, c0, c1); break;
+                case 8: s2 = keep_min_rows<8>(cs, a, c0, c1); break;
+                case 16: s2 = keep_min_rows<16>(cs, a, c0, c1); break;
+                default: s2 = keep_min_rows<32>(cs, a, c0, c1); break;
+                }
+            } else {
+                for (uint32_t c = c0; c < c1; ++c) {
+                    double acc = 0.0;
+                    for (uint32_t f = 0; f < d; ++f) {
+                        const double t = (double)cen[(size_t)f * k + a] - (double)cen[(size_t)f * k + c];
+                        acc = __builtin_fma(t, t, acc);
+                    }
+                    s2 = (c != a && acc < s2) ? acc : s2;
+                }
+            }
+        }
+        if (slices > 1) {
+            red[tid] = s2;
+            __syncthreads();
+            if (sl == 0 && mine)
+                for (uint32_t q = 1; q < slices; ++q)
+                    s2 = fmin(s2, red[tid + q * 64]);
+            __syncthreads();
+        }
+        if (sl == 0 && mine)
+            keep[a] = keep_threshold(s2);
+    }
+}
+
 __device__ void batch_finalize(BJob &J, double *red)
 {
     const uint32_t tid = threadIdx.x;
@@ -760,6 +959,7 @@ __global__ void __launch_bounds__(256)
 batch_finish_kernel(BJob *__restrict__ jobs, const uint32_t *__restrict__ active, BCtl *__restrict__ ctl)
 {
     __shared__ double red[256];
+    __shared__ float cs[kKeepStage];
     if (blockIdx.x == 0 && threadIdx.x < kClasses)
         ctl[threadIdx.x].next = 0;
     if (ctl[0].dry != 0)
@@ -770,9 +970,19 @@ batch_finish_kernel(BJob *__restrict__ jobs, const uint32_t *__restrict__ active
     const uint32_t nxt = (J.iter & 1) ^ 1;
     const bool table = J.kind == 1;
     batch_finalize(J, red);
-    if (table) {
-        __syncthreads();
+    __syncthreads();
+    if (table)
         mfma::build_table_block<FP16>(J.cen[nxt], J.d, J.k, J.nt16, J.nkb, J.table, J.norm, J.stats, red);
+    // FP32-scan jobs (state, shift and keep_ok are uniform: written before
+    // the barrier).
+    // Centroids that did not move keep their thresholds; a shift of 0 can
+    // hide only a cluster that turned empty (finite -> NaN), whose old
+    // centroid in the minimum only makes the other thresholds smaller.
+    if (J.state == CLK_JOB_ACTIVE && J.kind == 0 && J.k >= kKeepMinK && !(J.keep_ok && J.shift == 0.0)) {
+        __syncthreads();
+        build_keep_thresholds(J.cen[nxt], J.d, J.k, J.keep, cs, red);
+        if (threadIdx.x == 0)
+            J.keep_ok = 1;
     }
 }
 
@@ -890,6 +1100,7 @@ uint64_t job_bytes(uint64_t n, uint32_t d, uint32_t k, uint32_t tile_pts, bool p
     b += clk::align_up(n * 4, 256);                       // labels
     b += 2 * clk::align_up((uint64_t)d * k * 4, 256);     // centroids x2
     b += clk::align_up((uint64_t)k * 4, 256);             // masses
+    b += clk::align_up((uint64_t)k * 4, 256);             // keep thresholds
     b += clk::align_up(nt * (uint64_t)k * (d + 1) * 8, 256);  // partial rows
     return b;
 }
@@ -1299,6 +1510,9 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
     p += clk::align_up((uint64_t)d * k * 4, 256);
     J.masses = reinterpret_cast<uint32_t *>(p);
     p += clk::align_up((uint64_t)k * 4, 256);
+    J.keep = reinterpret_cast<float *>(p);
+    p += clk::align_up((uint64_t)k * 4, 256);
+    J.keep_ok = 0;  // the first iteration scans every centroid
     J.part = reinterpret_cast<double *>(p);
     J.n = n;
     J.d = d;
@@ -1530,6 +1744,7 @@ extern "C" int clk_batch_run_serial(clk_batch *b, void *stream)
             if (rc != CLK_OK)
                 break;
             J.iter += 1;
+            J.keep_ok = 0;  // the keep thresholds are of older centroids
             if (J.tol > 0.0) {
                 hipError_t e = hipMemcpyAsync(&J.shift, shift_d, sizeof(double), hipMemcpyDeviceToHost, st);
                 if (e == hipSuccess)
