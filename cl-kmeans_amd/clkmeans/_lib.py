@@ -144,10 +144,19 @@ def load(path=None):
         "clk_reduce_vector_parcol_u32": (i32, [vp, u64, u64, vp, vp]),
         "clk_reduce_vector_parcol_f64": (i32, [vp, u64, u64, vp, vp]),
         "clk_fill_zero": (i32, [vp, vp, u64, vp]),
+        "clk_fused_f64": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp, vp, vp]),
+        "clk_lloyd_step_f64": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "clk_labeling_f64": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp]),
+        "clk_mass_update_u64": (i32, [vp, cfgp, u64, u64, vp, vp, vp]),
+        "clk_centroid_update_f64": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp, vp]),
+        "clk_matrix_divide_f64": (i32, [vp, u64, u64, vp, vp, vp, vp, vp, vp]),
+        "clk_matrix_add_u64": (i32, [vp, u64, vp, vp, vp]),
         "clk_init_first_x": (i32, [vp, u64, u64, u64, vp, vp, vp]),
         "clk_generate_blobs": (i32, [u64, u64, u64, dp, dp, u64, vp, vp, i32]),
         "clk_kmeans_run": (i32, [ctypes.c_char_p, i32, u64, u64, u64, vp, vp, vp, vp, u64, dp,
                                  ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+        "clk_kmeans_run_f64": (i32, [ctypes.c_char_p, i32, u64, u64, u64, vp, vp, vp, vp, u64, dp,
+                                     ctypes.POINTER(u64), ctypes.POINTER(u64)]),
         "clk_conf_describe": (i32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
         "clk_labeling_refined_points": (i32, [vp, ctypes.POINTER(ctypes.c_uint64)]),
         "clk_batch_job_bytes": (u64, [ctypes.POINTER(BatchConfig), u64, ctypes.c_uint32, ctypes.c_uint32]),
@@ -171,6 +180,14 @@ def load(path=None):
         fn.restype, fn.argtypes = res, args
     _lib = L
     return L
+
+
+def _dtype(buf):
+    return np.dtype(getattr(buf, "dtype", np.float32))
+
+
+def _is_f64(buf):
+    return _dtype(buf) == np.float64
 
 
 def check(rc):
@@ -215,19 +232,23 @@ def conf_describe(conf_text):
 
 def kmeans_run(conf_text, points, centroids, device=0, buffer_size=0, tolerance=0.0):
     """One clustering run of the strategy the .conf names (host arrays).
-    points [d][n] float32, centroids [d][k] float32 (initial).  Returns
-    (centroids, labels, masses, iterations_run, total_time_ns)."""
-    pts = np.ascontiguousarray(points, dtype=np.float32)
+    points [d][n] float32 and centroids [d][k] float32 (initial) for the
+    float/uint32/uint32 triple; float64 points select the double/uint64/uint64
+    triple (the .conf must name it).  Returns (centroids, labels, masses,
+    iterations_run, total_time_ns)."""
+    f64 = np.asarray(points).dtype == np.float64
+    pt, lt = (np.float64, np.uint64) if f64 else (np.float32, np.uint32)
+    pts = np.ascontiguousarray(points, dtype=pt)
     d, n = pts.shape
-    cen = np.array(centroids, dtype=np.float32, copy=True, order="C")
+    cen = np.array(centroids, dtype=pt, copy=True, order="C")
     k = cen.shape[1]
-    masses = np.zeros(k, np.uint32)
-    labels = np.zeros(max(n, 1), np.uint32)
+    masses = np.zeros(k, lt)
+    labels = np.zeros(max(n, 1), lt)
     it = ctypes.c_uint64()
     ns = ctypes.c_uint64()
-    check(load().clk_kmeans_run(conf_text.encode(), device, d, n, k, pts.ctypes.data, cen.ctypes.data,
-                                masses.ctypes.data, labels.ctypes.data, int(buffer_size), float(tolerance),
-                                ctypes.byref(it), ctypes.byref(ns)))
+    fn = load().clk_kmeans_run_f64 if f64 else load().clk_kmeans_run
+    check(fn(conf_text.encode(), device, d, n, k, pts.ctypes.data, cen.ctypes.data, masses.ctypes.data,
+             labels.ctypes.data, int(buffer_size), float(tolerance), ctypes.byref(it), ctypes.byref(ns)))
     return cen, labels[:n], masses, it.value, ns.value
 
 
@@ -284,12 +305,16 @@ class Context:
 
     # -- operators (device buffers, enqueue on `stream`) ---------------------
     def fused(self, cfg, d, n, k, points, old_centroids, sums, labels, masses, stream=None):
-        check(self.L.clk_fused(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),
+        """float32 points: uint32 labels/masses; float64 points: uint64 (the
+        double/uint64/uint64 triple)."""
+        fn = self.L.clk_fused_f64 if _is_f64(points) else self.L.clk_fused
+        check(fn(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),
                                _p(old_centroids), _p(sums), _p(labels), _p(masses), _s(stream)))
 
     def lloyd_step(self, cfg, d, n, k, points, old_centroids, new_centroids, sums, labels, masses,
                    max_shift=None, stream=None):
-        check(self.L.clk_lloyd_step(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),
+        fn = self.L.clk_lloyd_step_f64 if _is_f64(points) else self.L.clk_lloyd_step
+        check(fn(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),
                                     _p(old_centroids), _p(new_centroids), _p(sums), _p(labels), _p(masses),
                                     _p(max_shift), _s(stream)))
 
@@ -300,23 +325,28 @@ class Context:
         return out.value
 
     def labeling(self, cfg, d, n, k, points, centroids, labels, stream=None):
-        check(self.L.clk_labeling(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),
+        fn = self.L.clk_labeling_f64 if _is_f64(points) else self.L.clk_labeling
+        check(fn(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),
                                   _p(centroids), _p(labels), _s(stream)))
 
     def mass_update(self, cfg, n, k, labels, masses, stream=None):
-        check(self.L.clk_mass_update(self.handle, ctypes.byref(cfg) if cfg else None, n, k, _p(labels),
+        fn = self.L.clk_mass_update_u64 if _dtype(labels) == np.uint64 else self.L.clk_mass_update
+        check(fn(self.handle, ctypes.byref(cfg) if cfg else None, n, k, _p(labels),
                                      _p(masses), _s(stream)))
 
     def centroid_update(self, cfg, d, n, k, points, sums, labels, masses=None, stream=None):
-        check(self.L.clk_centroid_update(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k,
+        fn = self.L.clk_centroid_update_f64 if _is_f64(points) else self.L.clk_centroid_update
+        check(fn(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k,
                                          _p(points), _p(sums), _p(labels), _p(masses), _s(stream)))
 
     def divide(self, d, k, sums, masses, centroids, old_centroids=None, max_shift=None, stream=None):
-        check(self.L.clk_matrix_divide_row(self.handle, d, k, _p(sums), _p(masses), _p(centroids),
+        fn = self.L.clk_matrix_divide_f64 if _is_f64(centroids) else self.L.clk_matrix_divide_row
+        check(fn(self.handle, d, k, _p(sums), _p(masses), _p(centroids),
                                            _p(old_centroids), _p(max_shift), _s(stream)))
 
     def matrix_add(self, count, fst, snd, stream=None):
-        fn = self.L.clk_matrix_add_f64 if fst.dtype == np.float64 else self.L.clk_matrix_add_u32
+        fn = {np.dtype(np.float64): self.L.clk_matrix_add_f64,
+              np.dtype(np.uint64): self.L.clk_matrix_add_u64}.get(np.dtype(fst.dtype), self.L.clk_matrix_add_u32)
         check(fn(self.handle, count, _p(fst), _p(snd), _s(stream)))
 
     def reduce_vector_parcol(self, num_cols, rows, data, stream=None):

@@ -82,24 +82,24 @@ struct CmdOptions {
     }
 };
 
-int run(const CmdOptions &opt)
+// Bench<PointT, LabelT, MassT>::run (bench.cpp:200-452) for one type triple.
+template <typename P, typename L, typename M>
+int run_typed(const CmdOptions &opt, const ConfigurationParser &config)
 {
-    ConfigurationParser config;
-    config.parse_file(opt.config_file);
     const BenchmarkConfiguration bm_config = config.get_benchmark_configuration();
     const KmeansConfiguration km_config = config.get_kmeans_configuration();
 
-    Matrix<float> points;
+    Matrix<P> points;
     BinaryFormat::read(opt.input_file, points);
     const size_t n = points.rows, d = points.cols;
-    ClusteringBenchmark bm((uint32_t)bm_config.runs, km_config.iterations, std::move(points));
-    bm.initialize(km_config.clusters, KmeansInitializer::first_x);
+    ClusteringBenchmark<P, L, M> bm((uint32_t)bm_config.runs, km_config.iterations, std::move(points));
+    bm.initialize(km_config.clusters, KmeansInitializer::first_x<P>);
 
     const bool verify = opt.verify || bm_config.verify;
     if (verify)
-        bm.set_verification_reference(KmeansNaive());
+        bm.set_verification_reference(KmeansNaive<P, L, M>());
 
-    auto kmeans = make_kmeans(config, opt.driver);
+    auto kmeans = make_kmeans<P, L, M>(config, opt.driver);
     ClusteringBenchmarkStats bs((uint32_t)bm_config.runs);
     uint64_t wrong = 0;
     if (verify)
@@ -130,6 +130,16 @@ int run(const CmdOptions &opt)
     if (verify && wrong != 0)
         return 3;
     return 0;
+}
+
+// The type switch of bench.cpp:469-503.
+int run(const CmdOptions &opt)
+{
+    ConfigurationParser config;
+    config.parse_file(opt.config_file);
+    if (conf_is_double(config))
+        return run_typed<double, uint64_t, uint64_t>(opt, config);
+    return run_typed<float, uint32_t, uint32_t>(opt, config);
 }
 
 }  // namespace

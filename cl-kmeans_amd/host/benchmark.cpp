@@ -31,6 +31,15 @@ void BinaryFormat::read(const std::string &file, Matrix<float> &points)
         throw std::invalid_argument("\"" + file + "\": truncated point data");
 }
 
+void BinaryFormat::read(const std::string &file, Matrix<double> &points)
+{
+    Matrix<float> f;
+    read(file, f);
+    points.resize(f.rows, f.cols);
+    for (size_t i = 0; i < f.data.size(); ++i)
+        points.data[i] = f.data[i];
+}
+
 void BinaryFormat::write(const std::string &file, const Matrix<float> &points)
 {
     std::ofstream fh(file, std::ios::binary | std::ios::trunc);
@@ -40,14 +49,16 @@ void BinaryFormat::write(const std::string &file, const Matrix<float> &points)
              (std::streamsize)(points.data.size() * sizeof(float)));
 }
 
-void KmeansInitializer::first_x(const Matrix<float> &points, Matrix<float> &centroids)
+template <typename T>
+void KmeansInitializer::first_x(const Matrix<T> &points, Matrix<T> &centroids)
 {
     for (size_t f = 0; f < centroids.cols; ++f)
         for (size_t c = 0; c < centroids.rows; ++c)
             centroids(c, f) = points(c % points.rows, f);
 }
 
-void KmeansInitializer::forgy(const Matrix<float> &points, Matrix<float> &centroids)
+template <typename T>
+void KmeansInitializer::forgy(const Matrix<T> &points, Matrix<T> &centroids)
 {
     std::random_device rand;
     for (size_t c = 0; c < centroids.rows; ++c) {
@@ -57,53 +68,55 @@ void KmeansInitializer::forgy(const Matrix<float> &points, Matrix<float> &centro
     }
 }
 
-void KmeansNaive::operator()(uint32_t max_iterations, const Matrix<float> &points,
-                             Matrix<float> &centroids, std::vector<uint32_t> &masses,
-                             std::vector<uint32_t> &labels) const
+template <typename PointT, typename LabelT, typename MassT>
+void KmeansNaive<PointT, LabelT, MassT>::operator()(uint32_t max_iterations, const Matrix<PointT> &points,
+                                                    Matrix<PointT> &centroids, std::vector<MassT> &masses,
+                                                    std::vector<LabelT> &labels) const
 {
     const size_t n = points.rows, d = points.cols, k = centroids.rows;
     labels.assign(n, 0);
     masses.assign(k, 0);
-    std::vector<float> comp(k * d);
+    std::vector<PointT> comp(k * d);
     for (uint32_t it = 0; it < max_iterations; ++it) {
         for (size_t p = 0; p < n; ++p) {
-            float best = std::numeric_limits<float>::max();
-            uint32_t lab = 0;
+            PointT best = std::numeric_limits<PointT>::max();
+            LabelT lab = 0;
             for (size_t c = 0; c < k; ++c) {
-                float dist = 0.0f;
+                PointT dist = 0;
                 for (size_t f = 0; f < d; ++f) {
-                    float t = points(p, f) - centroids(c, f);
+                    PointT t = points(p, f) - centroids(c, f);
                     dist = std::fma(t, t, dist);
                 }
                 if (dist < best) {
                     best = dist;
-                    lab = (uint32_t)c;
+                    lab = (LabelT)c;
                 }
             }
             labels[p] = lab;
         }
-        std::fill(masses.begin(), masses.end(), 0u);
-        std::fill(centroids.data.begin(), centroids.data.end(), 0.0f);
-        std::fill(comp.begin(), comp.end(), 0.0f);
+        std::fill(masses.begin(), masses.end(), MassT(0));
+        std::fill(centroids.data.begin(), centroids.data.end(), PointT(0));
+        std::fill(comp.begin(), comp.end(), PointT(0));
         for (size_t p = 0; p < n; ++p) {
-            const uint32_t c = labels[p];
+            const LabelT c = labels[p];
             masses[c] += 1;
             for (size_t f = 0; f < d; ++f) {
-                float &s = centroids(c, f);
-                float &cm = comp[f * k + c];
-                float y = points(p, f) - cm;
-                float t = s + y;
+                PointT &s = centroids(c, f);
+                PointT &cm = comp[f * k + c];
+                PointT y = points(p, f) - cm;
+                PointT t = s + y;
                 cm = (t - s) - y;
                 s = t;
             }
         }
         for (size_t f = 0; f < d; ++f)
             for (size_t c = 0; c < k; ++c)
-                centroids(c, f) = centroids(c, f) / (float)masses[c];
+                centroids(c, f) = centroids(c, f) / (PointT)masses[c];
     }
 }
 
-double mse(const std::vector<float> &a, const std::vector<float> &b)
+template <typename T>
+double mse(const std::vector<T> &a, const std::vector<T> &b)
 {
     double sum = 0.0, comp = 0.0;
     uint64_t count = 0;
@@ -151,12 +164,15 @@ std::vector<std::string> ClusteringBenchmarkStats::to_csv(const std::string &csv
     return files;
 }
 
-ClusteringBenchmark::ClusteringBenchmark(uint32_t runs, size_t iterations, Matrix<float> &&points)
+template <typename PointT, typename LabelT, typename MassT>
+ClusteringBenchmark<PointT, LabelT, MassT>::ClusteringBenchmark(uint32_t runs, size_t iterations,
+                                                              Matrix<PointT> &&points)
     : runs_(runs), iterations_(iterations), points_(std::move(points))
 {
 }
 
-void ClusteringBenchmark::initialize(size_t num_clusters, InitCentroidsFunction init)
+template <typename PointT, typename LabelT, typename MassT>
+void ClusteringBenchmark<PointT, LabelT, MassT>::initialize(size_t num_clusters, InitCentroidsFunction init)
 {
     init_ = init;
     centroids_.resize(num_clusters, points_.cols);
@@ -164,15 +180,16 @@ void ClusteringBenchmark::initialize(size_t num_clusters, InitCentroidsFunction 
     labels_.assign(points_.rows, 0);
 }
 
-ClusteringBenchmarkStats ClusteringBenchmark::run(const ClClusteringFunction &f)
+template <typename PointT, typename LabelT, typename MassT>
+ClusteringBenchmarkStats ClusteringBenchmark<PointT, LabelT, MassT>::run(const ClClusteringFunction &f)
 {
     ClusteringBenchmarkStats bs(runs_);
     bs.set_dimensions(points_.cols, points_.rows, centroids_.rows);
     // Non-owning views of the benchmark's vectors (clustering_benchmark.cpp:178-193).
-    std::shared_ptr<const std::vector<float>> pts(&points_.data, [](const std::vector<float> *) {});
-    std::shared_ptr<std::vector<float>> cen(&centroids_.data, [](std::vector<float> *) {});
-    std::shared_ptr<std::vector<uint32_t>> ms(&masses_, [](std::vector<uint32_t> *) {});
-    std::shared_ptr<std::vector<uint32_t>> lb(&labels_, [](std::vector<uint32_t> *) {});
+    std::shared_ptr<const std::vector<PointT>> pts(&points_.data, [](const std::vector<PointT> *) {});
+    std::shared_ptr<std::vector<PointT>> cen(&centroids_.data, [](std::vector<PointT> *) {});
+    std::shared_ptr<std::vector<MassT>> ms(&masses_, [](std::vector<MassT> *) {});
+    std::shared_ptr<std::vector<LabelT>> lb(&labels_, [](std::vector<LabelT> *) {});
     for (uint32_t r = 0; r < runs_; ++r) {
         init_(points_, centroids_);
         auto t0 = std::chrono::steady_clock::now();
@@ -185,14 +202,17 @@ ClusteringBenchmarkStats ClusteringBenchmark::run(const ClClusteringFunction &f)
     return bs;
 }
 
-void ClusteringBenchmark::set_verification_reference(const KmeansNaive &naive)
+template <typename PointT, typename LabelT, typename MassT>
+void ClusteringBenchmark<PointT, LabelT, MassT>::set_verification_reference(
+    const KmeansNaive<PointT, LabelT, MassT> &naive)
 {
     reference_centroids_.resize(centroids_.rows, centroids_.cols);
     init_(points_, reference_centroids_);
     naive((uint32_t)iterations_, points_, reference_centroids_, reference_masses_, reference_labels_);
 }
 
-uint64_t ClusteringBenchmark::verify(const ClClusteringFunction &f)
+template <typename PointT, typename LabelT, typename MassT>
+uint64_t ClusteringBenchmark<PointT, LabelT, MassT>::verify(const ClClusteringFunction &f)
 {
     run(f);
     uint64_t wrong = 0;
@@ -201,11 +221,16 @@ uint64_t ClusteringBenchmark::verify(const ClClusteringFunction &f)
     return wrong;
 }
 
-double ClusteringBenchmark::mse() const { return Clustering::mse(centroids_.data, reference_centroids_.data); }
-
-void ClusteringBenchmark::print_result() const
+template <typename PointT, typename LabelT, typename MassT>
+double ClusteringBenchmark<PointT, LabelT, MassT>::mse() const
 {
-    auto dump = [](const char *title, const Matrix<float> &m) {
+    return Clustering::mse(centroids_.data, reference_centroids_.data);
+}
+
+template <typename PointT, typename LabelT, typename MassT>
+void ClusteringBenchmark<PointT, LabelT, MassT>::print_result() const
+{
+    auto dump = [](const char *title, const Matrix<PointT> &m) {
         std::cout << title << std::endl;
         for (size_t r = 0; r < m.rows; ++r) {
             for (size_t c = 0; c < m.cols; ++c)
@@ -223,5 +248,16 @@ void ClusteringBenchmark::print_result() const
         std::cout << m << " ";
     std::cout << std::endl;
 }
+
+template void KmeansInitializer::first_x(const Matrix<float> &, Matrix<float> &);
+template void KmeansInitializer::first_x(const Matrix<double> &, Matrix<double> &);
+template void KmeansInitializer::forgy(const Matrix<float> &, Matrix<float> &);
+template void KmeansInitializer::forgy(const Matrix<double> &, Matrix<double> &);
+template struct KmeansNaive<float, uint32_t, uint32_t>;
+template struct KmeansNaive<double, uint64_t, uint64_t>;
+template double mse(const std::vector<float> &, const std::vector<float> &);
+template double mse(const std::vector<double> &, const std::vector<double> &);
+template class ClusteringBenchmark<float, uint32_t, uint32_t>;
+template class ClusteringBenchmark<double, uint64_t, uint64_t>;
 
 }  // namespace Clustering

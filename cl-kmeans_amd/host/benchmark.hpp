@@ -32,29 +32,37 @@ struct Matrix {
 };
 
 // .bin files: uint64 num_features, uint64 num_clusters (must be 0),
-// uint64 num_points, then float32 [num_features][num_points].
+// uint64 num_points, then float32 [num_features][num_points].  A double
+// matrix is filled from the float32 values (binary_format.cpp:36-43 reads a
+// float per element into any matrix type).
 struct BinaryFormat {
     static void read(const std::string &file, Matrix<float> &points);
+    static void read(const std::string &file, Matrix<double> &points);
     static void write(const std::string &file, const Matrix<float> &points);
 };
 
 struct KmeansInitializer {
     // centroid c = point (c mod n)
-    static void first_x(const Matrix<float> &points, Matrix<float> &centroids);
+    template <typename T>
+    static void first_x(const Matrix<T> &points, Matrix<T> &centroids);
     // centroid c = a uniformly random point (std::random_device seeded)
-    static void forgy(const Matrix<float> &points, Matrix<float> &centroids);
+    template <typename T>
+    static void forgy(const Matrix<T> &points, Matrix<T> &centroids);
 };
 
-// Sequential CPU Lloyd with kmeans_naive's arithmetic: fma distance, strict
-// '<' argmin from FLT_MAX, Kahan-compensated FP32 sums, sum / (float)mass.
+// Sequential CPU Lloyd with kmeans_naive's arithmetic in PointT: fma
+// distance, strict '<' argmin from the type's max, Kahan-compensated sums,
+// sum / (PointT)mass.
+template <typename PointT, typename LabelT, typename MassT>
 struct KmeansNaive {
     const char *name() const { return "Lloyd_Naive"; }
-    void operator()(uint32_t max_iterations, const Matrix<float> &points, Matrix<float> &centroids,
-                    std::vector<uint32_t> &masses, std::vector<uint32_t> &labels) const;
+    void operator()(uint32_t max_iterations, const Matrix<PointT> &points, Matrix<PointT> &centroids,
+                    std::vector<MassT> &masses, std::vector<LabelT> &labels) const;
 };
 
 // Kahan-compensated mean of squared differences.
-double mse(const std::vector<float> &a, const std::vector<float> &b);
+template <typename T>
+double mse(const std::vector<T> &a, const std::vector<T> &b);
 
 struct ClusteringBenchmarkStats {
     explicit ClusteringBenchmarkStats(uint32_t runs = 1) : num_runs(runs) {}
@@ -76,34 +84,40 @@ struct ClusteringBenchmarkStats {
 };
 
 // Runs a strategy over host data: fresh initial centroids each run, labels
-// compared against a reference clustering in verify().
+// compared against a reference clustering in verify().  Instantiated for the
+// float/uint32/uint32 and double/uint64/uint64 triples (bench.cpp:469-503).
+template <typename PointT, typename LabelT, typename MassT>
 class ClusteringBenchmark {
 public:
     using ClClusteringFunction = std::function<std::shared_ptr<Measurement::Measurement>(
-        size_t iterations, size_t features, std::shared_ptr<const std::vector<float>> points,
-        std::shared_ptr<std::vector<float>> centroids, std::shared_ptr<std::vector<uint32_t>> masses,
-        std::shared_ptr<std::vector<uint32_t>> labels)>;
-    using InitCentroidsFunction = std::function<void(const Matrix<float> &, Matrix<float> &)>;
+        size_t iterations, size_t features, std::shared_ptr<const std::vector<PointT>> points,
+        std::shared_ptr<std::vector<PointT>> centroids, std::shared_ptr<std::vector<MassT>> masses,
+        std::shared_ptr<std::vector<LabelT>> labels)>;
+    using InitCentroidsFunction = std::function<void(const Matrix<PointT> &, Matrix<PointT> &)>;
 
-    ClusteringBenchmark(uint32_t runs, size_t iterations, Matrix<float> &&points);
+    ClusteringBenchmark(uint32_t runs, size_t iterations, Matrix<PointT> &&points);
     void initialize(size_t num_clusters, InitCentroidsFunction init);
     ClusteringBenchmarkStats run(const ClClusteringFunction &f);
-    void set_verification_reference(const KmeansNaive &naive);
+    void set_verification_reference(const KmeansNaive<PointT, LabelT, MassT> &naive);
     uint64_t verify(const ClClusteringFunction &f);
     double mse() const;
     void print_result() const;
 
-    const Matrix<float> &points() const { return points_; }
-    const std::vector<float> &centroids() const { return centroids_.data; }
-    const std::vector<uint32_t> &labels() const { return labels_; }
-    const std::vector<uint32_t> &masses() const { return masses_; }
+    const Matrix<PointT> &points() const { return points_; }
+    const std::vector<PointT> &centroids() const { return centroids_.data; }
+    const std::vector<LabelT> &labels() const { return labels_; }
+    const std::vector<MassT> &masses() const { return masses_; }
 
 private:
     uint32_t runs_;
     size_t iterations_;
-    Matrix<float> points_, centroids_, reference_centroids_;
-    std::vector<uint32_t> labels_, masses_, reference_labels_, reference_masses_;
+    Matrix<PointT> points_, centroids_, reference_centroids_;
+    std::vector<LabelT> labels_, reference_labels_;
+    std::vector<MassT> masses_, reference_masses_;
     InitCentroidsFunction init_;
 };
+
+using ClusteringBenchmarkF32 = ClusteringBenchmark<float, uint32_t, uint32_t>;
+using ClusteringBenchmarkF64 = ClusteringBenchmark<double, uint64_t, uint64_t>;
 
 }  // namespace Clustering

@@ -10,17 +10,23 @@ extern "C" void clk_set_error_message(const char *msg);
 
 namespace Clustering {
 
-using P = float;
-using L = uint32_t;
-using M = uint32_t;
-
-ClusteringBenchmark::ClClusteringFunction make_kmeans(const ConfigurationParser &cfg,
-                                                      const DriverOptions &opt,
-                                                      std::shared_ptr<size_t> iterations_run)
+bool conf_is_double(const ConfigurationParser &cfg)
 {
     const KmeansConfiguration km = cfg.get_kmeans_configuration();
-    if (!(km.point_type == "float" && km.label_type == "uint32" && km.mass_type == "uint32"))
-        throw std::invalid_argument("Invalid type");  // bench.cpp:502; double/uint64 has no gfx950 kernels
+    if (km.point_type == "double" && km.label_type == "uint64" && km.mass_type == "uint64")
+        return true;
+    if (km.point_type == "float" && km.label_type == "uint32" && km.mass_type == "uint32")
+        return false;
+    throw std::invalid_argument("Invalid type");  // bench.cpp:501-503
+}
+
+template <typename P, typename L, typename M>
+typename ClusteringBenchmark<P, L, M>::ClClusteringFunction
+make_kmeans(const ConfigurationParser &cfg, const DriverOptions &opt, std::shared_ptr<size_t> iterations_run)
+{
+    const KmeansConfiguration km = cfg.get_kmeans_configuration();
+    if (conf_is_double(cfg) != std::is_same<P, double>::value)
+        throw std::invalid_argument("Invalid type");
     ContextHandle ctx(opt.device);
     Queue queue(opt.device);
     std::shared_ptr<AbstractKmeans<P, L, M>> strategy;
@@ -60,10 +66,9 @@ ClusteringBenchmark::ClClusteringFunction make_kmeans(const ConfigurationParser 
     }
     strategy->set_convergence_tolerance(opt.tolerance);
     strategy->set_measurement(std::make_shared<Measurement::Measurement>());
-    return [strategy, iterations_run](size_t iters, size_t d, std::shared_ptr<const std::vector<float>> pts,
-                                      std::shared_ptr<std::vector<float>> cen,
-                                      std::shared_ptr<std::vector<uint32_t>> ms,
-                                      std::shared_ptr<std::vector<uint32_t>> lb) {
+    return [strategy, iterations_run](size_t iters, size_t d, std::shared_ptr<const std::vector<P>> pts,
+                                      std::shared_ptr<std::vector<P>> cen, std::shared_ptr<std::vector<M>> ms,
+                                      std::shared_ptr<std::vector<L>> lb) {
         auto m = (*strategy)(iters, d, pts, cen, ms, lb);
         if (iterations_run)
             *iterations_run = strategy->iterations_run();
@@ -71,17 +76,19 @@ ClusteringBenchmark::ClClusteringFunction make_kmeans(const ConfigurationParser 
     };
 }
 
-}  // namespace Clustering
+template ClusteringBenchmarkF32::ClClusteringFunction make_kmeans<float, uint32_t, uint32_t>(
+    const ConfigurationParser &, const DriverOptions &, std::shared_ptr<size_t>);
+template ClusteringBenchmarkF64::ClClusteringFunction make_kmeans<double, uint64_t, uint64_t>(
+    const ConfigurationParser &, const DriverOptions &, std::shared_ptr<size_t>);
 
-extern "C" int clk_kmeans_run(const char *conf_text, int device, uint64_t d, uint64_t n, uint64_t k,
-                              const float *host_points, float *host_centroids, uint32_t *host_masses,
-                              uint32_t *host_labels, uint64_t buffer_size, double tolerance,
-                              uint64_t *iterations_run, uint64_t *total_time_ns)
+// One clustering run of the .conf's strategy over host arrays of one triple.
+template <typename P, typename L, typename M>
+int kmeans_run(const char *conf_text, int device, uint64_t d, uint64_t n, uint64_t k, const P *host_points,
+               P *host_centroids, M *host_masses, L *host_labels, uint64_t buffer_size, double tolerance,
+               uint64_t *iterations_run, uint64_t *total_time_ns)
 {
-    using namespace Clustering;
     try {
-        if (!conf_text || !host_points || !host_centroids || !host_masses || !host_labels || d == 0 ||
-            k == 0)
+        if (!conf_text || !host_points || !host_centroids || !host_masses || !host_labels || d == 0 || k == 0)
             throw std::invalid_argument("clk_kmeans_run: null buffer or zero d/k");
         ConfigurationParser cfg;
         cfg.parse_string(conf_text);
@@ -94,16 +101,16 @@ extern "C" int clk_kmeans_run(const char *conf_text, int device, uint64_t d, uin
             opt.buffer_size = buffer_size;
         opt.tolerance = tolerance;
         auto iters = std::make_shared<size_t>(0);
-        auto f = make_kmeans(cfg, opt, iters);
-        auto pts = std::make_shared<const std::vector<float>>(host_points, host_points + n * d);
-        auto cen = std::make_shared<std::vector<float>>(host_centroids, host_centroids + k * d);
-        auto ms = std::make_shared<std::vector<uint32_t>>(k);
-        auto lb = std::make_shared<std::vector<uint32_t>>(n);
+        auto f = make_kmeans<P, L, M>(cfg, opt, iters);
+        auto pts = std::make_shared<const std::vector<P>>(host_points, host_points + n * d);
+        auto cen = std::make_shared<std::vector<P>>(host_centroids, host_centroids + k * d);
+        auto ms = std::make_shared<std::vector<M>>(k);
+        auto lb = std::make_shared<std::vector<L>>(n);
         auto m = f(km.iterations, d, pts, cen, ms, lb);
-        std::memcpy(host_centroids, cen->data(), k * d * sizeof(float));
-        std::memcpy(host_masses, ms->data(), k * sizeof(uint32_t));
+        std::memcpy(host_centroids, cen->data(), k * d * sizeof(P));
+        std::memcpy(host_masses, ms->data(), k * sizeof(M));
         if (n)
-            std::memcpy(host_labels, lb->data(), n * sizeof(uint32_t));
+            std::memcpy(host_labels, lb->data(), n * sizeof(L));
         if (iterations_run)
             *iterations_run = *iters;
         if (total_time_ns)
@@ -119,6 +126,30 @@ extern "C" int clk_kmeans_run(const char *conf_text, int device, uint64_t d, uin
         clk_set_error_message(e.what());
         return CLK_ERR_HIP;
     }
+}
+
+}  // namespace Clustering
+
+extern "C" int clk_kmeans_run(const char *conf_text, int device, uint64_t d, uint64_t n, uint64_t k,
+                              const float *host_points, float *host_centroids, uint32_t *host_masses,
+                              uint32_t *host_labels, uint64_t buffer_size, double tolerance,
+                              uint64_t *iterations_run, uint64_t *total_time_ns)
+{
+    return Clustering::kmeans_run<float, uint32_t, uint32_t>(conf_text, device, d, n, k, host_points,
+                                                            host_centroids, host_masses, host_labels,
+                                                            buffer_size, tolerance, iterations_run,
+                                                            total_time_ns);
+}
+
+extern "C" int clk_kmeans_run_f64(const char *conf_text, int device, uint64_t d, uint64_t n, uint64_t k,
+                                  const double *host_points, double *host_centroids, uint64_t *host_masses,
+                                  uint64_t *host_labels, uint64_t buffer_size, double tolerance,
+                                  uint64_t *iterations_run, uint64_t *total_time_ns)
+{
+    return Clustering::kmeans_run<double, uint64_t, uint64_t>(conf_text, device, d, n, k, host_points,
+                                                             host_centroids, host_masses, host_labels,
+                                                             buffer_size, tolerance, iterations_run,
+                                                             total_time_ns);
 }
 
 extern "C" int clk_conf_describe(const char *conf_text, char *out, size_t out_len)

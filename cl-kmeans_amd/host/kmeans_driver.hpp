@@ -18,9 +18,16 @@ struct DriverOptions {
 };
 
 // The returned function keeps the strategy alive; iterations_run (if given)
-// receives the number of iterations of the last call.
-ClusteringBenchmark::ClClusteringFunction make_kmeans(const ConfigurationParser &cfg,
-                                                      const DriverOptions &opt,
-                                                      std::shared_ptr<size_t> iterations_run = nullptr);
+// receives the number of iterations of the last call.  The .conf's types
+// must be the template's triple (std::invalid_argument("Invalid type")
+// otherwise, bench.cpp:502).
+template <typename PointT, typename LabelT, typename MassT>
+typename ClusteringBenchmark<PointT, LabelT, MassT>::ClClusteringFunction
+make_kmeans(const ConfigurationParser &cfg, const DriverOptions &opt,
+            std::shared_ptr<size_t> iterations_run = nullptr);
+
+// Whether the .conf names the double/uint64/uint64 triple (else it must name
+// float/uint32/uint32).
+bool conf_is_double(const ConfigurationParser &cfg);
 
 }  // namespace Clustering

@@ -27,10 +27,93 @@ using AccT = double;
 template <typename PointT, typename LabelT, typename MassT>
 constexpr void require_supported_types()
 {
-    static_assert(std::is_same<PointT, float>::value && std::is_same<LabelT, uint32_t>::value &&
-                      std::is_same<MassT, uint32_t>::value,
-                  "the gfx950 kernels implement the float/uint32/uint32 type triple");
+    static_assert((std::is_same<PointT, float>::value && std::is_same<LabelT, uint32_t>::value &&
+                   std::is_same<MassT, uint32_t>::value) ||
+                      (std::is_same<PointT, double>::value && std::is_same<LabelT, uint64_t>::value &&
+                       std::is_same<MassT, uint64_t>::value),
+                  "the gfx950 kernels implement the float/uint32/uint32 and double/uint64/uint64 "
+                  "type triples (bench.cpp:469-503)");
 }
+
+// The point type that goes with a label type (uint32 -> float, uint64 -> double).
+template <typename LabelT>
+using PointOfLabel = typename std::conditional<std::is_same<LabelT, uint64_t>::value, double, float>::type;
+
+// The C-ABI entry point of each operator for either type triple.
+namespace abi {
+inline int lloyd_step(clk_context *c, const clk_kernel_config *k, size_t d, size_t n, size_t kk, const float *p,
+                      const float *oc, float *nc, double *s, uint32_t *l, uint32_t *m, double *sh, hipStream_t st)
+{
+    return clk_lloyd_step(c, k, d, n, kk, p, oc, nc, s, l, m, sh, st);
+}
+inline int lloyd_step(clk_context *c, const clk_kernel_config *k, size_t d, size_t n, size_t kk, const double *p,
+                      const double *oc, double *nc, double *s, uint64_t *l, uint64_t *m, double *sh, hipStream_t st)
+{
+    return clk_lloyd_step_f64(c, k, d, n, kk, p, oc, nc, s, l, m, sh, st);
+}
+inline int fused(clk_context *c, const clk_kernel_config *k, size_t d, size_t n, size_t kk, const float *p,
+                 const float *oc, double *s, uint32_t *l, uint32_t *m, hipStream_t st)
+{
+    return clk_fused(c, k, d, n, kk, p, oc, s, l, m, st);
+}
+inline int fused(clk_context *c, const clk_kernel_config *k, size_t d, size_t n, size_t kk, const double *p,
+                 const double *oc, double *s, uint64_t *l, uint64_t *m, hipStream_t st)
+{
+    return clk_fused_f64(c, k, d, n, kk, p, oc, s, l, m, st);
+}
+inline int labeling(clk_context *c, const clk_kernel_config *k, size_t d, size_t n, size_t kk, const float *p,
+                    const float *cen, uint32_t *l, hipStream_t st)
+{
+    return clk_labeling(c, k, d, n, kk, p, cen, l, st);
+}
+inline int labeling(clk_context *c, const clk_kernel_config *k, size_t d, size_t n, size_t kk, const double *p,
+                    const double *cen, uint64_t *l, hipStream_t st)
+{
+    return clk_labeling_f64(c, k, d, n, kk, p, cen, l, st);
+}
+inline int mass_update(clk_context *c, const clk_kernel_config *k, size_t n, size_t kk, const uint32_t *l,
+                       uint32_t *m, hipStream_t st)
+{
+    return clk_mass_update(c, k, n, kk, l, m, st);
+}
+inline int mass_update(clk_context *c, const clk_kernel_config *k, size_t n, size_t kk, const uint64_t *l,
+                       uint64_t *m, hipStream_t st)
+{
+    return clk_mass_update_u64(c, k, n, kk, l, m, st);
+}
+inline int centroid_update(clk_context *c, const clk_kernel_config *k, size_t d, size_t n, size_t kk,
+                           const float *p, double *s, const uint32_t *l, const uint32_t *m, hipStream_t st)
+{
+    return clk_centroid_update(c, k, d, n, kk, p, s, l, m, st);
+}
+inline int centroid_update(clk_context *c, const clk_kernel_config *k, size_t d, size_t n, size_t kk,
+                           const double *p, double *s, const uint64_t *l, const uint64_t *m, hipStream_t st)
+{
+    return clk_centroid_update_f64(c, k, d, n, kk, p, s, l, m, st);
+}
+inline int divide(clk_context *c, size_t d, size_t k, const double *s, const uint32_t *m, float *cen,
+                  const float *old, double *sh, hipStream_t st)
+{
+    return clk_matrix_divide_row(c, d, k, s, m, cen, old, sh, st);
+}
+inline int divide(clk_context *c, size_t d, size_t k, const double *s, const uint64_t *m, double *cen,
+                  const double *old, double *sh, hipStream_t st)
+{
+    return clk_matrix_divide_f64(c, d, k, s, m, cen, old, sh, st);
+}
+inline int matrix_add(clk_context *c, size_t count, uint32_t *a, const uint32_t *b, hipStream_t st)
+{
+    return clk_matrix_add_u32(c, count, a, b, st);
+}
+inline int matrix_add(clk_context *c, size_t count, uint64_t *a, const uint64_t *b, hipStream_t st)
+{
+    return clk_matrix_add_u64(c, count, a, b, st);
+}
+inline int matrix_add(clk_context *c, size_t count, double *a, const double *b, hipStream_t st)
+{
+    return clk_matrix_add_f64(c, count, a, b, st);
+}
+}  // namespace abi
 
 // DataPoint name of a fused strategy id (the reference's kernel class names).
 inline std::string fused_kernel_name(int id)
@@ -104,7 +187,7 @@ public:
                                   double *shift, Measurement::DataPoint &dp, const WaitList &events) {
             dp.set_name(name);
             return timed_enqueue(q, dp, events, [&](hipStream_t s) {
-                check_status(clk_lloyd_step(context.get(), &k, d, n, kk, pts, oc, nc, sums, labels,
+                check_status(abi::lloyd_step(context.get(), &k, d, n, kk, pts, oc, nc, sums, labels,
                                             masses, shift, s),
                              "clk_lloyd_step");
             });
@@ -140,7 +223,7 @@ public:
             require_len(mb, me, kk, "fused masses");
             dp.set_name(name);
             return timed_enqueue(q, dp, events, [&](hipStream_t s) {
-                check_status(clk_fused(context.get(), &k, d, n, kk, pb.get(), ob.get(), nb.get(),
+                check_status(abi::fused(context.get(), &k, d, n, kk, pb.get(), ob.get(), nb.get(),
                                        lb.get(), mb.get(), s),
                              "clk_fused");
             });
@@ -163,7 +246,7 @@ public:
     LabelingFunction create(ContextHandle context, LabelingConfiguration config,
                             Measurement::Measurement &measurement)
     {
-        require_supported_types<PointT, LabelT, uint32_t>();
+        require_supported_types<PointT, LabelT, LabelT>();
         measurement.set_parameter("LabelingGlobalSize", std::to_string(config.global_size[0]));
         measurement.set_parameter("LabelingLocalSize", std::to_string(config.local_size[0]));
         int id = clk_strategy_id(CLK_STAGE_LABELING, config.strategy.c_str());
@@ -188,7 +271,7 @@ public:
             require_len(lb, le, n, "labeling labels");
             dp.set_name(k.strategy == CLK_LABELING_UNROLL_VECTOR ? "LabelingUnrollVector" : "LabelingMfma");
             return timed_enqueue(q, dp, events, [&](hipStream_t s) {
-                check_status(clk_labeling(context.get(), &k, d, n, kk, pb.get(), cb.get(), lb.get(), s),
+                check_status(abi::labeling(context.get(), &k, d, n, kk, pb.get(), cb.get(), lb.get(), s),
                              "clk_labeling");
             });
         };
@@ -209,7 +292,7 @@ public:
     MassUpdateFunction create(ContextHandle context, MassUpdateConfiguration config,
                               Measurement::Measurement &measurement)
     {
-        require_supported_types<float, LabelT, MassT>();
+        require_supported_types<PointOfLabel<LabelT>, LabelT, MassT>();
         measurement.set_parameter("MassUpdateGlobalSize", std::to_string(config.global_size[0]));
         measurement.set_parameter("MassUpdateLocalSize", std::to_string(config.local_size[0]));
         if (config.strategy == "part_private")
@@ -233,7 +316,7 @@ public:
             require_len(mb, me, kk, "mass_update masses");
             dp.set_name(name);
             return timed_enqueue(q, dp, events, [&](hipStream_t s) {
-                check_status(clk_mass_update(context.get(), &k, n, kk, lb.get(), mb.get(), s),
+                check_status(abi::mass_update(context.get(), &k, n, kk, lb.get(), mb.get(), s),
                              "clk_mass_update");
             });
         };
@@ -292,7 +375,7 @@ public:
             require_len(mb, me, kk, "centroid_update masses");
             dp.set_name(name);
             return timed_enqueue(q, dp, events, [&](hipStream_t s) {
-                check_status(clk_centroid_update(context.get(), &k, d, n, kk, pb.get(), cb.get(),
+                check_status(abi::centroid_update(context.get(), &k, d, n, kk, pb.get(), cb.get(),
                                                  lb.get(), mb.get(), s),
                              "clk_centroid_update");
             });
@@ -302,7 +385,7 @@ public:
 
 // ------------------------------------------------------ MatrixBinaryOp --
 
-// Divide: centroids[f,c] = (float)(sums[f,c] / masses[c]) in FP64 (empty
+// Divide: centroids[f,c] = (PointT)(sums[f,c] / masses[c]) in FP64 (empty
 // cluster -> NaN); Add: fst += snd.  The reference divides PointT sums in
 // place; here the FP64 sums and the FP32 centroids are separate buffers.
 template <typename T1, typename T2>
@@ -329,7 +412,7 @@ public:
         require_len(out_begin, out_end, num_cols * num_rows, "divide centroids");
         dp.set_name("MatrixBinaryOpRow");
         return timed_enqueue(q, dp, events, [&](hipStream_t s) {
-            check_status(clk_matrix_divide_row(context_.get(), num_cols, num_rows, sums_begin.get(),
+            check_status(abi::divide(context_.get(), num_cols, num_rows, sums_begin.get(),
                                                vector_begin.get(), out_begin.get(), old_centroids,
                                                max_shift, s),
                          "clk_matrix_divide_row");
@@ -347,14 +430,8 @@ public:
         require_len(snd_begin, snd_end, num_cols * num_rows, "matrix snd");
         dp.set_name("MatrixBinaryOpMatrix");
         return timed_enqueue(q, dp, events, [&](hipStream_t s) {
-            int rc;
-            if constexpr (std::is_same<T1, double>::value)
-                rc = clk_matrix_add_f64(context_.get(), num_cols * num_rows, fst_begin.get(),
-                                        snd_begin.get(), s);
-            else
-                rc = clk_matrix_add_u32(context_.get(), num_cols * num_rows, fst_begin.get(),
-                                        snd_begin.get(), s);
-            check_status(rc, "clk_matrix_add");
+            check_status(abi::matrix_add(context_.get(), num_cols * num_rows, fst_begin.get(), snd_begin.get(), s),
+                         "clk_matrix_add");
         });
     }
 

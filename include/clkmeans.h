@@ -248,6 +248,48 @@ int clk_fill_zero(clk_context *ctx, void *ptr, uint64_t bytes, void *stream);
 int clk_init_first_x(clk_context *ctx, uint64_t d, uint64_t n, uint64_t k,
                      const float *points, float *centroids, void *stream);
 
+/* ---- the double / uint64 / uint64 type triple ------------------------- */
+/*
+ * The reference instantiates every strategy for double points with uint64
+ * labels and masses as well (bench.cpp:469-503; the OpenCL kernels built
+ * with CL_POINT=double, CL_POINT_MAX=DBL_MAX, CL_LABEL/CL_MASS=ulong,
+ * fused_feature_sum.hpp:80-95, labeling_unroll_vector.hpp:64-76).  Same
+ * layouts and semantics as the float entry points above with double points
+ * and centroids, uint64 labels and masses, FP64 sums: the distance is the
+ * sequential FP64 fma from 0, argmin with strict '<' from DBL_MAX, label 0
+ * when nothing is smaller; every strategy id of a stage runs the same FP64
+ * pass (an LDS-tiled pass with a fixed-order merge where the tile fits, FP64
+ * / uint64 global atomics beyond).  Labels and masses are exact; sums carry
+ * FP64 rounding of a different summation order.
+ */
+int clk_fused_f64(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
+                  uint64_t n, uint64_t k, const double *points,
+                  const double *old_centroids, double *sums, uint64_t *labels,
+                  uint64_t *masses, void *stream);
+int clk_lloyd_step_f64(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
+                       uint64_t n, uint64_t k, const double *points,
+                       const double *old_centroids, double *new_centroids,
+                       double *sums, uint64_t *labels, uint64_t *masses,
+                       double *max_shift, void *stream);
+int clk_labeling_f64(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
+                     uint64_t n, uint64_t k, const double *points,
+                     const double *centroids, uint64_t *labels, void *stream);
+int clk_mass_update_u64(clk_context *ctx, const clk_kernel_config *cfg, uint64_t n,
+                        uint64_t k, const uint64_t *labels, uint64_t *masses,
+                        void *stream);
+int clk_centroid_update_f64(clk_context *ctx, const clk_kernel_config *cfg,
+                            uint64_t d, uint64_t n, uint64_t k, const double *points,
+                            double *sums, const uint64_t *labels,
+                            const uint64_t *masses, void *stream);
+/* centroids[f,c] = sums[f,c] / (double)masses[c] (0/0 = NaN); optional
+ * max shift as clk_matrix_divide_row. */
+int clk_matrix_divide_f64(clk_context *ctx, uint64_t d, uint64_t k,
+                          const double *sums, const uint64_t *masses,
+                          double *centroids, const double *old_centroids,
+                          double *max_shift, void *stream);
+int clk_matrix_add_u64(clk_context *ctx, uint64_t count, uint64_t *fst,
+                       const uint64_t *snd, void *stream);
+
 /* ---- host-side synthetic data ----------------------------------------- */
 
 /*

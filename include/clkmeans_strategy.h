@@ -33,6 +33,18 @@ int clk_kmeans_run(const char *conf_text, int device, uint64_t d, uint64_t n, ui
                    uint64_t *iterations_run, uint64_t *total_time_ns);
 
 /*
+ * The same for a .conf whose [kmeans] types are point = double, label =
+ * uint64, mass = uint64 (bench.cpp:469-481): double points and centroids,
+ * uint64 masses and labels.  clk_kmeans_run takes the float / uint32 /
+ * uint32 types; a .conf naming the other triple is rejected with
+ * CLK_ERR_INVALID_ARGUMENT ("Invalid type", bench.cpp:501-503).
+ */
+int clk_kmeans_run_f64(const char *conf_text, int device, uint64_t d, uint64_t n, uint64_t k,
+                       const double *host_points, double *host_centroids, uint64_t *host_masses,
+                       uint64_t *host_labels, uint64_t buffer_size, double tolerance,
+                       uint64_t *iterations_run, uint64_t *total_time_ns);
+
+/*
  * Parse conf_text with the ConfigurationParser rules and write the resulting
  * configuration structs as "key=value" lines (every field, defaults
  * included) into out.  No GPU is touched.  CLK_ERR_INVALID_ARGUMENT for an

@@ -31,6 +31,8 @@ def lib():
             "ora_label_f64": [vp, u64, u64, vp, u64, vp],
             "ora_dist_f32": [vp, u64, u64, vp, u64, u64, vp],
             "ora_mass": [vp, u64, u64, vp],
+            "ora_mass_u64": [vp, u64, u64, vp],
+            "ora_sum_f64_dp": [vp, u64, u64, vp, u64, vp],
             "ora_sum_f64": [vp, u64, u64, vp, u64, vp],
             "ora_sum_kahan_f32": [vp, u64, u64, vp, u64, vp],
             "ora_divide_from_f64": [vp, vp, u64, u64, vp],
@@ -92,6 +94,24 @@ def mass(labels, k):
     l, lp = _c(labels, np.uint32)
     out = np.empty(k, np.uint32)
     lib().ora_mass(lp, l.size, k, out.ctypes.data)
+    return out
+
+
+def mass_u64(labels, k):
+    """uint64 masses of uint64 labels (the double / uint64 triple)."""
+    l, lp = _c(labels, np.uint64)
+    out = np.empty(k, np.uint64)
+    lib().ora_mass_u64(lp, l.size, k, out.ctypes.data)
+    return out
+
+
+def sums_f64_dp(points, labels, k):
+    """FP64 sums in point order of float64 points [d][n] with uint64 labels."""
+    p, pp = _c(points, np.float64)
+    l, lp = _c(labels, np.uint64)
+    d, n = p.shape
+    out = np.empty((d, k), np.float64)
+    lib().ora_sum_f64_dp(pp, n, d, lp, k, out.ctypes.data)
     return out
 
 

@@ -3,6 +3,7 @@ keys, defaults, size lists, errors.  Parsing touches no GPU."""
 import glob
 import os
 
+import numpy as np
 import pytest
 
 import clkmeans
@@ -19,7 +20,17 @@ def test_shipped_mi355x_confs_parse(path):
     assert c["kmeans.clusters"] == "10" and c["kmeans.iterations"] == "30"
     pipe = c["kmeans.pipeline"]
     assert pipe in ("single_stage", "three_stage", "single_stage_buffered", "three_stage_buffered")
-    assert os.path.basename(path) == "amd_mi355x_%s.conf" % pipe
+    name = os.path.basename(path)
+    assert name.startswith("amd_mi355x_%s" % pipe)
+    # the variant suffix says what differs from the plain configuration
+    if name.endswith("_double.conf"):
+        assert (c["kmeans.types.point"], c["kmeans.types.label"], c["kmeans.types.mass"]) == \
+            ("double", "uint64", "uint64")
+    else:
+        assert (c["kmeans.types.point"], c["kmeans.types.label"], c["kmeans.types.mass"]) == \
+            ("float", "uint32", "uint32")
+    if name.endswith("_bounded.conf"):
+        assert c["kmeans.fused.strategy"] == "cluster_merge_bounded"
 
 
 @pytest.mark.skipif(not REF_CONFS, reason="reference tree absent")
@@ -79,3 +90,18 @@ def test_kmeans_run_rejects_bad_conf_before_touching_the_gpu():
     with pytest.raises(ValueError):
         clkmeans.kmeans_run("[kmeans]\npipeline = single_stage\nbogus = 1\n",
                             np.zeros((2, 8), np.float32), np.zeros((2, 2), np.float32))
+
+
+def test_type_triples_checked_before_the_gpu():
+    # bench.cpp:469-503: float/uint32/uint32 or double/uint64/uint64, else
+    # invalid_argument("Invalid type"); the data's type must match the .conf
+    pts = np.zeros((2, 8), np.float32)
+    cen = np.zeros((2, 2), np.float32)
+    base = "[kmeans]\nclusters = 2\npipeline = single_stage\niterations = 1\n"
+    for types, data in [("types.point = half\n", pts),
+                        ("types.point = double\ntypes.label = uint32\ntypes.mass = uint32\n", pts),
+                        ("types.point = double\ntypes.label = uint64\ntypes.mass = uint64\n", pts),
+                        ("", pts.astype(np.float64))]:
+        with pytest.raises(clkmeans.InvalidArgument) as e:
+            clkmeans.kmeans_run(base + types, data, cen.astype(data.dtype))
+        assert "Invalid type" in str(e.value)

@@ -61,6 +61,30 @@ def test_strategy_matches_oracle(data, pipeline, stages):
     assert centroid_rel_err(c, ref64, pts, rl) <= 1e-5
 
 
+DOUBLE = "types.point = double\ntypes.label = uint64\ntypes.mass = uint64\n"
+
+
+@pytest.mark.parametrize("pipeline,stages", [
+    ("single_stage", {"fused": {"strategy": "cluster_merge"}}),
+    ("three_stage", {"mass_update": {"strategy": "part_local"}, "centroid_update": {"strategy": "feature_sum"}}),
+    ("single_stage_buffered", {"fused": {"strategy": "cluster_merge"}}),
+    ("three_stage_buffered", {}),
+])
+def test_double_triple_matches_oracle(pipeline, stages):
+    # the reference's Bench<double, uint64_t, uint64_t> (bench.cpp:469-481)
+    n, d, k, iters = 50000, 4, 10, 12
+    pts32, _ = clkmeans.generate_blobs(n, d, k, sigma=3.0, box=10.0, seed=21)
+    pts = pts32.astype(np.float64) + np.random.default_rng(1).standard_normal((d, n)) * 1e-9
+    init = np.ascontiguousarray(pts[:, :k])
+    text = conf(pipeline, iterations=iters, k=k, **stages).replace("[kmeans]\n", "[kmeans]\n" + DOUBLE, 1)
+    c, l, m, it, _ = clkmeans.kmeans_run(text, pts, init, buffer_size=8 * 4 * 15000)
+    rc, rl, rm = O.lloyd_f64(pts, init, iters)
+    assert it == iters and l.dtype == np.uint64 and m.dtype == np.uint64
+    np.testing.assert_array_equal(l, rl)
+    np.testing.assert_array_equal(m, rm)
+    np.testing.assert_allclose(c, rc, rtol=1e-12, atol=1e-12)
+
+
 @pytest.mark.parametrize("labeling", ["mfma_fp16", "mfma_bf16"])
 def test_dense_three_stage_matches_oracle(labeling):
     # d=64, k=1024: MFMA labeling with refinement and the feature-slice

@@ -160,6 +160,25 @@ def test_mass_sums_divide_closed_forms():
     np.testing.assert_array_equal(cen32[:, live], kah[:, live] / m[live].astype(np.float32))
 
 
+def test_double_triple_closed_forms():
+    # the double / uint64 / uint64 functions: bincount, math.fsum per cell,
+    # and a Lloyd step of the float64 loop equal to those pieces
+    rng = np.random.default_rng(5)
+    pts = rng.standard_normal((4, 3000)) * 30
+    labels = rng.integers(0, 9, 3000).astype(np.uint64)
+    np.testing.assert_array_equal(O.mass_u64(labels, 9), np.bincount(labels.astype(np.int64), minlength=9))
+    s = O.sums_f64_dp(pts, labels, 9)
+    for f in range(4):
+        for c in range(9):
+            cell = pts[f, labels == c]
+            assert abs(s[f, c] - math.fsum(cell)) <= 1e-12 * np.abs(cell).sum()
+    init = np.ascontiguousarray(pts[:, :9])
+    c1, l1, m1 = O.lloyd_f64(pts, init, 1)
+    np.testing.assert_array_equal(l1, O.label_f64(pts, init))
+    np.testing.assert_array_equal(m1, O.mass_u64(l1, 9))
+    np.testing.assert_array_equal(c1, O.sums_f64_dp(pts, l1, 9) / m1.astype(np.float64))
+
+
 def test_first_x_and_k1():
     pts, _ = clkmeans.generate_blobs(1000, 3, 4, seed=3)
     fx = O.first_x(pts, 6)
