@@ -63,12 +63,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="fused", choices=["fused", "dense", "batch", "serve"])
     ap.add_argument("--serve-jobs", type=int, default=1000, help="serve workload: jobs in the arrival stream")
-    ap.add_argument("--rate", type=float, default=300.0, help="serve workload: Poisson arrival rate (jobs/s)")
+    ap.add_argument("--rate", type=float, default=10000.0,
+                    help="serve workload: Poisson arrival rate (jobs/s; the default saturates both loops)")
     ap.add_argument("--large-frac", type=float, default=0.02,
                     help="serve workload: share of large jobs (N=2^24, d=32, k=256)")
     ap.add_argument("--no-serial", action="store_true",
                     help="batch workload: skip the serial per-job loop (for profiling the batch kernels)")
     ap.add_argument("--jobs", type=int, default=4096, help="batch workload: number of jobs")
+    ap.add_argument("--tile-work", type=int, default=0,
+                    help="serve workload: points per cluster in a tile (0: the engine's default)")
     ap.add_argument("--batch-labeling", default="mfma_bf16",
                     help="batch workload: labeling strategy of the batch engine")
     ap.add_argument("--labeling", default="mfma_fp16", help="dense workload: labeling strategy")
@@ -568,7 +571,8 @@ def serve(args):
 
     def run(resident):
         reqs = requests(resident)
-        eng = clkmeans.Batch(ctx, 1024, int(big_bytes * 12 + (8 << 30)), 32, 256, labeling=args.batch_labeling)
+        eng = clkmeans.Batch(ctx, 1024, int(big_bytes * 12 + (8 << 30)), 32, 256, tile_work=args.tile_work,
+                             labeling=args.batch_labeling)
         st = clkmeans.serve(eng, reqs, stream.handle)
         eng.close()
         sreqs = requests(resident)

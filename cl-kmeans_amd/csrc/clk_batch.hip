@@ -932,10 +932,14 @@ batch_tiles_mfma_kernel(const BJob *__restrict__ jobs, const BTile *__restrict__
     });
 }
 
-// Size classes: FP32 scan, d bucket 4 / 8 / 16 / 32 / 64 (classes 0-9); MFMA,
-// d <= 16 / d <= 32 / d <= 64 (10-15); each with k <= 64 (even) or k > 64 (odd).
-constexpr uint32_t kClasses = 16;
-constexpr uint32_t kMfmaClass0 = 10, kMfmaNkb2Class = 14;
+// Size classes: FP32 scan, d bucket 3 / 4 / 8 / 16 / 32 / 64 (classes 0-11);
+// MFMA, d <= 16 / d <= 32 / d <= 64 (12-17); each with k <= 64 (even) or
+// k > 64 (odd).
+constexpr uint32_t kClasses = 18;
+constexpr uint32_t kMfmaClass0 = 12, kMfmaNkb2Class = 16;
+// Register buckets of the FP32 classes (a padded feature costs a sub and an
+// fma per centroid, so d = 3 gets its own bucket).
+constexpr uint32_t kFp32Dmax[] = {3, 4, 8, 16, 32, 64};
 
 // The merge of every active job's tile rows: one work-group per element
 // slice of a job (large jobs get many).
@@ -1118,7 +1122,10 @@ uint32_t tile_points(uint32_t tile_work, uint32_t d, uint32_t k)
 
 // The autotuner's tile_work candidates; a job's partial rows are allocated for
 // the smallest, so the tuner can re-tile admitted jobs.
-constexpr uint32_t kTileWorkCandidates[] = {16, 32, 64, 128, 256};
+constexpr uint32_t kTileWorkCandidates[] = {4, 8, 16, 32, 64, 128, 256};
+// tile_work when the configuration leaves it 0: small tiles keep the
+// persistent work-groups busy when few jobs are resident (serving).
+constexpr uint32_t kDefaultTileWork = 8;
 
 uint32_t row_tile_points(uint32_t tile_work, uint32_t d, uint32_t k)
 {
@@ -1160,8 +1167,9 @@ uint32_t class_of(uint32_t kind, uint32_t d, uint32_t k)
 {
     const uint32_t big = k > kSmallK;
     if (kind == 0) {
-        const uint32_t dm = clk::bucket_of(d);  // 4 .. 64
-        const uint32_t i = dm == 4 ? 0 : dm == 8 ? 1 : dm == 16 ? 2 : dm == 32 ? 3 : 4;
+        uint32_t i = 0;
+        while (i + 1 < sizeof(clk::kFp32Dmax) / sizeof(clk::kFp32Dmax[0]) && clk::kFp32Dmax[i] < d)
+            ++i;
         return 2 * i + big;
     }
     return clk::kMfmaClass0 + 2 * (d <= 16 ? 0u : d <= 32 ? 1u : 2u) + big;
@@ -1171,10 +1179,11 @@ const void *class_kernel(const clk_batch *b, uint32_t c)
 {
     if (c < clk::kMfmaClass0) {
         switch (c / 2) {
-        case 0: return (const void *)clk::batch_tiles_kernel<4>;
-        case 1: return (const void *)clk::batch_tiles_kernel<8>;
-        case 2: return (const void *)clk::batch_tiles_kernel<16>;
-        case 3: return (const void *)clk::batch_tiles_kernel<32>;
+        case 0: return (const void *)clk::batch_tiles_kernel<3>;
+        case 1: return (const void *)clk::batch_tiles_kernel<4>;
+        case 2: return (const void *)clk::batch_tiles_kernel<8>;
+        case 3: return (const void *)clk::batch_tiles_kernel<16>;
+        case 4: return (const void *)clk::batch_tiles_kernel<32>;
         default: return (const void *)clk::batch_tiles_kernel<64>;
         }
     }
@@ -1191,7 +1200,7 @@ const void *class_kernel(const clk_batch *b, uint32_t c)
 size_t class_lds(uint32_t c, uint32_t waves, uint32_t kmax, uint32_t dmax, bool priv, clk::BLayout *layout)
 {
     if (c < clk::kMfmaClass0)
-        return clk::batch_lds_bytes(waves, kmax, 4u << (c / 2), dmax);
+        return clk::batch_lds_bytes(waves, kmax, clk::kFp32Dmax[c / 2], dmax);
     const clk::BLayout L = clk::mfma_layout_for(waves, kmax, dmax, priv);
     if (layout)
         *layout = L;
@@ -1312,10 +1321,11 @@ int launch_step(clk_batch *b, hipStream_t stream)
         const dim3 g(C.grid), t(64 * C.waves);
         if (c < clk::kMfmaClass0) {
             switch (c / 2) {
-            case 0: hipLaunchKernelGGL(clk::batch_tiles_kernel<4>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
-            case 1: hipLaunchKernelGGL(clk::batch_tiles_kernel<8>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
-            case 2: hipLaunchKernelGGL(clk::batch_tiles_kernel<16>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
-            case 3: hipLaunchKernelGGL(clk::batch_tiles_kernel<32>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
+            case 0: hipLaunchKernelGGL(clk::batch_tiles_kernel<3>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
+            case 1: hipLaunchKernelGGL(clk::batch_tiles_kernel<4>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
+            case 2: hipLaunchKernelGGL(clk::batch_tiles_kernel<8>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
+            case 3: hipLaunchKernelGGL(clk::batch_tiles_kernel<16>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
+            case 4: hipLaunchKernelGGL(clk::batch_tiles_kernel<32>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
             default: hipLaunchKernelGGL(clk::batch_tiles_kernel<64>, g, t, C.lds, stream, b->d_jobs, tl, ctl); break;
             }
         } else if (b->fp16) {
@@ -1353,7 +1363,7 @@ int launch_step(clk_batch *b, hipStream_t stream)
 
 extern "C" uint64_t clk_batch_job_bytes(const clk_batch_config *cfg, uint64_t n, uint32_t d, uint32_t k)
 {
-    const uint32_t tw = (cfg && cfg->tile_work) ? cfg->tile_work : 64;
+    const uint32_t tw = (cfg && cfg->tile_work) ? cfg->tile_work : kDefaultTileWork;
     return job_bytes(n, d, k, row_tile_points(tw, d, k));
 }
 
@@ -1368,7 +1378,7 @@ extern "C" int clk_batch_create(clk_context *ctx, const clk_batch_config *cfg, c
     clk_batch *b = new clk_batch;
     b->ctx = ctx;
     b->cfg = *cfg;
-    b->tile_work = cfg->tile_work ? cfg->tile_work : 64;
+    b->tile_work = cfg->tile_work ? cfg->tile_work : kDefaultTileWork;
     CLK_REQUIRE(cfg->labeling == CLK_LABELING_UNROLL_VECTOR || cfg->labeling == CLK_LABELING_MFMA_BF16 ||
                     cfg->labeling == CLK_LABELING_MFMA_FP16,
                 CLK_ERR_INVALID_ARGUMENT, "clk_batch_create: unknown labeling strategy id");
@@ -1380,7 +1390,7 @@ extern "C" int clk_batch_create(clk_context *ctx, const clk_batch_config *cfg, c
         for (uint32_t c = 0; c < clk::kClasses; ++c) {
             const bool big = (c & 1u) != 0;
             const uint32_t kmax = big ? cfg->max_k : std::min(cfg->max_k, kSmallK);
-            const uint32_t dcap = c < clk::kMfmaClass0 ? (4u << (c / 2)) : (16u << ((c - clk::kMfmaClass0) / 2));
+            const uint32_t dcap = c < clk::kMfmaClass0 ? clk::kFp32Dmax[c / 2] : (16u << ((c - clk::kMfmaClass0) / 2));
             const uint32_t dmax = std::min(cfg->max_d, dcap);
             b->class_priv[c] = class_private(b, c, kmax, dmax);
             b->class_waves[c] = class_waves(b, c, kmax, dmax, b->class_priv[c]);

@@ -35,7 +35,7 @@ typedef struct clk_batch clk_batch;
 typedef struct clk_batch_config {
     uint32_t max_jobs;        /* slots */
     uint64_t arena_bytes;     /* device memory for job data (points, labels, centroids, partials) */
-    uint32_t tile_work;       /* points per tile per cluster (tiles of 1024..16384 points); 0 = 64;
+    uint32_t tile_work;       /* points per tile per cluster (tiles of 1024..16384 points); 0 = 8;
                                  clk_batch_autotune picks one */
     uint32_t max_d;           /* largest d admitted (<= 64) */
     uint32_t max_k;           /* largest k admitted */
@@ -87,7 +87,7 @@ int clk_batch_run_serial(clk_batch *b, void *stream);
 
 /* Autotune tile_work on the current jobs, before their first iteration:
  * times label + accumulate passes (no job advances) for tile_work in
- * {16, 32, 64, 128, 256}, re-tiles the jobs with the fastest and keeps it for
+ * {4, 8, 16, 32, 64, 128, 256}, re-tiles the jobs with the fastest and keeps it for
  * jobs admitted later.  The tiling fixes each job's FP32 summation order, so
  * a job's bits depend on tile_work: pass the chosen value as
  * clk_batch_config.tile_work to reproduce them (the tuned value takes the

@@ -213,7 +213,7 @@ def test_batch_autotune_picks_a_tiling_and_reproduces(gpu_ctx, stream):
     b = clkmeans.Batch(gpu_ctx, max_jobs=16, arena_bytes=256 << 20, max_d=64, max_k=256, labeling="mfma_fp16")
     slots = [b.add(p, c, 5, stream=stream.handle) for p, c in jobs]
     chosen = b.autotune(stream.handle)
-    assert chosen in (16, 32, 64, 128, 256)
+    assert chosen in (4, 8, 16, 32, 64, 128, 256)
     for _ in range(5):
         b.step(stream.handle)
     tuned = [b.take(s, stream.handle) for s in slots]
