@@ -59,6 +59,7 @@ def main():
                     help="fused strategy; other than cluster_merge, the exhaustive cluster_merge pass is timed "
                          "in the same run too (--no-compare skips it)")
     ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--stage-labeling", default="unroll_vector", help="three_stage pipeline: labeling strategy")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="fused", choices=["fused", "dense", "batch", "serve"])
@@ -124,7 +125,7 @@ def main():
     sums = hip.DeviceArray((d, k), np.float64)
     masses = hip.DeviceArray((k,), np.uint32)
     labels = hip.DeviceArray((n,), np.uint32)
-    lcfg = ctx.config(clkmeans.STAGE_LABELING, "unroll_vector")
+    lcfg = ctx.config(clkmeans.STAGE_LABELING, args.stage_labeling)
     mcfg = ctx.config(clkmeans.STAGE_MASS_UPDATE, "part_private")
     ccfg = ctx.config(clkmeans.STAGE_CENTROID_UPDATE, "cluster_merge")
     s = stream.handle
@@ -151,7 +152,8 @@ def main():
 
     def hot_name(strategy):
         if args.pipeline != "single_stage":
-            return "LabelingUnrollVector"
+            return {"unroll_vector_bounded": "LabelingUnrollVectorBounded"}.get(args.stage_labeling,
+                                                                              "LabelingUnrollVector")
         return {"feature_sum": "FusedFeatureSum",
                 "cluster_merge_bounded": "FusedClusterMergeBounded"}.get(strategy, "FusedClusterMerge")
 
@@ -246,7 +248,7 @@ def main():
         "config": {"workload": "plain fused Lloyd, N=2^%d d=%d k=%d float32 feature-major, sigma=1 blobs "
                                "in [-50,50]^%d, seed 0, first-x init" % (args.log2n, d, k, d),
                    "pipeline": args.pipeline, "strategy": args.strategy if args.pipeline == "single_stage" else
-                   "unroll_vector/part_private/cluster_merge", "n": n, "d": d, "k": k},
+                   "%s/part_private/cluster_merge" % args.stage_labeling, "n": n, "d": d, "k": k},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": hot,
                      "kernel_ms": hot_ms, "bytes_per_launch": bytes_per_launch,

@@ -33,9 +33,11 @@ inline int dmax_bucket(uint64_t d)
 
 // Labels every point with the reference arithmetic; used by the fused ops
 // for the global-accumulation variant and by clk_labeling.
+// bounded: the "unroll_vector_bounded" variant (labels read first, kept
+// labels not rescanned; the same labels).
 int launch_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64_t n,
                     uint64_t k, const float *points, const float *centroids, uint32_t *labels,
-                    hipStream_t stream);
+                    hipStream_t stream, bool bounded = false);
 
 // Labels from x·cᵀ on MFMA (16-bit operands, FP32 accumulation) with an exact
 // FP32 relabel of near ties: the same labels as launch_labeling.  Supported

@@ -15,14 +15,23 @@ namespace {
 template <int DMAX>
 constexpr int label_lds_threads() { return DMAX >= 64 ? 512 : 1024; }
 
-template <int DMAX, int VEC>
+// BOUNDED ("unroll_vector_bounded"): the labels buffer is read first; a
+// point whose distance to its previous label's centroid is within that
+// centroid's keep threshold (keep_threshold: the scan provably returns the
+// same label) is not scanned, and only changed labels are stored.
+template <int DMAX, int VEC, bool BOUNDED>
 __global__ void __launch_bounds__(label_lds_threads<DMAX>())
 label_lds_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
                  const float *__restrict__ cen, uint32_t *__restrict__ labels)
 {
     extern __shared__ __align__(16) float cen_lds[];
+    float *thr = cen_lds + ((k * DMAX + 3u) & ~3u);  // BOUNDED: [k]
     stage_centroids<DMAX>(cen, d, k, cen_lds);
     __syncthreads();
+    if constexpr (BOUNDED) {
+        stage_keep_thresholds<DMAX>(cen_lds, k, thr);
+        __syncthreads();
+    }
     const uint64_t groups = n / VEC;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += stride) {
@@ -30,8 +39,37 @@ label_lds_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t
         float x[DMAX][VEC];
         load_points<DMAX, VEC>(pts, n, d, p, x);
         uint32_t lab[VEC];
-        argmin_lds<DMAX, VEC>(x, cen_lds, k, lab);
-        store_labels<VEC>(labels, p, lab);
+        if constexpr (BOUNDED) {
+            uint32_t prev[VEC];
+            stream_load_labels<VEC>(labels, p, prev);
+            bool keep = true;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) {
+                const bool valid = prev[v] < k;
+                const uint32_t a = valid ? prev[v] : 0u;
+                const float *ca = cen_lds + a * DMAX;
+                float t = x[0][v] - ca[0];
+                float dist = t * t;
+#pragma unroll
+                for (int f = 1; f < DMAX; ++f) {
+                    t = x[f][v] - ca[f];
+                    dist = __builtin_fmaf(t, t, dist);
+                }
+                keep = keep && valid && dist <= thr[a];
+                lab[v] = a;
+            }
+            if (!keep)
+                argmin_lds<DMAX, VEC>(x, cen_lds, k, lab);
+            bool changed = false;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v)
+                changed = changed || lab[v] != prev[v];
+            if (changed)
+                store_labels<VEC>(labels, p, lab);
+        } else {
+            argmin_lds<DMAX, VEC>(x, cen_lds, k, lab);
+            store_labels<VEC>(labels, p, lab);
+        }
     }
 }
 
@@ -151,29 +189,30 @@ using LabelKernel = void (*)(const float *, uint64_t, uint32_t, uint32_t, const 
 // Points per work-item at most (VEC x DMAX values in registers, no spills).
 constexpr int label_vec_cap(int dmax) { return dmax >= 64 ? 1 : dmax >= 32 ? 2 : 4; }
 
-template <int DMAX>
+template <int DMAX, bool BOUNDED>
 LabelKernel pick_label_vec(int vec)
 {
     constexpr int cap = label_vec_cap(DMAX);
     if constexpr (cap >= 4)
         if (vec >= 4)
-            return label_lds_kernel<DMAX, 4>;
+            return label_lds_kernel<DMAX, 4, BOUNDED>;
     if constexpr (cap >= 2)
         if (vec >= 2)
-            return label_lds_kernel<DMAX, 2>;
-    return label_lds_kernel<DMAX, 1>;
+            return label_lds_kernel<DMAX, 2, BOUNDED>;
+    return label_lds_kernel<DMAX, 1, BOUNDED>;
 }
 
+template <bool BOUNDED>
 LabelKernel pick_label_lds(int dmax, int vec)
 {
     switch (dmax) {
-    case 1: return pick_label_vec<1>(vec);
-    case 2: return pick_label_vec<2>(vec);
-    case 4: return pick_label_vec<4>(vec);
-    case 8: return pick_label_vec<8>(vec);
-    case 16: return pick_label_vec<16>(vec);
-    case 32: return pick_label_vec<32>(vec);
-    case 64: return pick_label_vec<64>(vec);
+    case 1: return pick_label_vec<1, BOUNDED>(vec);
+    case 2: return pick_label_vec<2, BOUNDED>(vec);
+    case 4: return pick_label_vec<4, BOUNDED>(vec);
+    case 8: return pick_label_vec<8, BOUNDED>(vec);
+    case 16: return pick_label_vec<16, BOUNDED>(vec);
+    case 32: return pick_label_vec<32, BOUNDED>(vec);
+    case 64: return pick_label_vec<64, BOUNDED>(vec);
     default: return nullptr;
     }
 }
@@ -226,7 +265,7 @@ LabelKernel pick_label_global(int dmax)
 
 int launch_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64_t n,
                     uint64_t k, const float *points, const float *centroids, uint32_t *labels,
-                    hipStream_t stream)
+                    hipStream_t stream, bool bounded)
 {
     if (n == 0)
         return CLK_OK;
@@ -234,18 +273,22 @@ int launch_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, 
     CLK_REQUIRE(B <= (uint64_t)ctx->max_threads_per_block, CLK_ERR_INVALID_ARGUMENT,
                 "labeling: local_size exceeds the device limit");
     const int dmax = dmax_bucket(d);
-    const uint64_t cen_bytes = dmax ? k * dmax * 4 : 0;
+    const uint64_t cen_plain = dmax ? ((k * dmax + 3) & ~3ull) * 4 : 0;
+    // the bounded variant adds k keep thresholds; beyond LDS it runs as the
+    // exhaustive kernels below (the same labels)
+    bounded = bounded && dmax && cen_plain + k * 4 <= ctx->lds_per_block;
+    const uint64_t cen_bytes = cen_plain + (bounded ? k * 4 : 0);
     if (dmax && cen_bytes <= ctx->lds_per_block) {
         const int vec = std::min(pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels),
                                  label_vec_cap(dmax));
-        LabelKernel kern = pick_label_lds(dmax, vec);
+        LabelKernel kern = bounded ? pick_label_lds<true>(dmax, vec) : pick_label_lds<false>(dmax, vec);
         CLK_REQUIRE(dmax < 64 || B <= 512, CLK_ERR_INVALID_ARGUMENT,
                     "labeling: local_size above 512 with more than 32 features");
         uint64_t grid = (cfg && cfg->global_size[0])
                             ? div_up(cfg->global_size[0], B)
                             : auto_grid(ctx, (const void *)kern, (int)B, cen_bytes, n / vec);
         grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, 1u << 20));
-        KernelTimer t(ctx, "LabelingUnrollVector", stream);
+        KernelTimer t(ctx, bounded ? "LabelingUnrollVectorBounded" : "LabelingUnrollVector", stream);
         hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3((unsigned)B), cen_bytes, stream,
                            points, n, (uint32_t)d, (uint32_t)k, centroids, labels);
         CLK_HIP_CHECK(hipGetLastError());
@@ -300,13 +343,14 @@ extern "C" int clk_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint
                 "clk_labeling: config is not a labeling-stage config");
     const int strategy = cfg ? cfg->strategy : CLK_LABELING_UNROLL_VECTOR;
     CLK_REQUIRE(strategy == CLK_LABELING_UNROLL_VECTOR || strategy == CLK_LABELING_MFMA_BF16 ||
-                    strategy == CLK_LABELING_MFMA_FP16,
+                    strategy == CLK_LABELING_MFMA_FP16 || strategy == CLK_LABELING_UNROLL_VECTOR_BOUNDED,
                 CLK_ERR_INVALID_ARGUMENT, "clk_labeling: unknown strategy id");
     CLK_REQUIRE(d >= 1 && k >= 1 && k <= 0xFFFFFFFFull, CLK_ERR_SIZE, "labeling: bad d or k");
     CLK_REQUIRE(points && centroids && labels, CLK_ERR_INVALID_ARGUMENT, "labeling: null buffer");
-    if (strategy != CLK_LABELING_UNROLL_VECTOR && clk::mfma_labeling_supported(ctx, d, n, k))
+    const bool mfma = strategy == CLK_LABELING_MFMA_BF16 || strategy == CLK_LABELING_MFMA_FP16;
+    if (mfma && clk::mfma_labeling_supported(ctx, d, n, k))
         return clk::launch_labeling_mfma(ctx, strategy == CLK_LABELING_MFMA_FP16, d, n, k, points,
                                          centroids, labels, clk::as_stream(stream));
-    return clk::launch_labeling(ctx, cfg, d, n, k, points, centroids, labels,
-                                clk::as_stream(stream));
+    return clk::launch_labeling(ctx, cfg, d, n, k, points, centroids, labels, clk::as_stream(stream),
+                                strategy == CLK_LABELING_UNROLL_VECTOR_BOUNDED);
 }

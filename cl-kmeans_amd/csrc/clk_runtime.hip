@@ -485,6 +485,7 @@ extern "C" int clk_strategy_id(int stage, const char *name)
         if (s == "unroll_vector") return CLK_LABELING_UNROLL_VECTOR;
         if (s == "mfma_bf16") return CLK_LABELING_MFMA_BF16;
         if (s == "mfma_fp16") return CLK_LABELING_MFMA_FP16;
+        if (s == "unroll_vector_bounded") return CLK_LABELING_UNROLL_VECTOR_BOUNDED;
         break;
     case CLK_STAGE_MASS_UPDATE:
         if (s == "global_atomic") return CLK_MASS_GLOBAL_ATOMIC;

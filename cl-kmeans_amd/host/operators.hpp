@@ -269,7 +269,9 @@ public:
             require_len(pb, pe, n * d, "labeling points");
             require_len(cb, ce, kk * d, "labeling centroids");
             require_len(lb, le, n, "labeling labels");
-            dp.set_name(k.strategy == CLK_LABELING_UNROLL_VECTOR ? "LabelingUnrollVector" : "LabelingMfma");
+            dp.set_name(k.strategy == CLK_LABELING_UNROLL_VECTOR           ? "LabelingUnrollVector"
+                        : k.strategy == CLK_LABELING_UNROLL_VECTOR_BOUNDED ? "LabelingUnrollVectorBounded"
+                                                                           : "LabelingMfma");
             return timed_enqueue(q, dp, events, [&](hipStream_t s) {
                 check_status(abi::labeling(context.get(), &k, d, n, kk, pb.get(), cb.get(), lb.get(), s),
                              "clk_labeling");

@@ -75,6 +75,11 @@ typedef struct clk_context clk_context;
  * sizes run the "unroll_vector" kernel. */
 #define CLK_LABELING_MFMA_BF16 1    /* "mfma_bf16" */
 #define CLK_LABELING_MFMA_FP16 2    /* "mfma_fp16" */
+/* Extension: "unroll_vector" that reads the labels buffer first and keeps a
+ * point's previous label after one distance where the keep radius of
+ * CLK_FUSED_CLUSTER_MERGE_BOUNDED proves the scan would return it; only
+ * changed labels are stored.  The labels equal "unroll_vector"'s. */
+#define CLK_LABELING_UNROLL_VECTOR_BOUNDED 3 /* "unroll_vector_bounded" */
 #define CLK_MASS_GLOBAL_ATOMIC 0    /* "global_atomic"  mass_update_factory.hpp:71-75 */
 #define CLK_MASS_PART_GLOBAL 1      /* "part_global"    mass_update_factory.hpp:76-80 */
 #define CLK_MASS_PART_LOCAL 2       /* "part_local"     mass_update_factory.hpp:81-85 */

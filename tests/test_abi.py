@@ -44,6 +44,7 @@ def test_strategy_ids_follow_the_factories():
     assert S.strategy_id(S.STAGE_LABELING, "unroll_vector") == 0
     assert S.strategy_id(S.STAGE_LABELING, "mfma_bf16") == 1  # extensions
     assert S.strategy_id(S.STAGE_LABELING, "mfma_fp16") == 2
+    assert S.strategy_id(S.STAGE_LABELING, "unroll_vector_bounded") == 3
     for i, s in enumerate(["global_atomic", "part_global", "part_local", "part_private"]):
         assert S.strategy_id(S.STAGE_MASS_UPDATE, s) == i
     for i, s in enumerate(["feature_sum", "feature_sum_pardim", "cluster_merge"]):

@@ -183,3 +183,31 @@ def test_bounded_beyond_private_tile(gpu_ctx):
         np.testing.assert_array_equal(lab, ref)
         np.testing.assert_array_equal(mass, np.bincount(ref, minlength=k).astype(np.uint32))
         cen = nxt
+
+
+LABEL_CASES = [
+    # n, d, k, config overrides
+    (1 << 16, 4, 10, {}),
+    (777, 5, 4, {}),
+    (30000, 8, 12, {"local_size": 128}),
+    (4096, 64, 3, {}),
+    (3000, 16, 1200, {}),        # centroids beyond LDS: the chunked scan (no keep test)
+]
+
+
+@pytest.mark.parametrize("n,d,k,kw", LABEL_CASES)
+def test_bounded_labeling_equals_scan(gpu_ctx, n, d, k, kw):
+    # "unroll_vector_bounded" keeps the labels buffer between calls as the
+    # three-stage strategies do; garbage first, then the previous labels
+    pts, _ = clkmeans.generate_blobs(n, d, k, sigma=1.0, box=20.0, seed=n + k)
+    cen = O.first_x(pts, k)
+    rng = np.random.default_rng(2)
+    dl = dev(rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32))
+    cfg = gpu_ctx.config(clkmeans.STAGE_LABELING, "unroll_vector_bounded", **kw)
+    dp = dev(pts)
+    for _ in range(4):
+        gpu_ctx.labeling(cfg, d, n, k, dp, dev(cen), dl)
+        hip.synchronize()
+        ref = O.label(pts, cen)
+        np.testing.assert_array_equal(dl.to_host(), ref)
+        cen = O.lloyd(pts, cen, 1)[0]

@@ -45,6 +45,8 @@ CASES = [
     ("single_stage_buffered", {"fused": {"strategy": "cluster_merge"}}),
     ("three_stage_buffered", {"mass_update": {"strategy": "part_global"}}),
     ("three_stage", {"labeling": {"strategy": "mfma_fp16"}}),
+    ("three_stage", {"labeling": {"strategy": "unroll_vector_bounded"}}),
+    ("three_stage_buffered", {"labeling": {"strategy": "unroll_vector_bounded"}}),
     ("three_stage_buffered", {"labeling": {"strategy": "mfma_bf16"}, "mass_update": {"strategy": "part_local"}}),
 ]
 
