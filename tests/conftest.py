@@ -18,7 +18,7 @@ def gpu_ctx():
     import clkmeans
     from clkmeans import hip
     n = hip.device_count()
-    assert n >= 1, "no HIP device visible: GPU tests must run on the GPU box"
+    assert n >= 1, "no HIP device visible: the GPU tests need an MI355X"
     ctx = clkmeans.Context(0)
     yield ctx
     ctx.close()
