@@ -82,6 +82,11 @@ struct BJob {
     // points seldom passes there, and the test measured slower.)
     float *keep;
     uint32_t keep_ok, pad1;
+    // MFMA jobs with d <= 32: the current centroids as rows [k][cen_t_w]
+    // (zero-padded to 4/8/16/32 floats), refreshed by the finish kernel, for
+    // the keep test and the exact rescan of the tile kernel
+    float *cen_t;
+    uint32_t cen_t_w, pad2;
 };
 
 struct BTile {
@@ -514,6 +519,12 @@ __device__ void merge_rows(const BJob &J, uint32_t e0)
 // centroids c, unrolled for independent LDS reads, and the slices' minima
 // meet in red.  The minimum does not depend on the split.
 constexpr uint32_t kKeepStage = 12288;  // floats of LDS for the staged centroids
+
+// Width of a staged centroid row: d rounded up to 4 / 8 / 16 / 32; 0 beyond.
+__host__ __device__ inline uint32_t keep_row_width(uint32_t d)
+{
+    return d <= 4 ? 4u : d <= 8 ? 8u : d <= 16 ? 16u : d <= 32 ? 32u : 0u;
+}
 // Jobs with fewer clusters scan every centroid: their scan is too short to
 // repay the keep test and the previous label read.
 constexpr uint32_t kKeepMinK = 16;
@@ -556,18 +567,25 @@ __device__ __forceinline__ double keep_min_rows(const float *__restrict__ cs, ui
     return s2;
 }
 
+// rows_out (optional): the staged rows also written to global memory.
+// thresholds = false: only the rows.
 __device__ __forceinline__ void build_keep_thresholds(const float *__restrict__ cen, uint32_t d, uint32_t k,
-                                      float *__restrict__ keep, float *cs, double *red)
+                                      float *__restrict__ keep, float *cs, double *red,
+                                      float *__restrict__ rows_out = nullptr, bool thresholds = true)
 {
     const uint32_t tid = threadIdx.x, lane = tid % 64, wave = tid / 64, W = blockDim.x / 64;
-    const uint32_t db = d <= 4 ? 4u : d <= 8 ? 8u : d <= 16 ? 16u : d <= 32 ? 32u : 0u;
+    const uint32_t db = keep_row_width(d);
     const bool staged = db != 0 && (uint64_t)k * db <= kKeepStage;
     if (staged)
         for (uint32_t i = tid; i < k * db; i += blockDim.x) {
             const uint32_t c = i / db, f = i % db;
             cs[i] = f < d ? cen[(size_t)f * k + c] : 0.0f;
+            if (rows_out)
+                rows_out[i] = cs[i];
         }
     __syncthreads();
+    if (!thresholds)
+        return;
     const uint32_t nab = (k + 63) / 64;
     const uint32_t slices = nab >= W ? 1u : W / nab;
     for (uint32_t ab0 = 0; ab0 < nab; ab0 += W / slices) {
@@ -730,6 +748,13 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
     unsigned long long *wmin = reinterpret_cast<unsigned long long *>(lds + Lo.o_min) + (size_t)wave * 64;
     uint32_t *pc = reinterpret_cast<uint32_t *>(lds + Lo.o_count) + wave;
     const float *cen = J.cen[J.iter & 1];
+    // keep test (jobs with centroid rows): a point inside its previous
+    // centroid's keep radius keeps its label (keep_threshold); a wave whose
+    // points all pass skips the scores, one with one or two failures rescans
+    // those exactly from the rows, others take the MFMA path
+    const bool keep_ok = J.keep_ok != 0;
+    const uint32_t w4 = J.cen_t_w / 4;
+    const f32v4 *rows = reinterpret_cast<const f32v4 *>(J.cen_t);
 
     for (uint32_t e = tid; e < nt16 * NKB * 64; e += blockDim.x)
         ltab[e] = J.table[e];
@@ -782,15 +807,97 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         PointOps<NKB> x;
         float X2[kGroups], RX2[kGroups];
         build_operands<FP16, NKB>(raw, x, X2, RX2);
+        const uint64_t pme = base + lane;  // lane l holds point base + l in xc
+        const bool vme = pme < end;
+        uint32_t prev = 0;
         {
-            const uint64_t pc = base + lane < end ? base + lane : first;
+            const uint64_t pc = vme ? pme : first;
 #pragma unroll
             for (int f = 0; f < (int)P; ++f)
                 if (f < (int)d)
                     xc[f] = J.pts[(uint64_t)f * n + pc];
+            if (keep_ok)
+                prev = J.labels[pc];
         }
         if (bt + waves < nb)
             fetch(first + (bt + waves) * 64);
+        bool keep = false;
+        if (keep_ok) {
+            const uint32_t a = prev < k ? prev : 0u;
+            float dist = 0.0f;
+#pragma unroll
+            for (int q = 0; q < (int)P / 4; ++q)
+                if ((uint32_t)q < w4) {
+                    const f32v4 c4 = rows[(size_t)a * w4 + q];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (4 * q + j < (int)d) {
+                            const float t = xc[4 * q + j] - c4[j];
+                            dist = __builtin_fmaf(t, t, dist);
+                        }
+                }
+            keep = prev < k && dist <= J.keep[a];
+        }
+        const unsigned long long fail = keep_ok ? __ballot(vme && !keep) : ~0ull;
+        const uint32_t nfail = (uint32_t)__builtin_popcountll(fail);
+        uint32_t mine;
+        uint64_t mp;
+        if (keep_ok && nfail <= 2) {
+            // kept labels stand; one or two failing points are rescanned
+            // exactly: G lanes per point over centroids r, r+G, ..., the
+            // (distance, index) minima combined with the lowest index on
+            // ties (the reference's strict '<' scan)
+            mine = prev < k ? prev : 0u;
+            mp = vme ? pme : ~0ull;
+            if (nfail != 0) {
+                const uint32_t G = nfail == 1 ? 64u : 32u;
+                const uint32_t src0 = (uint32_t)__builtin_ctzll(fail);
+                const uint32_t src1 = nfail == 2 ? (uint32_t)__builtin_ctzll(fail & (fail - 1)) : src0;
+                const uint32_t grp = lane / G, r = lane % G;
+                const uint32_t src = grp == 0 ? src0 : src1;
+                float xs[P];
+#pragma unroll
+                for (int f = 0; f < (int)P; ++f)
+                    xs[f] = f < (int)d ? __shfl(xc[f], (int)src) : 0.0f;
+                float best = FLT_MAX;
+                uint32_t bi = 0xFFFFFFFFu;
+                for (uint32_t c = r; c < k; c += G) {
+                    float dist = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < (int)P / 4; ++q)
+                        if ((uint32_t)q < w4) {
+                            const f32v4 c4 = rows[(size_t)c * w4 + q];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                if (4 * q + j < (int)d) {
+                                    const float t = xs[4 * q + j] - c4[j];
+                                    dist = __builtin_fmaf(t, t, dist);
+                                }
+                        }
+                    if (dist < best) {
+                        best = dist;
+                        bi = c;
+                    }
+                }
+                for (uint32_t off = 1; off < G; off <<= 1) {
+                    const float ob = __shfl_xor(best, (int)off);
+                    const uint32_t oi = (uint32_t)__shfl_xor((int)bi, (int)off);
+                    if (ob < best || (ob == best && oi < bi)) {
+                        best = ob;
+                        bi = oi;
+                    }
+                }
+                const uint32_t lab = bi == 0xFFFFFFFFu ? 0u : bi;
+                // the failing lane takes its group's result
+                const uint32_t got0 = (uint32_t)__shfl((int)lab, 0);
+                const uint32_t got1 = (uint32_t)__shfl((int)lab, (int)G % 64);
+                if (vme && !keep) {
+                    mine = lane == src0 ? got0 : got1;
+                    if (mine != prev)
+                        J.labels[pme] = mine;
+                }
+            }
+        } else {
         const WaveBounds w = wave_bounds<NKB>(X2, RX2, J.stats, J.prm);
         uint32_t best[kGroups], second[kGroups];
         rank_all<FP16, NKB>(ltab, lnorm, nt16, lane, h, w.K, kmask, x, best, second);
@@ -861,9 +968,9 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
                 }
             wave_lds_order();
         }
-        // labels: lane (q, h) stores group h's point
-        uint32_t mine = lab[0];
-        uint64_t mp = pid[0];
+        // labels: lane (q, h) stores group h's point (= base + lane)
+        mine = lab[0];
+        mp = pid[0];
 #pragma unroll
         for (int g = 1; g < kGroups; ++g)
             if ((uint32_t)g == h) {
@@ -872,6 +979,7 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
             }
         if (mp < n)
             J.labels[mp] = mine;
+        }
         // lane l holds point base + l (xc); sorted by label, the points are
         // summed along their runs (the sums replace xc)
         r = wave_runs(mine, mp < n, lane);
@@ -982,9 +1090,15 @@ batch_finish_kernel(BJob *__restrict__ jobs, const uint32_t *__restrict__ active
     // Centroids that did not move keep their thresholds; a shift of 0 can
     // hide only a cluster that turned empty (finite -> NaN), whose old
     // centroid in the minimum only makes the other thresholds smaller.
-    if (J.state == CLK_JOB_ACTIVE && J.kind == 0 && J.k >= kKeepMinK && !(J.keep_ok && J.shift == 0.0)) {
-        __syncthreads();
-        build_keep_thresholds(J.cen[nxt], J.d, J.k, J.keep, cs, red);
+    // MFMA jobs with d <= 32 also get their centroid rows every iteration.
+    const bool fp32_keep = J.kind == 0 && J.k >= kKeepMinK;
+    const bool mfma_keep = J.kind == 1 && J.cen_t_w != 0 && (uint64_t)J.k * J.cen_t_w <= kKeepStage;
+    if (J.state == CLK_JOB_ACTIVE && (fp32_keep || mfma_keep)) {
+        const bool thr = !(J.keep_ok && J.shift == 0.0);
+        if (thr || mfma_keep) {
+            __syncthreads();
+            build_keep_thresholds(J.cen[nxt], J.d, J.k, J.keep, cs, red, mfma_keep ? J.cen_t : nullptr, thr);
+        }
         if (threadIdx.x == 0)
             J.keep_ok = 1;
     }
@@ -1105,6 +1219,7 @@ uint64_t job_bytes(uint64_t n, uint32_t d, uint32_t k, uint32_t tile_pts, bool p
     b += 2 * clk::align_up((uint64_t)d * k * 4, 256);     // centroids x2
     b += clk::align_up((uint64_t)k * 4, 256);             // masses
     b += clk::align_up((uint64_t)k * 4, 256);             // keep thresholds
+    b += clk::align_up((uint64_t)k * clk::keep_row_width(d) * 4, 256);  // centroid rows
     b += clk::align_up(nt * (uint64_t)k * (d + 1) * 8, 256);  // partial rows
     return b;
 }
@@ -1523,6 +1638,9 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
     J.keep = reinterpret_cast<float *>(p);
     p += clk::align_up((uint64_t)k * 4, 256);
     J.keep_ok = 0;  // the first iteration scans every centroid
+    J.cen_t = reinterpret_cast<float *>(p);
+    J.cen_t_w = clk::keep_row_width(d);
+    p += clk::align_up((uint64_t)k * J.cen_t_w * 4, 256);
     J.part = reinterpret_cast<double *>(p);
     J.n = n;
     J.d = d;

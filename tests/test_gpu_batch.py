@@ -67,20 +67,24 @@ def test_batch_matches_oracle(gpu_ctx, stream, labeling):
     b.close()
 
 
-def test_batch_keep_radius_paths(gpu_ctx, stream):
-    # FP32-scan jobs from the second iteration on: points inside their
-    # centroid's keep radius keep their labels, waves with a few other points
-    # rescan them with all lanes, waves with many scan per lane.  Jobs with
+@pytest.mark.parametrize("labeling", ["unroll_vector", "mfma_bf16"])
+def test_batch_keep_radius_paths(gpu_ctx, stream, labeling):
+    # From the second iteration on: points inside their centroid's keep
+    # radius keep their labels; the FP32 classes rescan a wave's few other
+    # points with all lanes (many: per lane), the MFMA classes rescan one or
+    # two exactly from the centroid rows (more: the MFMA path).  Jobs with
     # separated blobs (mostly kept), overlapping blobs (many rescans), a
     # duplicated initial centroid (no keep radius for either copy) and an
-    # initial centroid far from every point (an empty cluster: NaN centroid).
-    b = clkmeans.Batch(gpu_ctx, max_jobs=8, arena_bytes=256 << 20, max_d=32, max_k=256, labeling="unroll_vector")
+    # initial centroid far from every point (an empty cluster: NaN centroid);
+    # with mfma_bf16 the k*d >= 1024 jobs take the MFMA classes.
+    b = clkmeans.Batch(gpu_ctx, max_jobs=8, arena_bytes=256 << 20, max_d=32, max_k=256, labeling=labeling)
     jobs = []
     specs = [(40000, 4, 10, 1.0, 50.0), (40000, 3, 40, 2.0, 10.0), (30000, 8, 60, 1.0, 20.0),
-             (20000, 16, 12, 3.0, 10.0), (25000, 32, 20, 1.0, 20.0)]
+             (20000, 16, 12, 3.0, 10.0), (25000, 32, 20, 1.0, 20.0), (30000, 16, 80, 1.0, 20.0),
+             (30000, 32, 48, 2.0, 10.0)]
     for i, (n, d, k, sigma, box) in enumerate(specs):
         pts, init = make_job(n, d, k, seed=300 + i, sigma=sigma, box=box)
-        if i == 2:
+        if i in (2, 5):
             init[:, 5] = init[:, 3]           # duplicated centroid
             init[:, 7] = 1e6                  # far away: empty from the first iteration
         jobs.append((b.add(pts, init, max_iterations=12, stream=stream.handle), pts, init))
