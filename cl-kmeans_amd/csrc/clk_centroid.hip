@@ -141,7 +141,7 @@ cu_feature_sum_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uin
 // multiplicity in the wave, usually 1-3 rounds).  The wave tables are summed
 // in FP64 into the work-group's partial row.  Work-groups are sized so a
 // table cell takes about 64 points (FP32 partial sums with few roundings).
-constexpr uint32_t kSliceF = 4, kSliceWaves = 4;
+constexpr uint32_t kSliceF = 4, kSliceWaves = 4, kSlicePrefetch = 8;
 
 size_t slice_lds_bytes(uint64_t k)
 {
@@ -154,7 +154,10 @@ cu_slice_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t 
 {
     extern __shared__ __align__(16) float tab[];  // [waves][k] float4, then [waves][k] owners
     typedef float f4 __attribute__((ext_vector_type(4)));
-    const uint32_t tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+    // the wave index as a scalar: tile indices and the loop below are then
+    // wave-uniform, so the prefetch ring needs no divergent control flow
+    const uint32_t tid = threadIdx.x, lane = tid % 64;
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid / 64));
     f4 *T = reinterpret_cast<f4 *>(tab) + (size_t)wave * k;
     uint32_t *O = reinterpret_cast<uint32_t *>(reinterpret_cast<f4 *>(tab) + (size_t)kSliceWaves * k) +
                   (size_t)wave * k;
@@ -165,25 +168,35 @@ cu_slice_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t 
     const uint32_t nf = min(kSliceF, d - f0);
     const uint64_t tiles = (n + 63) / 64;
     const uint64_t tstride = (uint64_t)gridDim.x * kSliceWaves;
-    // the next tile's features and label are loaded while this one is added
+    // Every load is unconditional and in bounds (a tile past the end reads
+    // the last tile, a feature past the slice reads its last feature; such
+    // values are never added or are zeroed at use), so no branch separates
+    // a load from its use and the loads of later tiles stay in flight.
     auto load = [&](uint64_t tile, f4 &v, uint32_t &L) {
-        const uint64_t p = tile * 64 + lane;
-        const uint64_t pc = p < n ? p : n - 1;
+        const uint64_t p = min(min(tile, tiles - 1) * 64 + lane, n - 1);
 #pragma unroll
         for (uint32_t j = 0; j < kSliceF; ++j)
-            v[j] = j < nf ? pts[(uint64_t)(f0 + j) * n + pc] : 0.0f;
-        L = labels[pc];
+            v[j] = pts[(uint64_t)(f0 + min(j, nf - 1)) * n + p];
+        L = labels[p];
     };
-    uint64_t tile = (uint64_t)blockIdx.x * kSliceWaves + wave;
-    f4 vn;
-    uint32_t Ln = 0;
-    if (tile < tiles)
-        load(tile, vn, Ln);
-    for (; tile < tiles; tile += tstride) {
-        const f4 v = vn;
-        const uint32_t L = Ln;
-        if (tile + tstride < tiles)
-            load(tile + tstride, vn, Ln);
+    // A ring of kSlicePrefetch tiles in registers: the loads of the next
+    // kSlicePrefetch tiles of this wave are in flight while one is added
+    // (one tile alone leaves too few bytes in flight to cover HBM latency
+    // at the few waves per CU the tables allow).  The ring is indexed by
+    // the unrolled slot, so it stays in registers.
+    f4 vr[kSlicePrefetch];
+    uint32_t Lr[kSlicePrefetch];
+    const uint64_t first = (uint64_t)blockIdx.x * kSliceWaves + wave;
+    // this wave's tiles first + i*tstride, i < mine: full rounds of the ring
+    // in a loop with no early exit (so every path into its head has issued
+    // the same loads), then the partial last round
+    const uint64_t mine = first < tiles ? (tiles - first + tstride - 1) / tstride : 0;
+    const uint64_t rounds = mine / kSlicePrefetch;
+    const uint32_t rest = (uint32_t)(mine % kSlicePrefetch);
+    auto add = [&](uint64_t tile, f4 v, uint32_t L) {
+#pragma unroll
+        for (uint32_t j = 0; j < kSliceF; ++j)
+            v[j] = j < nf ? v[j] : 0.0f;
         const bool valid = tile * 64 + lane < n;
         bool pending = valid && L < k;
         while (__ballot(pending) != 0) {
@@ -196,7 +209,24 @@ cu_slice_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t 
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             pending = pending && !win;
         }
+    };
+#pragma unroll
+    for (uint32_t s = 0; s < kSlicePrefetch; ++s)
+        load(first + s * tstride, vr[s], Lr[s]);
+    uint64_t base = first;
+    for (uint64_t r = 0; r < rounds; ++r, base += kSlicePrefetch * tstride) {
+#pragma unroll
+        for (uint32_t s = 0; s < kSlicePrefetch; ++s) {
+            const uint64_t tile = base + s * tstride;  // wave-uniform, < tiles
+            add(tile, vr[s], Lr[s]);
+            // refill the slot once its tile is added: its registers are free
+            load(tile + kSlicePrefetch * tstride, vr[s], Lr[s]);
+        }
     }
+#pragma unroll
+    for (uint32_t s = 0; s < kSlicePrefetch; ++s)
+        if (s < rest)
+            add(base + s * tstride, vr[s], Lr[s]);
     __syncthreads();
     const f4 *all = reinterpret_cast<const f4 *>(tab);
     for (uint32_t i = tid; i < k * nf; i += blockDim.x) {
