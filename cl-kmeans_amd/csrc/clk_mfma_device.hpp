@@ -85,10 +85,19 @@ __device__ __forceinline__ void push2(uint32_t &best, uint32_t &second, uint32_t
 }
 
 // Hide known bits of a value from the optimizer (emits no instruction), so
-// (score & mask) | index stays one v_and_or_b32.
-__device__ __forceinline__ uint32_t opaque(uint32_t v)
+// (score & mask) | index stays one v_and_or_b32: opaque_v pins a value to a
+// vector register, opaque_s a wave-uniform value to a scalar register (one
+// scalar operand per VALU instruction, so the mask goes in a VGPR and the
+// index in an SGPR).
+__device__ __forceinline__ uint32_t opaque_v(uint32_t v)
 {
     asm volatile("" : "+v"(v));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t opaque_s(uint32_t v)
+{
+    asm volatile("" : "+s"(v));
     return v;
 }
 
@@ -135,23 +144,36 @@ __device__ __forceinline__ void score_tile(f32v4 (&acc)[kGroups], const TileOps<
             acc[g] = mma16<FP16>(o.a[kb], x.b[g][kb], acc[g]);
 }
 
-// Keys of one scored tile into the running (best, second) of each group.
-__device__ __forceinline__ void rank_tile(const f32v4 (&acc)[kGroups], uint32_t c0, uint32_t kmask,
+// Keys of one scored tile t into the running (best, second) of each group.
+// Inside the ranking a key carries the lane-local index 4t + i of its row
+// (i < 4; the lane's rows are centroids 16t + 4h + i): it is the same for
+// every lane, so it sits in a scalar register and costs no VALU op, and for a
+// fixed h it orders like the centroid index.  rank_all converts the final
+// keys to centroid indices.
+__device__ __forceinline__ void rank_tile(const f32v4 (&acc)[kGroups], uint32_t t, uint32_t vkeep,
                                           uint32_t (&best)[kGroups], uint32_t (&second)[kGroups])
 {
     uint32_t idx[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-        idx[i] = opaque(c0 + i);
+        idx[i] = opaque_s(4 * t + i);
 #pragma unroll
     for (int g = 0; g < kGroups; ++g) {
         uint32_t key[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            key[i] = (__float_as_uint(acc[g][i]) & ~kmask) | idx[i];
+            key[i] = (__float_as_uint(acc[g][i]) & vkeep) | idx[i];
         push2(best[g], second[g], key[0], key[1]);
         push2(best[g], second[g], key[2], key[3]);
     }
+}
+
+// Lane-local key index 4t + i -> centroid index 16t + 4h + i (fits the key's
+// index bits: 16 * ntiles <= 2^key_bits).
+__device__ __forceinline__ uint32_t key_to_centroid(uint32_t key, uint32_t kmask, uint32_t h)
+{
+    const uint32_t l = key & kmask;
+    return (key & ~kmask) | ((l & ~3u) << 2) | (h << 2) | (l & 3u);
 }
 
 // Merge two (best, second) key pairs into the two smallest of the four.
@@ -372,11 +394,10 @@ __device__ __forceinline__ WaveBounds wave_bounds(const float (&X2)[kGroups],
     return w;
 }
 
-// Best / second keys over all centroid tiles, software-pipelined: the MFMAs
-// of tile t run while the keys of tile t-1 are ranked, and the LDS operands
-// of the next tile are already in flight (two accumulator and two operand
-// sets in turn; at the loop head accA holds the scored, unranked tile t-1
-// and o1 the operands of tile t).
+// Best / second keys over all centroid tiles, one tile at a time (the
+// kernels run enough waves per SIMD -- four in the streaming labeling -- that
+// other waves cover the LDS and MFMA latency; software-pipelining two tiles
+// in one wave needed more registers than four waves per SIMD leave).
 template <bool FP16, int NKB>
 __device__ __forceinline__ void rank_all(const u16x8 *lds_table, const float *lds_norm,
                                          uint32_t ntiles, uint32_t lane, uint32_t h, float K,
@@ -386,26 +407,16 @@ __device__ __forceinline__ void rank_all(const u16x8 *lds_table, const float *ld
 #pragma unroll
     for (int g = 0; g < kGroups; ++g)
         best[g] = second[g] = 0xFFFFFFFFu;
-    const uint32_t last = ntiles - 1;
-    f32v4 accA[kGroups], accB[kGroups];
-    TileOps<NKB> o0 = load_ops<NKB>(lds_table, lds_norm, 0, lane, h);
-    TileOps<NKB> o1 = load_ops<NKB>(lds_table, lds_norm, min(1u, last), lane, h);
-    score_tile<FP16, NKB>(accA, o0, K, x);
-    uint32_t t = 1;
-    for (; t + 1 < ntiles; t += 2) {
-        o0 = load_ops<NKB>(lds_table, lds_norm, t + 1, lane, h);
-        score_tile<FP16, NKB>(accB, o1, K, x);
-        rank_tile(accA, (t - 1) * kTile + 4 * h, kmask, best, second);
-        o1 = load_ops<NKB>(lds_table, lds_norm, min(t + 2, last), lane, h);
-        score_tile<FP16, NKB>(accA, o0, K, x);
-        rank_tile(accB, t * kTile + 4 * h, kmask, best, second);
+    const uint32_t vkeep = opaque_v(~kmask);
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        f32v4 acc[kGroups];
+        score_tile<FP16, NKB>(acc, load_ops<NKB>(lds_table, lds_norm, t, lane, h), K, x);
+        rank_tile(acc, t, vkeep, best, second);
     }
-    if (t < ntiles) {
-        score_tile<FP16, NKB>(accB, o1, K, x);
-        rank_tile(accA, (t - 1) * kTile + 4 * h, kmask, best, second);
-        rank_tile(accB, t * kTile + 4 * h, kmask, best, second);
-    } else {
-        rank_tile(accA, (t - 1) * kTile + 4 * h, kmask, best, second);
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g) {
+        best[g] = key_to_centroid(best[g], kmask, h);
+        second[g] = key_to_centroid(second[g], kmask, h);
     }
 }
 
@@ -417,9 +428,10 @@ __device__ __forceinline__ void wave_lds_order()
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
-__device__ __forceinline__ uint64_t lanemask_lt(uint32_t lane)
+// Number of lanes below this one whose bit is set in m (v_mbcnt_lo/hi).
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
 {
-    return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 // Stage the 16-bit operand table and the norms in LDS (whole work-group).
