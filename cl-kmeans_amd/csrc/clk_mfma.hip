@@ -217,7 +217,7 @@ label_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
 #pragma unroll
         for (int g = 0; g < kGroups; ++g) {
             uint32_t b = best[g], s = second[g];
-            merge_pair(b, s, __shfl_xor(b, 16), __shfl_xor(s, 16));
+            merge_pair(b, s, xor_swizzle<16>(b), xor_swizzle<16>(s));
             merge_pair(b, s, __shfl_xor(b, 32), __shfl_xor(s, 32));
             const float vb = __uint_as_float(b & ~kmask), vs = __uint_as_float(s & ~kmask);
             const float E = w.E0[g] + w.beta * w.K;

@@ -61,6 +61,21 @@ __device__ __forceinline__ f32v4 mma16(u16x8 a, u16x8 b, f32v4 c)
                                                        __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
+// Value of lane (lane ^ O) for O < 32, by ds_swizzle (bit mode: and 0x1F,
+// or 0, xor O).  Unlike __shfl_xor it needs no per-lane address register.
+template <int O>
+__device__ __forceinline__ uint32_t xor_swizzle(uint32_t v)
+{
+    static_assert(O > 0 && O < 32, "ds_swizzle bit mode stays within 32 lanes");
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (O << 10));
+}
+
+template <int O>
+__device__ __forceinline__ float xor_swizzle(float v)
+{
+    return __uint_as_float(xor_swizzle<O>(__float_as_uint(v)));
+}
+
 __device__ __forceinline__ uint32_t min3_u32(uint32_t a, uint32_t b, uint32_t c)
 {
     return min(a, min(b, c));  // v_min3_u32
@@ -273,9 +288,9 @@ __device__ __forceinline__ void load_points_core(PointOps<NKB> &x, float (&X2)[k
             }
             x.b[g][kb] = vb;
         }
-        s += __shfl_xor(s, 16);
+        s += xor_swizzle<16>(s);
         s += __shfl_xor(s, 32);
-        r += __shfl_xor(r, 16);
+        r += xor_swizzle<16>(r);
         r += __shfl_xor(r, 32);
         X2[g] = s * (1.0f + (float)(2 * NKB * kKBlock + 8) * kU) + 1e-37f;
         RX2[g] = r * (1.0f + (float)(2 * NKB * kKBlock + 8) * kU) + 1e-37f;
@@ -350,9 +365,9 @@ __device__ __forceinline__ void build_operands(const float (&raw)[kGroups][NKB *
             }
             x.b[g][kb] = vb;
         }
-        s += __shfl_xor(s, 16);
+        s += xor_swizzle<16>(s);
         s += __shfl_xor(s, 32);
-        r += __shfl_xor(r, 16);
+        r += xor_swizzle<16>(r);
         r += __shfl_xor(r, 32);
         X2[g] = s * (1.0f + (float)(2 * NKB * kKBlock + 8) * kU) + 1e-37f;
         RX2[g] = r * (1.0f + (float)(2 * NKB * kKBlock + 8) * kU) + 1e-37f;
@@ -387,8 +402,12 @@ __device__ __forceinline__ WaveBounds wave_bounds(const float (&X2)[kGroups],
         w.E0[g] = 1.01f * (in + gm * (Nx + Pm) + 2.0f * kU * Nx) + absP;
         kneed = fmaxf(kneed, X2[g] + w.E0[g]);  // a NaN point does not poison K
     }
-    for (int o = 1; o < 64; o <<= 1)
-        kneed = fmaxf(kneed, __shfl_xor(kneed, o));
+    kneed = fmaxf(kneed, xor_swizzle<1>(kneed));
+    kneed = fmaxf(kneed, xor_swizzle<2>(kneed));
+    kneed = fmaxf(kneed, xor_swizzle<4>(kneed));
+    kneed = fmaxf(kneed, xor_swizzle<8>(kneed));
+    kneed = fmaxf(kneed, xor_swizzle<16>(kneed));
+    kneed = fmaxf(kneed, __shfl_xor(kneed, 32));
     w.beta = 1.01f * (gm + kU);
     w.K = kneed * (1.0f + 1e-5f) / (1.0f - w.beta) + 1e-30f;
     return w;
@@ -485,7 +504,7 @@ __device__ __forceinline__ Verdict decide(uint32_t best, uint32_t second, const 
 {
     const float tr1 = prm.trunc / (1.0f - prm.trunc), ep1 = prm.eps / (1.0f - prm.eps);
     uint32_t b = best, s = second;
-    merge_pair(b, s, __shfl_xor(b, 16), __shfl_xor(s, 16));
+    merge_pair(b, s, xor_swizzle<16>(b), xor_swizzle<16>(s));
     merge_pair(b, s, __shfl_xor(b, 32), __shfl_xor(s, 32));
     const float vb = __uint_as_float(b & ~kmask), vs = __uint_as_float(s & ~kmask);
     const float E = w.E0[g] + w.beta * w.K;
