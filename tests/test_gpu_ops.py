@@ -193,6 +193,23 @@ def test_centroid_slices_repeated_labels(gpu_ctx, hot):
     assert np.all(np.abs(ds.to_host() - ref) <= sum_tolerance(pts, labels, k, per_item=n))
 
 
+@pytest.mark.parametrize("n,d,k", [
+    (1_500_000, 24, 2000),  # ~34 tiles per wave: full rounds of the 8-tile ring, then a partial one
+    (10_000, 33, 1500),     # fewer tiles than waves: some waves get none
+])
+def test_centroid_slices_prefetch_ring(gpu_ctx, n, d, k):
+    # The feature-slice kernel keeps 8 tiles per wave in flight; the loads
+    # past a wave's last tile are clamped and must never be added.
+    pts = blobs(n, d, 16, seed=d)
+    labels = np.random.default_rng(d).integers(0, k, size=n).astype(np.uint32)
+    s0 = np.full((d, k), -1.5)
+    ds = dev(s0)
+    gpu_ctx.centroid_update(gpu_ctx.config(CU, "cluster_merge"), d, n, k, dev(pts), ds, dev(labels))
+    hip.synchronize()
+    ref = O.sums_f64(pts, labels, k)
+    assert np.all(np.abs(ds.to_host() - s0 - ref) <= sum_tolerance(pts, labels, k, per_item=n))
+
+
 def test_divide_bit_exact_and_nan(gpu_ctx):
     rng = np.random.default_rng(2)
     d, k = 4, 10
