@@ -1271,11 +1271,16 @@ constexpr uint32_t kSmallK = 64;
 
 // Jobs with a small k*d label faster with the FP32 scan than on the matrix
 // cores (the 16-bit path has a fixed cost per point: conversion, bounds,
-// margin test, and d is padded to 32; measured: d = 3 or 8 with k up to 256
-// are faster on the scan); the labels are the same either way.
+// margin test, and d is padded to 32; the scan's keep test skips most of its
+// work from the second iteration on).  Measured on the 4096-job mix (d in
+// {3, 8, 16, 32}, k up to 256), solves/s by the k*d threshold: 1024: 16.6k,
+// 1536: 18.1k, 2048: 19.0k, 3072: 18.9k, 4096: 16.8k, never: 14.6k.  The
+// labels are the same either way.
+constexpr uint64_t kMfmaMinKD = 2048;
+
 bool use_mfma(const clk_batch *b, uint32_t d, uint32_t k)
 {
-    return b->mfma && k >= 16 && (uint64_t)k * d >= 1024;
+    return b->mfma && k >= 16 && (uint64_t)k * d >= kMfmaMinKD;
 }
 
 uint32_t class_of(uint32_t kind, uint32_t d, uint32_t k)

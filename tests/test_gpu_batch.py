@@ -76,12 +76,13 @@ def test_batch_keep_radius_paths(gpu_ctx, stream, labeling):
     # separated blobs (mostly kept), overlapping blobs (many rescans), a
     # duplicated initial centroid (no keep radius for either copy) and an
     # initial centroid far from every point (an empty cluster: NaN centroid);
-    # with mfma_bf16 the k*d >= 1024 jobs take the MFMA classes.
+    # with mfma_bf16 the k*d >= 2048 jobs take the MFMA classes (k <= 64
+    # and k > 64 each).
     b = clkmeans.Batch(gpu_ctx, max_jobs=8, arena_bytes=256 << 20, max_d=32, max_k=256, labeling=labeling)
     jobs = []
     specs = [(40000, 4, 10, 1.0, 50.0), (40000, 3, 40, 2.0, 10.0), (30000, 8, 60, 1.0, 20.0),
-             (20000, 16, 12, 3.0, 10.0), (25000, 32, 20, 1.0, 20.0), (30000, 16, 80, 1.0, 20.0),
-             (30000, 32, 48, 2.0, 10.0)]
+             (20000, 16, 12, 3.0, 10.0), (25000, 32, 20, 1.0, 20.0), (30000, 16, 160, 1.0, 20.0),
+             (30000, 32, 64, 2.0, 10.0)]
     for i, (n, d, k, sigma, box) in enumerate(specs):
         pts, init = make_job(n, d, k, seed=300 + i, sigma=sigma, box=box)
         if i in (2, 5):
