@@ -159,3 +159,18 @@ def test_mfma_refinement_is_the_exception_on_blobs(gpu_ctx, strategy, frac):
     sample = np.arange(0, n, 7)
     np.testing.assert_array_equal(got[sample], O.label(np.ascontiguousarray(pts[:, sample]), cen))
 
+
+
+@pytest.mark.parametrize("strategy", PRECISIONS)
+@pytest.mark.parametrize("n,d,k", [((1 << 20) + 77, 32, 256), ((1 << 19) + 3, 64, 512)])
+def test_mfma_many_batches_per_wave_match_fp32_scan(gpu_ctx, strategy, n, d, k):
+    # Enough points that every wave of the persistent streaming kernel labels
+    # several 64-point batches (the grid holds one work-group of 16 waves per
+    # CU); the labels must equal the exact FP32 scan's (unroll_vector, itself
+    # pinned to the oracle by the other tests) on every point.
+    pts, centres = clkmeans.generate_blobs(n, d, k, sigma=0.5, box=1.0, seed=d + k)
+    cen = np.ascontiguousarray(pts[:, :k])
+    got = mfma_labels(gpu_ctx, strategy, pts, cen)
+    ref = mfma_labels(gpu_ctx, "unroll_vector", pts, cen)
+    bad = np.flatnonzero(got != ref)
+    assert bad.size == 0, "%d labels differ, first at point %d" % (bad.size, bad[0])
