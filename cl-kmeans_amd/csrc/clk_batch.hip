@@ -103,6 +103,10 @@ struct BCtl {
 constexpr uint32_t kBatchMaxWaves = 4;
 constexpr uint32_t kBatchPairs = 256;  // candidate pairs per wave and flush (MFMA refinement)
 
+// Global-memory (address space 1) views for pointers read from job records.
+typedef __attribute__((address_space(1))) const float gfloat;
+typedef __attribute__((address_space(1))) const uint32_t guint;
+
 // Row stride of a wave table: d sums and the count, made odd so that the
 // last lanes of 64 runs (distinct labels) spread over the LDS banks.
 __host__ __device__ inline uint32_t table_stride(uint32_t d) { return (d + 1) | 1u; }
@@ -383,8 +387,16 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
     const uint64_t first = (uint64_t)tindex * J.tile_pts;
     const uint64_t end = std::min<uint64_t>(first + J.tile_pts, J.n);
     const uint64_t n = J.n;
-    // the next pass's points (and previous labels) are loaded while this
-    // pass is labeled
+    // The job's arrays as global-memory pointers, read from the record once
+    // (through the record they would be flat loads, each waiting for every
+    // earlier load).
+    const gfloat *pts = (const gfloat *)J.pts;
+    const guint *prev_labels = (const guint *)J.labels;
+    const gfloat *keep_thr = (const gfloat *)J.keep;
+    // The next pass's points and previous labels are loaded while this pass
+    // is labeled.  Every load is unconditional and in bounds (past the tile:
+    // its first point; past d: feature d-1, zeroed at use), so no branch
+    // separates the loads from their use.
     const bool keep_ok = J.keep_ok != 0;
     float xn[DMAX];
     uint32_t ln = 0;
@@ -393,10 +405,8 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
         const uint64_t pc = p < end ? p : first;
 #pragma unroll
         for (int f = 0; f < DMAX; ++f)
-            if (f < (int)d)
-                xn[f] = J.pts[(uint64_t)f * n + pc];
-        if (keep_ok)
-            ln = J.labels[pc];
+            xn[f] = pts[(uint64_t)min((uint32_t)f, d - 1) * n + pc];
+        ln = prev_labels[pc];
     };
     fetch(first);
     for (uint64_t base = first; base < end; base += blockDim.x) {
@@ -407,14 +417,16 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
         for (int f = 0; f < DMAX; ++f)
             x[f][0] = (f < (int)d) ? xn[f] : 0.0f;
         const uint32_t prev = ln;
-        if (base + blockDim.x < end)
-            fetch(base + blockDim.x);
+        const uint32_t a = prev < k ? prev : 0u;
+        // the keep threshold is loaded before the next pass's points, so
+        // waiting for it does not wait for them
+        const float thr = keep_ok ? keep_thr[a] : 0.0f;
+        fetch(base + blockDim.x);
         // a point inside its previous centroid's keep radius keeps the label
         // (the scan would return it, keep_threshold); the others scan
         uint32_t lab[1] = {0u};
         bool keep = false;
         if (keep_ok) {
-            const uint32_t a = prev < k ? prev : 0u;
             const float *ca = cen_s + (size_t)a * DMAX;
             float t = x[0][0] - ca[0];
             float dist = t * t;
@@ -423,7 +435,7 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
                 t = x[f][0] - ca[f];
                 dist = __builtin_fmaf(t, t, dist);
             }
-            keep = prev < k && dist <= J.keep[a];
+            keep = prev < k && dist <= thr;
             lab[0] = a;
         }
         const unsigned long long fail = __ballot(valid && !keep);
