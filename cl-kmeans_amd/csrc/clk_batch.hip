@@ -106,6 +106,7 @@ constexpr uint32_t kBatchPairs = 256;  // candidate pairs per wave and flush (MF
 // Global-memory (address space 1) views for pointers read from job records.
 typedef __attribute__((address_space(1))) const float gfloat;
 typedef __attribute__((address_space(1))) const uint32_t guint;
+typedef __attribute__((address_space(1))) const mfma::f32v4 gf32v4;
 
 // Row stride of a wave table: d sums and the count, made odd so that the
 // last lanes of 64 runs (distinct labels) spread over the LDS banks.
@@ -766,7 +767,11 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
     // those exactly from the rows, others take the MFMA path
     const bool keep_ok = J.keep_ok != 0;
     const uint32_t w4 = J.cen_t_w / 4;
-    const f32v4 *rows = reinterpret_cast<const f32v4 *>(J.cen_t);
+    // global-memory views of the job's arrays (global, not flat, loads)
+    const gf32v4 *rows = (const gf32v4 *)J.cen_t;
+    const gfloat *gpts = (const gfloat *)J.pts;
+    const guint *glabels = (const guint *)J.labels;
+    const gfloat *gkeep = (const gfloat *)J.keep;
 
     for (uint32_t e = tid; e < nt16 * NKB * 64; e += blockDim.x)
         ltab[e] = J.table[e];
@@ -823,16 +828,18 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         const bool vme = pme < end;
         uint32_t prev = 0;
         {
+            // unconditional in-bounds loads (past d: feature d-1, zeroed)
             const uint64_t pc = vme ? pme : first;
 #pragma unroll
-            for (int f = 0; f < (int)P; ++f)
-                if (f < (int)d)
-                    xc[f] = J.pts[(uint64_t)f * n + pc];
-            if (keep_ok)
-                prev = J.labels[pc];
+            for (int f = 0; f < (int)P; ++f) {
+                const float v = gpts[(uint64_t)min((uint32_t)f, d - 1) * n + pc];
+                xc[f] = f < (int)d ? v : 0.0f;
+            }
+            prev = glabels[pc];
         }
-        if (bt + waves < nb)
-            fetch(first + (bt + waves) * 64);
+        // The keep test runs before the next batch's loads are issued: loads
+        // complete in order, so a load issued after them (the centroid row,
+        // the threshold) would wait for them.
         bool keep = false;
         if (keep_ok) {
             const uint32_t a = prev < k ? prev : 0u;
@@ -848,8 +855,9 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
                             dist = __builtin_fmaf(t, t, dist);
                         }
                 }
-            keep = prev < k && dist <= J.keep[a];
+            keep = prev < k && dist <= gkeep[a];
         }
+        fetch(first + (bt + waves) * 64);  // past the tile: every point missing, zeros read
         const unsigned long long fail = keep_ok ? __ballot(vme && !keep) : ~0ull;
         const uint32_t nfail = (uint32_t)__builtin_popcountll(fail);
         uint32_t mine;

@@ -325,6 +325,9 @@ __device__ __forceinline__ void load_points_keep(PointOps<NKB> &x, float (&X2)[k
 // = feature kb*32 + 8h + j of point pid[g], already 0 for a missing point or
 // feature (read from the zero word *zero instead: the address select is paid
 // anyway, and build_operands then needs no masks).
+// A float in global memory (address space 1).
+typedef __attribute__((address_space(1))) const float gfloat1;
+
 template <int NKB>
 __device__ __forceinline__ void fetch_points(float (&raw)[kGroups][NKB * 8], const float *__restrict__ pts,
                                              uint64_t n, uint32_t d, const uint64_t (&pid)[kGroups],
@@ -339,7 +342,10 @@ __device__ __forceinline__ void fetch_points(float (&raw)[kGroups][NKB * 8], con
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const bool ok = valid && (uint32_t)(kb * kKBlock + 8 * h + j) < d;
-                raw[g][kb * 8 + j] = *(ok ? row + (uint64_t)(kb * kKBlock + j) * n : zero);
+                // both candidates are global memory: a global load (a flat
+                // load would also count against the LDS wait counter, so
+                // every LDS wait of the caller would wait for it)
+                raw[g][kb * 8 + j] = *(const gfloat1 *)(ok ? row + (uint64_t)(kb * kKBlock + j) * n : zero);
             }
     }
 }
@@ -525,14 +531,15 @@ __device__ __forceinline__ float exact_dist_global(const float *__restrict__ pts
                                                    uint64_t p, const float *__restrict__ cen,
                                                    uint32_t d, uint32_t k, uint32_t c)
 {
+    const gfloat1 *gp = (const gfloat1 *)pts, *gc = (const gfloat1 *)cen;  // global loads
     float dist = 0.0f;
     for (uint32_t f0 = 0; f0 < d; f0 += 8) {
         float xv[8], cv[8];
 #pragma unroll
         for (uint32_t j = 0; j < 8; ++j) {
             const uint32_t f = f0 + j < d ? f0 + j : 0;
-            xv[j] = pts[(uint64_t)f * n + p];
-            cv[j] = cen[(uint64_t)f * k + c];
+            xv[j] = gp[(uint64_t)f * n + p];
+            cv[j] = gc[(uint64_t)f * k + c];
         }
 #pragma unroll
         for (uint32_t j = 0; j < 8; ++j)
