@@ -106,6 +106,7 @@ constexpr uint32_t kBatchPairs = 256;  // candidate pairs per wave and flush (MF
 // Global-memory (address space 1) views for pointers read from job records.
 typedef __attribute__((address_space(1))) const float gfloat;
 typedef __attribute__((address_space(1))) const uint32_t guint;
+typedef __attribute__((address_space(1))) const double gdouble;
 typedef __attribute__((address_space(1))) const mfma::f32v4 gf32v4;
 
 // Row stride of a wave table: d sums and the count, made odd so that the
@@ -504,7 +505,7 @@ __device__ void merge_rows(const BJob &J, uint32_t e0)
     const bool mine = e < rl && e < e0 + per;
     double acc = 0.0;
     if (mine) {
-        const double *col = J.part + e;
+        const gdouble *col = (const gdouble *)(J.part + e);  // global, not flat, loads
         uint32_t t = s;
         for (; t + 3 * S < nt; t += 4 * S) {
             const double a0 = col[(size_t)t * rl], a1 = col[(size_t)(t + S) * rl];
