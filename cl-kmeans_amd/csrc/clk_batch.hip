@@ -714,8 +714,10 @@ __device__ __forceinline__ void for_each_tile(const BJob *__restrict__ jobs, con
 }
 
 // Tiles of the jobs of one size class (FP32 labeling, d <= DMAX).
+// d <= 32: at most 168 VGPRs, so three work-groups fit per CU (the
+// prefetched and the current point's 32 features are both live).
 template <int DMAX>
-__global__ void __launch_bounds__(64 * kBatchMaxWaves)
+__global__ void __launch_bounds__(64 * kBatchMaxWaves, DMAX == 32 ? 3 : 1)
 batch_tiles_kernel(const BJob *__restrict__ jobs, const BTile *__restrict__ tiles, BCtl *__restrict__ ctl)
 {
     extern __shared__ __align__(16) float lds[];
