@@ -129,6 +129,10 @@ def main():
     mcfg = ctx.config(clkmeans.STAGE_MASS_UPDATE, "part_private")
     ccfg = ctx.config(clkmeans.STAGE_CENTROID_UPDATE, "cluster_merge")
     s = stream.handle
+    # The sum grid of the data set, pinned once (setup, like the upload; the
+    # strategies do the same): every iteration's sums are exact on it.
+    ctx.sum_grid_from_points(d, n, dp, stream=s)
+    stream.synchronize()
 
     def iteration(fcfg):
         old, new = cen[0], cen[1]
@@ -290,6 +294,8 @@ def dense(args):
     lcfg = ctx.config(clkmeans.STAGE_LABELING, args.labeling)
     mcfg = ctx.config(clkmeans.STAGE_MASS_UPDATE, "part_local")
     ccfg = ctx.config(clkmeans.STAGE_CENTROID_UPDATE, "cluster_merge")
+    ctx.sum_grid_from_points(d, n, dp, stream=s)  # pinned once (setup)
+    stream.synchronize()
 
     def iteration(lab_cfg):
         old, new = cen[0], cen[1]

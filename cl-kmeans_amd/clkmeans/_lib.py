@@ -152,6 +152,10 @@ def load(path=None):
         "clk_matrix_divide_f64": (i32, [vp, u64, u64, vp, vp, vp, vp, vp, vp]),
         "clk_matrix_add_u64": (i32, [vp, u64, vp, vp, vp]),
         "clk_init_first_x": (i32, [vp, u64, u64, u64, vp, vp, vp]),
+        "clk_sum_grid_from_points": (i32, [vp, u64, u64, vp, vp]),
+        "clk_sum_grid_from_max": (i32, [vp, u64, u64, vp]),
+        "clk_sum_grid_clear": (i32, [vp]),
+        "clk_sum_grid_units": (i32, [vp, u64, u64, vp, vp, vp]),
         "clk_generate_blobs": (i32, [u64, u64, u64, dp, dp, u64, vp, vp, i32]),
         "clk_kmeans_run": (i32, [ctypes.c_char_p, i32, u64, u64, u64, vp, vp, vp, vp, u64, dp,
                                  ctypes.POINTER(u64), ctypes.POINTER(u64)]),
@@ -360,6 +364,24 @@ class Context:
 
     def first_x(self, d, n, k, points, centroids, stream=None):
         check(self.L.clk_init_first_x(self.handle, d, n, k, _p(points), _p(centroids), _s(stream)))
+
+    # -- the sum grid (exact sums, clkmeans.h) -------------------------------
+    def sum_grid_from_points(self, d, n, points, stream=None):
+        """Pin the grid of device points [d][n] for this context's later ops."""
+        check(self.L.clk_sum_grid_from_points(self.handle, d, n, _p(points), _s(stream)))
+
+    def sum_grid_from_max(self, d, n, max_abs):
+        m = np.ascontiguousarray(max_abs, dtype=np.float32)
+        check(self.L.clk_sum_grid_from_max(self.handle, d, n, m.ctypes.data))
+
+    def sum_grid_clear(self):
+        check(self.L.clk_sum_grid_clear(self.handle))
+
+    def sum_grid_units(self, d, n, points, stream=None):
+        """The per-feature units an op would use for (d, n, points)."""
+        out = np.empty(d, np.float64)
+        check(self.L.clk_sum_grid_units(self.handle, d, n, _p(points), out.ctypes.data, _s(stream)))
+        return out
 
     # -- timing records ------------------------------------------------------
     def profiling(self, enable=True):

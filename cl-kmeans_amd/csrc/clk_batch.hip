@@ -26,16 +26,18 @@
 // centroids).  The finish kernel also builds the job's table for the next
 // iteration.
 //
-// Accumulation without float atomics: each wave owns an LDS table T[k][ts]
-// (FP32; columns 0..d-1 the feature sums, column d the point count, ts the
-// odd row stride d+1 or d+2).  For every batch of 64 points the wave sorts
-// its (label, lane) keys (a bitonic network over the lanes), so the points of
-// one label form a run of lanes; a segmented scan along the sorted lanes
-// (DPP row shifts and row broadcasts) leaves each run's sum on its last lane,
-// which adds it to T[label] -- one read-modify-write per distinct label, in
-// an order fixed by the batch's labels alone.  The tile's partial row sums
-// the waves' tables in wave order.
+// Exact accumulation (clk_grid.hpp): every coordinate is rounded to its grid
+// integer (a per-job, per-feature power-of-two unit).  For every batch of 64
+// points a wave sorts its (label, lane) keys (a bitonic network over the
+// lanes), so the points of one label form a run of lanes; an integer
+// segmented scan along the sorted lanes (DPP row shifts and row broadcasts)
+// leaves each run's sum on its last lane (|sum| < 64 * 2^25 fits an int32),
+// which adds it to the work-group's int64 accumulator A[d][k] with one
+// ds_add_u64 per feature -- one per distinct label.  Integer addition is
+// exact, so a tile's row, and a job's merged sums, are the same for every
+// tile size, wave count and job mix, and equal those of clk_lloyd_step.
 #include "clk_common.hpp"
+#include "clk_grid.hpp"
 #include "clk_kernels.hpp"
 #include "clk_mfma_device.hpp"
 #include "clkmeans_batch.h"
@@ -87,6 +89,11 @@ struct BJob {
     // the keep test and the exact rescan of the tile kernel
     float *cen_t;
     uint32_t cen_t_w, pad2;
+    // exact sums (clk_grid.hpp): the per-feature max |x| bits of the job's
+    // points (its grid), and per cell [d][k] the flags of off-grid values
+    // (set by the tiles, consumed and cleared by the finish kernel)
+    uint32_t *gmax;
+    uint32_t *flags;
 };
 
 struct BTile {
@@ -100,7 +107,8 @@ struct BCtl {
     uint32_t pad;
 };
 
-constexpr uint32_t kBatchMaxWaves = 4;
+constexpr uint32_t kBatchMaxWaves = 4;      // FP32-scan tile kernels
+constexpr uint32_t kBatchMfmaWaves = 8;      // MFMA tile kernels with one K-block (two waves per SIMD)
 constexpr uint32_t kBatchPairs = 256;  // candidate pairs per wave and flush (MFMA refinement)
 
 // Global-memory (address space 1) views for pointers read from job records.
@@ -109,33 +117,59 @@ typedef __attribute__((address_space(1))) const uint32_t guint;
 typedef __attribute__((address_space(1))) const double gdouble;
 typedef __attribute__((address_space(1))) const mfma::f32v4 gf32v4;
 
-// Row stride of a wave table: d sums and the count, made odd so that the
-// last lanes of 64 runs (distinct labels) spread over the LDS banks.
-__host__ __device__ inline uint32_t table_stride(uint32_t d) { return (d + 1) | 1u; }
-
-// LDS of the MFMA batch kernel for the largest admitted job (byte offsets):
-// table | norms | the accumulation tables [kmax][ts(dmax)] (one per wave, or
-// one for the work-group) | W pair lists | W running minima [64] | W pair
-// counters
-struct BLayout {
-    uint32_t kmax, waves;
-    uint32_t priv;  // 1: one table per wave (no phases); 0: one per work-group
-    uint32_t o_norm, o_tabs, o_pairs, o_min, o_count, bytes;
+// The exact accumulator of a tile in LDS (byte offsets from its start):
+// A [d][k] int64 sums of grid integers | counts [k] | grid scales [smax].
+struct BAcc {
+    long long *A;
+    uint32_t *cnt;
+    float *scl;
 };
 
-inline BLayout mfma_layout_for(uint32_t waves, uint32_t kmax, uint32_t dmax_feat, bool priv)
+__host__ __device__ inline size_t acc_bytes(uint32_t k, uint32_t kd, uint32_t smax)
+{
+    return (size_t)kd * 8 + (size_t)((k + 3) & ~3u) * 4 + (size_t)((smax + 3) & ~3u) * 4;
+}
+
+// Carve the accumulator of a job (d, k) at p (8-byte aligned), zero it and
+// stage the job's grid scales (callers add a barrier).
+__device__ inline BAcc acc_carve(char *p, const BJob &J, uint32_t smax)
+{
+    BAcc a;
+    const uint32_t d = J.d, k = J.k;
+    a.A = reinterpret_cast<long long *>(p);
+    a.cnt = reinterpret_cast<uint32_t *>(a.A + (size_t)d * k);
+    a.scl = reinterpret_cast<float *>(a.cnt + ((k + 3) & ~3u));
+    const uint32_t bits = grid_bits(J.n);
+    for (uint32_t i = threadIdx.x; i < d * k; i += blockDim.x)
+        a.A[i] = 0;
+    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x)
+        a.cnt[i] = 0u;
+    for (uint32_t f = threadIdx.x; f < smax; f += blockDim.x)
+        a.scl[f] = f < d ? grid_scale(J.gmax[f], bits) : 1.0f;
+    return a;
+}
+
+// LDS of the MFMA batch kernel for the jobs of a class (byte offsets):
+// table | norms | the exact accumulator (largest k, largest k*d) | W pair
+// lists | W running minima [64] | W pair counters
+struct BLayout {
+    uint32_t kmax, waves;
+    uint32_t o_norm, o_acc, o_pairs, o_min, o_count, bytes;
+};
+
+inline BLayout mfma_layout_for(uint32_t waves, uint32_t kmax, uint32_t kdmax, uint32_t dmax_feat)
 {
     BLayout L{};
     const uint32_t nt16 = (kmax + mfma::kTile - 1) / mfma::kTile;
     const uint32_t nkb = (dmax_feat + mfma::kKBlock - 1) / mfma::kKBlock;
     L.kmax = kmax;
     L.waves = waves;
-    L.priv = priv ? 1u : 0u;
     size_t o = (size_t)nt16 * nkb * 64 * 16;
     L.o_norm = (uint32_t)o;
     o += (size_t)nt16 * mfma::kTile * 4;
-    L.o_tabs = (uint32_t)o;
-    o += (size_t)(priv ? waves : 1) * kmax * table_stride(dmax_feat) * 4;
+    L.o_acc = (uint32_t)o;  // 16-byte aligned
+    o += acc_bytes(kmax, kdmax, nkb * mfma::kKBlock);
+    o = (o + 15) / 16 * 16;
     L.o_pairs = (uint32_t)o;
     o += (size_t)waves * kBatchPairs * 4;
     o = (o + 15) / 16 * 16;
@@ -147,11 +181,11 @@ inline BLayout mfma_layout_for(uint32_t waves, uint32_t kmax, uint32_t dmax_feat
     return L;
 }
 
-// LDS of the plain kernel: centroids [k][DMAX] | W tables [k][ts(dmax)] |
-// W rescan lists [64]
-__host__ __device__ inline size_t batch_lds_bytes(uint32_t waves, uint32_t k, uint32_t dm, uint32_t dmax_feat)
+// LDS of the plain kernel: the exact accumulator (largest k, largest k*d) |
+// centroids [k][DMAX] | W rescan lists [64]
+__host__ __device__ inline size_t batch_lds_bytes(uint32_t waves, uint32_t k, uint32_t kd, uint32_t dm)
 {
-    return (size_t)k * dm * 4 + (size_t)waves * k * table_stride(dmax_feat) * 4 + (size_t)waves * 64 * 4;
+    return acc_bytes(k, kd, dm) + (size_t)k * dm * 4 + (size_t)waves * 64 * 4;
 }
 
 // Labels of the lanes in `fail` (at most 16 of the wave's lanes: points that
@@ -324,66 +358,57 @@ __device__ __forceinline__ WaveRuns wave_runs(uint32_t label, bool valid, uint32
 }
 
 // Inclusive scan of v along the sorted lanes within runs: the last lane of a
-// run ends with the run's sum (a fixed summation tree).
+// run ends with the run's sum (exact: at most 64 grid integers, |sum| < 2^31).
 // Steps that no run needs are skipped (wave-uniform branches): a step adds
 // only within a run, and a run of length L is complete after the in-row
 // shifts below L plus, if it crosses a row boundary, the broadcasts.
-__device__ __forceinline__ float run_sum(float v, const WaveRuns &r)
+__device__ __forceinline__ int32_t run_sum(int32_t v, const WaveRuns &r)
 {
-    float t;
+    int32_t t;
     if (r.steps & 1u) {
-        t = __uint_as_float(dpp_move<kDppRowShr + 1, 0xF>(__float_as_uint(v)));
+        t = (int32_t)dpp_move<kDppRowShr + 1, 0xF>((uint32_t)v);
         v = r.step[0] ? v + t : v;
     }
     if (r.steps & 2u) {
-        t = __uint_as_float(dpp_move<kDppRowShr + 2, 0xF>(__float_as_uint(v)));
+        t = (int32_t)dpp_move<kDppRowShr + 2, 0xF>((uint32_t)v);
         v = r.step[1] ? v + t : v;
     }
     if (r.steps & 4u) {
-        t = __uint_as_float(dpp_move<kDppRowShr + 4, 0xF>(__float_as_uint(v)));
+        t = (int32_t)dpp_move<kDppRowShr + 4, 0xF>((uint32_t)v);
         v = r.step[2] ? v + t : v;
     }
     if (r.steps & 8u) {
-        t = __uint_as_float(dpp_move<kDppRowShr + 8, 0xF>(__float_as_uint(v)));
+        t = (int32_t)dpp_move<kDppRowShr + 8, 0xF>((uint32_t)v);
         v = r.step[3] ? v + t : v;
     }
     if (r.steps & 16u) {
-        t = __uint_as_float(dpp_move<kDppBcast15, 0xA>(__float_as_uint(v)));
+        t = (int32_t)dpp_move<kDppBcast15, 0xA>((uint32_t)v);
         v = r.step[4] ? v + t : v;
-        t = __uint_as_float(dpp_move<kDppBcast31, 0xC>(__float_as_uint(v)));
+        t = (int32_t)dpp_move<kDppBcast31, 0xC>((uint32_t)v);
         v = r.step[5] ? v + t : v;
     }
     return v;
 }
 
-// Partial row of a tile: the waves' tables summed in wave order (FP64).
-__device__ __forceinline__ void write_partial_row(const float *tabs, uint32_t waves, uint32_t kstride,
-                                                  uint32_t d, uint32_t k, double *prow)
+// Partial row of a tile from the exact accumulator: sums [d][k] (grid
+// integers times the unit, exact in FP64), then the counts [k].
+__device__ __forceinline__ void write_partial_row(const BAcc &a, uint32_t d, uint32_t k, double *prow)
 {
-    const uint32_t ts = table_stride(d);
-    for (uint32_t i = threadIdx.x; i < k * (d + 1); i += blockDim.x) {
-        // sums [d][k] then masses [k]: element i is (feature f, cluster c)
-        const uint32_t f = i / k, c = i % k;
-        double s = 0.0;
-        for (uint32_t w = 0; w < waves; ++w)
-            s += (double)tabs[((size_t)w * kstride + c) * ts + f];
-        prow[i] = s;
-    }
+    for (uint32_t i = threadIdx.x; i < k * (d + 1); i += blockDim.x)
+        prow[i] = i < d * k ? (double)a.A[i] * grid_unit(a.scl[i / k]) : (double)a.cnt[i - d * k];
 }
 
 template <int DMAX>
 __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float *lds, uint32_t waves)
 {
+    (void)waves;
     const uint32_t tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-    const uint32_t d = J.d, k = J.k, ts = table_stride(d);
-    float *cen_s = lds;
-    float *tabs = cen_s + (size_t)k * DMAX;
-    float *T = tabs + (size_t)wave * k * ts;
-    uint32_t *slot = reinterpret_cast<uint32_t *>(tabs + (size_t)waves * k * ts) + wave * 64;
+    const uint32_t d = J.d, k = J.k;
+    const BAcc acc = acc_carve(reinterpret_cast<char *>(lds), J, DMAX);
+    float *cen_s = reinterpret_cast<float *>(reinterpret_cast<char *>(lds) + acc_bytes(k, k * d, DMAX));
+    uint32_t *slot = reinterpret_cast<uint32_t *>(cen_s + (size_t)k * DMAX) + wave * 64;
 
     stage_centroids<DMAX>(J.cen[J.iter & 1], d, k, cen_s);
-    for (uint32_t i = tid; i < waves * k * ts; i += blockDim.x)
-        tabs[i] = 0.0f;
     __syncthreads();
 
     const uint64_t first = (uint64_t)tindex * J.tile_pts;
@@ -454,21 +479,35 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
             if (valid && !keep)
                 J.labels[p] = lab[0];
         }
+        // grid integers summed along the runs; each run's last lane adds
+        // its sums to the work-group's accumulator
         const WaveRuns r = wave_runs(lab[0], valid, lane);
-        float *row = T + (size_t)(r.last ? r.label : 0u) * ts;
+        bool off = false;
 #pragma unroll
         for (int f = 0; f < DMAX; ++f)
             if (f < (int)d) {
-                const float v = run_sum(__shfl(x[f][0], (int)r.src), r);
+                bool ok;
+                const int32_t q = grid_q(x[f][0], acc.scl[f], ok);
+                off = off || (valid && !ok);
+                const int32_t v = run_sum((int32_t)__shfl((int)q, (int)r.src), r);
                 if (r.last)
-                    row[f] += v;
+                    lds_add64(acc.A + (uint32_t)f * k + r.label, (long long)v);
             }
         if (r.last)
-            row[d] += (float)r.count;
+            lds_add(acc.cnt + r.label, r.count);
+        if (__builtin_expect(__ballot(off) != 0, 0)) {
+            // off-grid values (added as 0): flag their cells for the finish
+            if (off) {
+#pragma unroll
+                for (int f = 0; f < DMAX; ++f)
+                    if (f < (int)d && !(__builtin_fabsf(x[f][0] * acc.scl[f]) < kGridLimit))
+                        atomicOr(&J.flags[(size_t)f * k + lab[0]], special_flag(x[f][0]));
+            }
+        }
         mfma::wave_lds_order();
     }
     __syncthreads();
-    write_partial_row(tabs, waves, k, d, k, J.part + (size_t)tindex * ((size_t)k * (d + 1)));
+    write_partial_row(acc, d, k, J.part + (size_t)tindex * ((size_t)k * (d + 1)));
 }
 
 // Lanes per element in the merge of a job's tile rows: a power of two <= 64
@@ -653,7 +692,13 @@ __device__ void batch_finalize(BJob &J, double *red)
         const double m = tot[(size_t)k * d + c];
         double sq = 0.0;
         for (uint32_t f = 0; f < d; ++f) {
-            const float v = (float)(tot[(size_t)f * k + c] / m);
+            // exact sums, or NaN / inf where off-grid values were flagged
+            // (the flags are cleared for the next iteration)
+            const size_t i = (size_t)f * k + c;
+            const uint32_t fl = J.flags[i];
+            if (fl)
+                J.flags[i] = 0u;
+            const float v = (float)((fl ? special_value(fl) : tot[i]) / m);
             nxt[(size_t)f * k + c] = v;
             const double diff = (double)v - (double)old[(size_t)f * k + c];
             sq += diff * diff;
@@ -731,19 +776,6 @@ batch_tiles_kernel(const BJob *__restrict__ jobs, const BTile *__restrict__ tile
 // Read in place of a missing point or feature (the loads need no masks).
 __device__ const float kZeroWord = 0.0f;
 
-// Columns S, S+4, S+8, ... of a table row: feature sums v[f] for f < d and
-// the count in column d.
-template <int P, int S>
-__device__ __forceinline__ void add_columns(float *row, const float (&v)[P], uint32_t d, uint32_t count)
-{
-#pragma unroll
-    for (int f = S; f < P; f += 4)
-        if (f < (int)d)
-            row[f] += v[f];
-    if (d % 4 == S)
-        row[d] += (float)count;
-}
-
 template <bool FP16, int NKB>
 __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, char *lds, const BLayout &Lo)
 {
@@ -756,10 +788,7 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
     const uint64_t n = J.n;
     u16x8 *ltab = reinterpret_cast<u16x8 *>(lds);
     float *lnorm = reinterpret_cast<float *>(lds + Lo.o_norm);
-    const uint32_t ts = table_stride(d);
-    const bool priv = Lo.priv != 0;
-    float *tabs = reinterpret_cast<float *>(lds + Lo.o_tabs);
-    float *T = priv ? tabs + (size_t)wave * k * ts : tabs;  // this wave's or the work-group's table
+    const BAcc acc = acc_carve(lds + Lo.o_acc, J, P);
     uint32_t *pl = reinterpret_cast<uint32_t *>(lds + Lo.o_pairs) + (size_t)wave * kBatchPairs;
     unsigned long long *wmin = reinterpret_cast<unsigned long long *>(lds + Lo.o_min) + (size_t)wave * 64;
     uint32_t *pc = reinterpret_cast<uint32_t *>(lds + Lo.o_count) + wave;
@@ -780,8 +809,6 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         ltab[e] = J.table[e];
     for (uint32_t c = tid; c < nt16 * kTile; c += blockDim.x)
         lnorm[c] = J.norm[c];
-    for (uint32_t i = tid; i < (priv ? waves : 1u) * k * ts; i += blockDim.x)
-        tabs[i] = 0.0f;
     __syncthreads();
     const bool table_ok = J.stats[2] == 0.0f;
     const uint32_t kmask = (1u << J.prm.key_bits) - 1u;
@@ -806,13 +833,9 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
     };
     if (wave < nb)
         fetch(first + (uint64_t)wave * 64);
-    // Rounds: in round rd wave w labels batch rd*W + w.  With one table per
-    // wave the run sums go straight to it; with one table per work-group the
-    // waves add them in W phases, wave w owning column slice (w + phase) mod
-    // W in each phase (a fixed order per column; with 4 waves slice s holds
-    // the columns s, s+4, s+8, ...).
+    // Rounds: in round rd wave w labels batch rd*W + w; the run sums go to
+    // the work-group's accumulator (exact integer atomics, any order).
     const uint32_t rounds = (uint32_t)((nb + waves - 1) / waves);
-    const uint32_t cols = d + 1, fs = (cols + waves - 1) / waves;
     for (uint32_t rd = 0; rd < rounds; ++rd) {
         const uint64_t bt = (uint64_t)rd * waves + wave;
         WaveRuns r;
@@ -1003,56 +1026,48 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         if (mp < n)
             J.labels[mp] = mine;
         }
-        // lane l holds point base + l (xc); sorted by label, the points are
-        // summed along their runs (the sums replace xc)
+        // lane l holds point base + l (xc): its coordinates become grid
+        // integers (kept in xc's registers), which are summed along the runs
+        // of the points sorted by label
         r = wave_runs(mine, mp < n, lane);
+        bool off = false;
+#pragma unroll
+        for (int f = 0; f < (int)P; ++f)
+            if (f < (int)d) {
+                bool ok;
+                xc[f] = __int_as_float(grid_q(xc[f], acc.scl[f], ok));
+                off = off || (mp < n && !ok);
+            }
+        if (__builtin_expect(__ballot(off) != 0, 0)) {
+            // off-grid values (added as 0): flag their cells for the finish
+            if (off)
+                for (uint32_t f = 0; f < d; ++f) {
+                    const float v = gpts[(uint64_t)f * n + mp];
+                    if (!(__builtin_fabsf(v * acc.scl[f]) < kGridLimit))
+                        atomicOr(&J.flags[(size_t)f * k + mine], special_flag(v));
+                }
+        }
 #pragma unroll
         for (int f = 0; f < (int)P; ++f)
             if (f < (int)d)
-                xc[f] = run_sum(__shfl(xc[f], (int)r.src), r);
+                xc[f] = __int_as_float(run_sum((int32_t)__shfl(__float_as_int(xc[f]), (int)r.src), r));
         }
-        float *row = T + (size_t)r.label * ts;
-        if (priv) {  // the wave's own table: no other wave touches it
-            if (r.last) {
+        if (r.last) {
 #pragma unroll
-                for (int f = 0; f < (int)P; ++f)
-                    if (f < (int)d)
-                        row[f] += xc[f];
-                row[d] += (float)r.count;
-            }
-            wave_lds_order();
-            continue;
+            for (int f = 0; f < (int)P; ++f)
+                if (f < (int)d)
+                    lds_add64(acc.A + (uint32_t)f * k + r.label, (long long)__float_as_int(xc[f]));
+            lds_add(acc.cnt + r.label, r.count);
         }
-        for (uint32_t ph = 0; ph < waves; ++ph) {
-            __syncthreads();
-            const uint32_t sl = (wave + ph) % waves;
-            if (!r.last)
-                continue;
-            if (waves == kBatchMaxWaves) {  // slice sl: columns sl, sl+4, ...
-                switch (sl) {
-                case 0: add_columns<P, 0>(row, xc, d, r.count); break;
-                case 1: add_columns<P, 1>(row, xc, d, r.count); break;
-                case 2: add_columns<P, 2>(row, xc, d, r.count); break;
-                default: add_columns<P, 3>(row, xc, d, r.count); break;
-                }
-            } else {  // contiguous slices of fs columns
-                const uint32_t lo = sl * fs, hi = min(cols, lo + fs);
-#pragma unroll
-                for (int f = 0; f < (int)P; ++f)
-                    if (f < (int)d && (uint32_t)f >= lo && (uint32_t)f < hi)
-                        row[f] += xc[f];
-                if (d >= lo && d < hi)
-                    row[d] += (float)r.count;
-            }
-        }
+        wave_lds_order();
     }
     __syncthreads();
-    write_partial_row(tabs, priv ? waves : 1u, k, d, k, J.part + (size_t)tindex * ((size_t)k * (d + 1)));
+    write_partial_row(acc, d, k, J.part + (size_t)tindex * ((size_t)k * (d + 1)));
 }
 
 // Tiles of the jobs of one size class (MFMA labeling, NKB K-blocks).
 template <bool FP16, int NKB>
-__global__ void __launch_bounds__(64 * kBatchMaxWaves)
+__global__ void __launch_bounds__(NKB == 1 ? 64 * kBatchMfmaWaves : 64 * kBatchMaxWaves)
 batch_tiles_mfma_kernel(const BJob *__restrict__ jobs, const BTile *__restrict__ tiles, BCtl *__restrict__ ctl,
                         BLayout Lo)
 {
@@ -1223,8 +1238,6 @@ struct clk_batch {
     BClassState cls[clk::kClasses];
     BJob *h_rec = nullptr;  // pinned staging of job records for activation
     BJob *h_rec_dev = nullptr;  // its device address
-    uint32_t class_waves[clk::kClasses] = {};  // fixed at creation (they fix the summation order)
-    bool class_priv[clk::kClasses] = {};
 };
 
 namespace {
@@ -1244,6 +1257,8 @@ uint64_t job_bytes(uint64_t n, uint32_t d, uint32_t k, uint32_t tile_pts, bool p
     b += clk::align_up((uint64_t)k * 4, 256);             // keep thresholds
     b += clk::align_up((uint64_t)k * clk::keep_row_width(d) * 4, 256);  // centroid rows
     b += clk::align_up(nt * (uint64_t)k * (d + 1) * 8, 256);  // partial rows
+    b += clk::align_up((uint64_t)d * 4, 256);                  // grid (max |x| bits)
+    b += clk::align_up((uint64_t)d * k * 4, 256);              // off-grid flags
     return b;
 }
 
@@ -1291,6 +1306,10 @@ bool arena_alloc(clk_batch *b, uint64_t need, uint64_t *where)
 }
 
 constexpr uint32_t kSmallK = 64;
+
+// Host-side state of a slot whose copies are enqueued but which the kernels
+// do not see yet (serving).
+constexpr uint32_t kJobStaged = 3;
 
 // Jobs with a small k*d label faster with the FP32 scan than on the matrix
 // cores (the 16-bit path has a fixed cost per point: conversion, bounds,
@@ -1340,31 +1359,44 @@ const void *class_kernel(const clk_batch *b, uint32_t c)
 
 // LDS of class c for jobs up to kmax clusters and dmax features with the
 // given waves per work-group.
-size_t class_lds(uint32_t c, uint32_t waves, uint32_t kmax, uint32_t dmax, bool priv, clk::BLayout *layout)
+size_t class_lds(uint32_t c, uint32_t waves, uint32_t kmax, uint32_t kdmax, uint32_t dmax, clk::BLayout *layout)
 {
     if (c < clk::kMfmaClass0)
-        return clk::batch_lds_bytes(waves, kmax, clk::kFp32Dmax[c / 2], dmax);
-    const clk::BLayout L = clk::mfma_layout_for(waves, kmax, dmax, priv);
+        return clk::batch_lds_bytes(waves, kmax, kdmax, clk::kFp32Dmax[c / 2]);
+    const clk::BLayout L = clk::mfma_layout_for(waves, kmax, kdmax, dmax);
     if (layout)
         *layout = L;
     return L.bytes;
 }
 
-// The most waves per work-group (<= 4) whose LDS fits; 0 if none does.
-uint32_t class_waves(const clk_batch *b, uint32_t c, uint32_t kmax, uint32_t dmax, bool priv)
+// The most waves per work-group whose LDS fits (at most 4 for the FP32
+// scan, 8 for the one-K-block MFMA classes, whose work-group shares one
+// accumulator and the 16-bit table); 0 if none does.
+uint32_t class_waves(const clk_batch *b, uint32_t c, uint32_t kmax, uint32_t kdmax, uint32_t dmax)
 {
-    for (uint32_t w = clk::kBatchMaxWaves; w >= 1; --w)
-        if (class_lds(c, w, kmax, dmax, priv, nullptr) + 4096 <= b->ctx->lds_per_block)  // + static LDS
+    // (two K-blocks need more than 256 VGPRs: four waves)
+    const uint32_t most = c < clk::kMfmaClass0 || c >= clk::kMfmaNkb2Class ? clk::kBatchMaxWaves
+                                                                             : clk::kBatchMfmaWaves;
+    for (uint32_t w = most; w >= 1; --w)
+        if (class_lds(c, w, kmax, kdmax, dmax, nullptr) + 4096 <= b->ctx->lds_per_block)  // + static LDS
             return w;
     return 0;
 }
 
-// MFMA classes: one table per wave (no barrier phases) when two 4-wave
-// work-groups still fit a CU's LDS, else one shared table per work-group.
-bool class_private(const clk_batch *b, uint32_t c, uint32_t kmax, uint32_t dmax)
+// Whether class c still fits with one more job (k, d) beside its active and
+// staged jobs (the sums of a tile need k*d int64 cells in LDS).
+bool class_fits_with(const clk_batch *b, uint32_t c, uint32_t k, uint32_t d)
 {
-    return c >= clk::kMfmaClass0 &&
-           2 * (class_lds(c, clk::kBatchMaxWaves, kmax, dmax, true, nullptr) + 4096) <= b->ctx->lds_per_block;
+    uint32_t kmax = k, kdmax = k * d, dmax = d;
+    for (size_t s = 0; s < b->jobs.size(); ++s) {
+        const BJob &J = b->jobs[s];
+        if ((J.state != CLK_JOB_ACTIVE && J.state != kJobStaged) || class_of(J.kind, J.d, J.k) != c)
+            continue;
+        kmax = std::max(kmax, J.k);
+        kdmax = std::max(kdmax, J.k * J.d);
+        dmax = std::max(dmax, J.d);
+    }
+    return class_waves(b, c, kmax, kdmax, dmax) > 0;
 }
 
 int rebuild(clk_batch *b, hipStream_t stream)
@@ -1376,13 +1408,14 @@ int rebuild(clk_batch *b, hipStream_t stream)
         BClassState &C = b->cls[c];
         C = BClassState{};
         C.tile0 = (uint32_t)tiles.size();
-        uint32_t kmax = 0, dmax = 0;
+        uint32_t kmax = 0, kdmax = 0, dmax = 0;
         for (uint32_t s = 0; s < b->jobs.size(); ++s) {
             BJob &J = b->jobs[s];
             if (J.state != CLK_JOB_ACTIVE || class_of(J.kind, J.d, J.k) != c)
                 continue;
             active.push_back(s);
             kmax = std::max(kmax, J.k);
+            kdmax = std::max(kdmax, J.k * J.d);
             dmax = std::max(dmax, J.d);
             J.tile0 = (uint32_t)tiles.size();
             for (uint32_t t = 0; t < J.ntiles; ++t)
@@ -1392,14 +1425,14 @@ int rebuild(clk_batch *b, hipStream_t stream)
         ctl[c].ntiles = C.ntiles;
         if (C.ntiles == 0)
             continue;
-        // the waves per work-group are the class's (fixed at creation): the
-        // FP32 summation order of a tile depends on them, never on the mix
-        C.waves = b->class_waves[c];
-        if (C.waves == 0) {  // admitted jobs fit by construction (clk_batch_create checks max_k x max_d)
+        // the most waves whose LDS fits the class's jobs (the sums are
+        // exact, so the wave count does not change any result)
+        C.waves = class_waves(b, c, kmax, kdmax, dmax);
+        if (C.waves == 0) {  // admission checks every job's class (class_fits_with)
             clk::set_error("clk_batch: a size class does not fit the work-group's LDS");
             return CLK_ERR_UNSUPPORTED;
         }
-        C.lds = class_lds(c, C.waves, kmax, dmax, b->class_priv[c], &C.layout);
+        C.lds = class_lds(c, C.waves, kmax, kdmax, dmax, &C.layout);
         const void *kern = class_kernel(b, c);
         if (C.lds > 65536)
             CLK_HIP_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)C.lds));
@@ -1527,24 +1560,19 @@ extern "C" int clk_batch_create(clk_context *ctx, const clk_batch_config *cfg, c
                 CLK_ERR_INVALID_ARGUMENT, "clk_batch_create: unknown labeling strategy id");
     b->mfma = cfg->labeling != CLK_LABELING_UNROLL_VECTOR;
     b->fp16 = cfg->labeling == CLK_LABELING_MFMA_FP16;
-    // waves per work-group of each class, for its largest admissible job;
-    // every class a job up to max_k x max_d can land in must fit one wave
+    // every class must take jobs up to 64 clusters with max_d features (a
+    // job with more clusters is checked at admission: a tile's exact sums
+    // need k*d int64 cells in LDS)
     {
-        for (uint32_t c = 0; c < clk::kClasses; ++c) {
-            const bool big = (c & 1u) != 0;
-            const uint32_t kmax = big ? cfg->max_k : std::min(cfg->max_k, kSmallK);
-            const uint32_t dcap = c < clk::kMfmaClass0 ? clk::kFp32Dmax[c / 2] : (16u << ((c - clk::kMfmaClass0) / 2));
-            const uint32_t dmax = std::min(cfg->max_d, dcap);
-            b->class_priv[c] = class_private(b, c, kmax, dmax);
-            b->class_waves[c] = class_waves(b, c, kmax, dmax, b->class_priv[c]);
+        bool fits = cfg->max_d >= 1 && cfg->max_d <= 64 && cfg->max_k >= 1;
+        for (uint32_t kind = 0; fits && kind <= (b->mfma ? 1u : 0u); ++kind) {
+            const uint32_t k = std::min(cfg->max_k, kSmallK);
+            const uint32_t c = class_of(kind, cfg->max_d, k);
+            fits = class_waves(b, c, k, k * cfg->max_d, cfg->max_d) > 0;
         }
-        bool fits = true;
-        for (uint32_t kind = 0; kind <= (b->mfma ? 1u : 0u); ++kind)
-            for (uint32_t k : {std::min(cfg->max_k, kSmallK), cfg->max_k})
-                fits = fits && b->class_waves[class_of(kind, cfg->max_d, k)] > 0;
         if (!fits) {
             delete b;
-            clk::set_error("clk_batch_create: max_k x max_d too large for the work-group's LDS");
+            clk::set_error("clk_batch_create: max_d outside 1..64, or a size class does not fit the LDS");
             return CLK_ERR_UNSUPPORTED;
         }
     }
@@ -1607,7 +1635,6 @@ namespace {
 // Host-side slot state while a job's data is copied in but the job is not
 // yet in the device job table (the serving loop stages copies while the GPU
 // works).
-constexpr uint32_t kJobStaged = 3;
 
 // Stage a job: a free slot and arena range, the job record, and the copies of
 // its points and initial centroids (async on stream).  The slot is not yet
@@ -1630,6 +1657,17 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
     if (s < 0) {
         clk::set_error("clk_batch_add: no free slot");
         return CLK_ERR_NOMEM;
+    }
+    {
+        const uint32_t cls = class_of(use_mfma(b, d, k) ? 1u : 0u, d, k);
+        if (!class_fits_with(b, cls, k, d)) {
+            // alone it never fits; beside the resident jobs of its class it
+            // fits once some of them have finished
+            const bool alone = class_waves(b, cls, k, k * d, d) > 0;
+            clk::set_error(alone ? "clk_batch_add: the job's size class is full (LDS); retry after jobs finish"
+                                 : "clk_batch_add: k*d too large for the batch engine's LDS accumulator");
+            return alone ? CLK_ERR_NOMEM : CLK_ERR_UNSUPPORTED;
+        }
     }
     const uint32_t tp = tile_points(b->tile_work, d, k);
     const uint64_t need = job_bytes(n, d, k, row_tile_points(b->tile_work, d, k), !device_points);
@@ -1670,6 +1708,10 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
     J.cen_t_w = clk::keep_row_width(d);
     p += clk::align_up((uint64_t)k * J.cen_t_w * 4, 256);
     J.part = reinterpret_cast<double *>(p);
+    p += clk::align_up(clk::div_up(n, row_tile_points(b->tile_work, d, k)) * (uint64_t)k * (d + 1) * 8, 256);
+    J.gmax = reinterpret_cast<uint32_t *>(p);
+    p += clk::align_up((uint64_t)d * 4, 256);
+    J.flags = reinterpret_cast<uint32_t *>(p);
     J.n = n;
     J.d = d;
     J.k = k;
@@ -1686,6 +1728,10 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
         CLK_HIP_CHECK(
             hipMemcpyAsync(const_cast<float *>(J.pts), host_points, n * d * 4, hipMemcpyHostToDevice, st));
     CLK_HIP_CHECK(hipMemcpyAsync(J.cen[0], host_centroids, (size_t)d * k * 4, hipMemcpyHostToDevice, st));
+    // the job's sum grid (the one clk_sum_grid_from_points computes for these
+    // points) and cleared flags
+    CLK_TRY(clk::launch_grid_max(b->ctx, d, n, J.pts, J.gmax, st));
+    CLK_HIP_CHECK(hipMemsetAsync(J.flags, 0, (size_t)d * k * 4, st));
     if (zero_outputs || max_iterations == 0) {  // else the first iteration writes both
         CLK_HIP_CHECK(hipMemsetAsync(J.masses, 0, (size_t)k * 4, st));
         CLK_HIP_CHECK(hipMemsetAsync(J.labels, 0, n * 4, st));
@@ -1892,7 +1938,11 @@ extern "C" int clk_batch_run_serial(clk_batch *b, void *stream)
         BJob &J = dev[s];
         if (b->jobs[s].state == CLK_JOB_FREE || J.state != CLK_JOB_ACTIVE)
             continue;
-        // the job's partial rows double as its FP64 sums buffer [d][k]
+        // the job's sum grid (as the batch's admission computes it); the
+        // job's partial rows double as its FP64 sums buffer [d][k]
+        rc = clk_sum_grid_from_points(b->ctx, J.d, J.n, J.pts, st);
+        if (rc != CLK_OK)
+            break;
         while (J.state == CLK_JOB_ACTIVE) {
             float *cur = J.cen[J.iter & 1], *nxt = J.cen[(J.iter & 1) ^ 1];
             rc = clk_lloyd_step(b->ctx, &cfg, J.d, J.n, J.k, J.pts, cur, nxt, J.part, J.labels,
@@ -1917,6 +1967,7 @@ extern "C" int clk_batch_run_serial(clk_batch *b, void *stream)
     }
     (void)hipStreamSynchronize(st);
     (void)hipFree(shift_d);
+    (void)clk_sum_grid_clear(b->ctx);
     CLK_TRY(rc);
     CLK_HIP_CHECK(hipMemcpy(b->d_jobs, dev.data(), dev.size() * sizeof(BJob), hipMemcpyHostToDevice));
     return CLK_OK;
@@ -1989,6 +2040,8 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
         CLK_REQUIRE(job_bytes(j.n, j.d, j.k, row_tile_points(b->tile_work, j.d, j.k), j.device_points == 0) <=
                         b->cfg.arena_bytes,
                     CLK_ERR_NOMEM, "clk_serve_run: a job is larger than the arena");
+        CLK_REQUIRE(class_waves(b, class_of(use_mfma(b, j.d, j.k) ? 1u : 0u, j.d, j.k), j.k, j.k * j.d, j.d) > 0,
+                    CLK_ERR_UNSUPPORTED, "clk_serve_run: a job's k*d is too large for the batch engine's LDS");
     }
     const uint32_t slots = (uint32_t)b->jobs.size();
     hipStream_t cs = nullptr;  // admission copies, concurrent with the iterations
@@ -2336,6 +2389,11 @@ extern "C" int clk_serve_serial(clk_context *ctx, clk_serve_job *jobs, uint32_t 
             rc = CLK_ERR_HIP;
             break;
         }
+        // the job's sum grid, pinned for its iterations (the batch computes
+        // the same at admission)
+        rc = clk_sum_grid_from_points(ctx, j.d, j.n, jp, st);
+        if (rc != CLK_OK)
+            break;
         uint32_t it = 0;
         double shift = 0.0;
         int cur = 0;
@@ -2378,6 +2436,7 @@ extern "C" int clk_serve_serial(clk_context *ctx, clk_serve_job *jobs, uint32_t 
         j.finished = seconds_since(t0);
     }
     (void)hipStreamSynchronize(st);
+    (void)clk_sum_grid_clear(ctx);
     for (void *p : {(void *)pts, (void *)cen[0], (void *)cen[1], (void *)sums, (void *)labels, (void *)masses,
                     (void *)shift_d})
         if (p)

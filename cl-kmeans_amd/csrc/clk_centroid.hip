@@ -7,19 +7,20 @@
 //                       local_features*thread_features features, threadIdx.y
 //                       a thread_features sub-slice; private LDS columns
 //                       (lloyd_feature_sum_pardim.cl's points x feature tiles)
-//   feature_sum         one LDS tile [d][k] shared by the work-group, work-
-//                       items stride over the flattened feature-major matrix
-//                       (coalesced rows), ds_add_f32 resolves collisions; the
-//                       smallest LDS footprint, for large k*d
-// Tiles are summed per work-group in FP64 and merged by a fixed-order column
-// reduction, except feature_sum whose in-tile FP32 addition order is the
-// hardware's.  For large k*d (when neither fits) every strategy uses the
-// cluster-major slice kernel below (per-wave LDS tables [k][4] per feature
-// slice, updated by plain read-modify-writes made collision-free by an owner
-// claim; LDS float atomics run ~9x slower than read-modify-writes on
-// gfx950); the FP64 global-atomic pass remains for k too large for that.
-// `masses` is not read, as in the reference kernels.
+//   feature_sum         one LDS accumulator [d][k] shared by the work-group,
+//                       work-items stride over the flattened feature-major
+//                       matrix (coalesced rows); the smallest LDS footprint
+// Every coordinate is rounded to its grid integer and added exactly
+// (clk_grid.hpp): the private int32 columns flush into the work-group's
+// int64 accumulator (ds_add_u64) at least every 64 values, shared
+// accumulators take ds_add_u64 directly.  The work-group's sums (times the
+// unit) form its FP64 partial row; a column reduction adds the rows (exact).
+// So every strategy and geometry gives the same bits.  For large k*d (when
+// neither fits) every strategy uses the cluster-major slice kernel below;
+// the FP64 global-atomic pass (of grid values, so exact as well) remains for
+// k too large for that.  `masses` is not read, as in the reference kernels.
 #include "clk_common.hpp"
+#include "clk_grid.hpp"
 #include "clk_kernels.hpp"
 
 #include <algorithm>
@@ -27,22 +28,87 @@
 namespace clk {
 namespace {
 
+__host__ __device__ constexpr uint32_t align4u(uint32_t v) { return (v + 3u) & ~3u; }
+
+// LDS words before a kernel's tile: scales [nf] | acc [nf][k] int64 |
+// flags [nf][k].
+__host__ __device__ inline uint32_t acc_head_words(uint32_t nf, uint32_t k)
+{
+    return align4u(nf) + 2u * nf * k + align4u(nf * k);
+}
+
+struct AccHead {
+    float *scl;
+    long long *acc;
+    uint32_t *flg;
+    uint32_t *tail;
+};
+
+// Carve the head; stage the scales of features f0 .. f0+nf-1 and zero the
+// accumulator and flags (callers add a barrier).
+__device__ inline AccHead acc_head(void *lds, uint32_t nf, uint32_t k, uint32_t f0, uint32_t d,
+                                   const GridRef &grid, uint32_t tid, uint32_t nthreads)
+{
+    AccHead h;
+    h.scl = static_cast<float *>(lds);
+    h.acc = reinterpret_cast<long long *>(h.scl + align4u(nf));
+    h.flg = reinterpret_cast<uint32_t *>(h.acc + (size_t)nf * k);
+    h.tail = h.flg + align4u(nf * k);
+    for (uint32_t j = tid; j < nf; j += nthreads)
+        h.scl[j] = f0 + j < d ? grid_scale(grid.gmax[f0 + j], grid.bits) : 1.0f;
+    for (uint32_t i = tid; i < nf * k; i += nthreads) {
+        h.acc[i] = 0;
+        h.flg[i] = 0u;
+    }
+    return h;
+}
+
+// Add x (feature slot j of the head) of a point of cluster c into an int32
+// private cell, flagging off-grid values.
+__device__ __forceinline__ void add_private(uint32_t *cell, float x, float scale, const AccHead &h,
+                                            uint32_t j, uint32_t k, uint32_t c)
+{
+    bool ok;
+    const int32_t q = grid_q(x, scale, ok);
+    lds_add(cell, (uint32_t)q);
+    if (__builtin_expect(!ok, 0))
+        lds_or(h.flg + j * k + c, special_flag(x));
+}
+
 template <int DMAX, int VEC, int BLOCK>
 __global__ void __launch_bounds__(BLOCK > 0 ? BLOCK : 1024)
 cu_cluster_merge_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
-                        const uint32_t *__restrict__ labels, double *__restrict__ part)
+                        const uint32_t *__restrict__ labels, GridRef grid, double *__restrict__ part)
 {
-    extern __shared__ __align__(16) float tile[];  // [k*d][B]
+    extern __shared__ __align__(16) float lds_cm[];
     const uint32_t B = BLOCK > 0 ? BLOCK : blockDim.x;
     const uint32_t tid = threadIdx.x;
     const uint32_t rows = k * d;
+    const AccHead h = acc_head(lds_cm, d, k, 0, d, grid, tid, B);
+    uint32_t *tile = h.tail;  // [k*d][B] int32 grid sums
     for (uint32_t i = tid; i < rows * B; i += B)
-        tile[i] = 0.0f;
+        tile[i] = 0u;
     __syncthreads();
-    float *mine = tile + tid;
+    uint32_t *mine = tile + tid;
     const uint32_t row_stride = d * B;
     const uint64_t groups = n / VEC;
     const uint64_t stride = (uint64_t)gridDim.x * B;
+    // a cell takes at most VEC values per step: flush every 64 / VEC steps
+    constexpr uint32_t kFlushSteps = 64 / VEC;
+    uint32_t since = 0;
+    auto flush = [&]() {
+        uint32_t c = tid % k;
+        for (uint32_t i = 0; i < k; ++i) {
+            uint32_t *cell = mine + __umul24(c, row_stride);
+#pragma unroll 1
+            for (uint32_t f = 0; f < d; ++f, cell += B) {
+                const int32_t v = (int32_t)*cell;
+                *cell = 0u;
+                lds_add64(h.acc + f * k + c, (long long)v);
+            }
+            c = (c + 1 == k) ? 0u : c + 1;
+        }
+    };
     for (uint64_t g = (uint64_t)blockIdx.x * B + tid; g < groups; g += stride) {
         const uint64_t p = g * VEC;
         float x[DMAX][VEC];
@@ -53,125 +119,151 @@ cu_cluster_merge_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, u
         for (int v = 0; v < VEC; ++v) {
             if (lab[v] >= k)
                 continue;
-            float *t = mine + __umul24(lab[v], row_stride);
+            uint32_t *t = mine + __umul24(lab[v], row_stride);
 #pragma unroll
             for (int f = 0; f < DMAX; ++f)
                 if (f < (int)d)
-                    lds_acc(t + f * B, x[f][v]);
+                    add_private(t + f * B, x[f][v], h.scl[f], h, (uint32_t)f, k, lab[v]);
+        }
+        if (++since == kFlushSteps) {
+            flush();
+            since = 0;
         }
     }
+    if (since != 0)
+        flush();
     __syncthreads();
-    for (uint32_t r = tid; r < rows; r += B) {
-        const uint32_t c = r / d, f = r % d;
-        part[(uint64_t)blockIdx.x * rows + (uint64_t)f * k + c] =
-            lds_row_sum_f32(tile + (uint64_t)r * B, B, r);
+    for (uint32_t i = tid; i < rows; i += B) {
+        const uint32_t f = i / k;  // i = f*k + c
+        part[(uint64_t)blockIdx.x * rows + i] = cell_value(h.acc[i], grid_unit(h.scl[f]), h.flg[i]);
     }
 }
 
 // blockDim = (Bx, lf); block (x, y) owns features [y*FS, y*FS+FS), FS = lf*tf;
 // thread (tx, ty) owns features y*FS + ty*tf + [0, tf) of points
-// x*Bx + tx + i*gridDim.x*Bx.  LDS rows c*FS + (ty*tf + j), columns tx.
+// x*Bx + tx + i*gridDim.x*Bx.  Private LDS rows c*FS + (ty*tf + j), columns tx.
 __global__ void __launch_bounds__(1024)
 cu_pardim_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k, uint32_t tf,
-                 const uint32_t *__restrict__ labels, double *__restrict__ part)
+                 const uint32_t *__restrict__ labels, GridRef grid, double *__restrict__ part)
 {
-    extern __shared__ __align__(16) float tile[];
+    extern __shared__ __align__(16) float lds_pd[];
     const uint32_t Bx = blockDim.x, lf = blockDim.y;
     const uint32_t FS = lf * tf;
     const uint32_t f0 = blockIdx.y * FS;
     const uint32_t tlin = threadIdx.y * Bx + threadIdx.x;
     const uint32_t nthreads = Bx * lf;
     const uint32_t rows = k * FS;
+    const AccHead h = acc_head(lds_pd, FS, k, f0, d, grid, tlin, nthreads);
+    uint32_t *tile = h.tail;
     for (uint32_t i = tlin; i < rows * Bx; i += nthreads)
-        tile[i] = 0.0f;
+        tile[i] = 0u;
     __syncthreads();
     const uint32_t fsub = threadIdx.y * tf;  // first feature of this thread in the slice
     const uint64_t stride = (uint64_t)gridDim.x * Bx;
+    // a cell takes at most one value per point: flush every 64 points
+    uint32_t since = 0;
+    auto flush = [&]() {
+        for (uint32_t c = 0; c < k; ++c) {
+            uint32_t *t = tile + ((uint64_t)c * FS + fsub) * Bx + threadIdx.x;
+            for (uint32_t j = 0; j < tf; ++j) {
+                const int32_t v = (int32_t)t[j * Bx];
+                t[j * Bx] = 0u;
+                lds_add64(h.acc + (fsub + j) * k + c, (long long)v);
+            }
+        }
+    };
     for (uint64_t p = (uint64_t)blockIdx.x * Bx + threadIdx.x; p < n; p += stride) {
         const uint32_t c = labels[p];
-        if (c >= k)
-            continue;
-        float *t = tile + ((uint64_t)c * FS + fsub) * Bx + threadIdx.x;
-        for (uint32_t j = 0; j < tf; ++j) {
-            const uint32_t f = f0 + fsub + j;
-            if (f < d)
-                lds_acc(t + j * Bx, pts[(uint64_t)f * n + p]);
+        if (c < k) {
+            uint32_t *t = tile + ((uint64_t)c * FS + fsub) * Bx + threadIdx.x;
+            for (uint32_t j = 0; j < tf; ++j) {
+                const uint32_t f = f0 + fsub + j;
+                if (f < d)
+                    add_private(t + j * Bx, pts[(uint64_t)f * n + p], h.scl[fsub + j], h, fsub + j, k, c);
+            }
+        }
+        if (++since == 64) {
+            flush();
+            since = 0;
         }
     }
+    if (since != 0)
+        flush();
     __syncthreads();
-    for (uint32_t r = tlin; r < rows; r += nthreads) {
-        const uint32_t c = r / FS, f = f0 + r % FS;
+    for (uint32_t i = tlin; i < FS * k; i += nthreads) {
+        const uint32_t j = i / k, c = i % k, f = f0 + j;
         if (f < d)
             part[(uint64_t)blockIdx.x * ((uint64_t)d * k) + (uint64_t)f * k + c] =
-                lds_row_sum_f32(tile + (uint64_t)r * Bx, Bx, r);
+                cell_value(h.acc[i], grid_unit(h.scl[j]), h.flg[i]);
     }
 }
 
-// Shared [d][k] tile; work-items stride over elements e = f*n + p.
+// Shared accumulator [d][k]; work-items stride over elements e = f*n + p.
 __global__ void __launch_bounds__(1024)
 cu_feature_sum_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
-                      const uint32_t *__restrict__ labels, double *__restrict__ part)
+                      const uint32_t *__restrict__ labels, GridRef grid, double *__restrict__ part)
 {
-    extern __shared__ __align__(16) float tile[];
+    extern __shared__ __align__(16) float lds_fs[];
     const uint32_t rows = d * k;
-    for (uint32_t i = threadIdx.x; i < rows; i += blockDim.x)
-        tile[i] = 0.0f;
+    const AccHead h = acc_head(lds_fs, d, k, 0, d, grid, threadIdx.x, blockDim.x);
     __syncthreads();
     const uint64_t total = n * d;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
         const uint64_t f = e / n, p = e - f * n;
         const uint32_t c = labels[p];
-        if (c < k)
-            lds_add(tile + f * k + c, pts[e]);
+        if (c < k) {
+            const float x = pts[e];
+            bool ok;
+            const int32_t q = grid_q(x, h.scl[f], ok);
+            lds_add64(h.acc + f * k + c, (long long)q);
+            if (__builtin_expect(!ok, 0))
+                lds_or(h.flg + f * k + c, special_flag(x));
+        }
     }
     __syncthreads();
-    for (uint32_t r = threadIdx.x; r < rows; r += blockDim.x)
-        part[(uint64_t)blockIdx.x * rows + r] = (double)tile[r];
+    for (uint32_t i = threadIdx.x; i < rows; i += blockDim.x)
+        part[(uint64_t)blockIdx.x * rows + i] = cell_value(h.acc[i], grid_unit(h.scl[i / k]), h.flg[i]);
 }
 
 // Cluster-major feature slices for large k*d.  blockIdx.y selects features
-// [4*blockIdx.y, +4); each wave owns a private FP32 table T[c] (one float4 per
-// cluster) and an owner array O[c] in LDS, so no two waves ever touch one
-// table.  A wave takes 64 consecutive points (lane = point: 4 coalesced
-// feature loads and the label) and adds each point's float4 into T[label]
-// with a plain read-modify-write: lanes write their id to O[label], read it
-// back, and the lanes that find their own id -- one per distinct label --
-// update their row; the others retry (at most the largest label
-// multiplicity in the wave, usually 1-3 rounds).  The wave tables are summed
-// in FP64 into the work-group's partial row.  Work-groups are sized so a
-// table cell takes about 64 points (FP32 partial sums with few roundings).
+// [4*blockIdx.y, +4); the work-group shares one int64 accumulator [k][4] in
+// LDS.  A wave takes 64 consecutive points (lane = point: 4 coalesced
+// feature loads and the label) and adds each point's grid integers into its
+// cluster's row with ds_add_u64 (exact, so no ordering between lanes or
+// waves is needed).  The accumulator (times the units) is the work-group's
+// partial row.
 constexpr uint32_t kSliceF = 4, kSliceWaves = 4, kSlicePrefetch = 8;
 
 size_t slice_lds_bytes(uint64_t k)
 {
-    return (size_t)kSliceWaves * k * (kSliceF * 4 + 4);
+    return (size_t)acc_head_words(kSliceF, (uint32_t)k) * 4;
 }
 
 __global__ void __launch_bounds__(64 * kSliceWaves)
 cu_slice_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
-                const uint32_t *__restrict__ labels, double *__restrict__ part)
+                const uint32_t *__restrict__ labels, GridRef grid, double *__restrict__ part)
 {
-    extern __shared__ __align__(16) float tab[];  // [waves][k] float4, then [waves][k] owners
+    extern __shared__ __align__(16) float lds_sl[];
     typedef float f4 __attribute__((ext_vector_type(4)));
     // the wave index as a scalar: tile indices and the loop below are then
     // wave-uniform, so the prefetch ring needs no divergent control flow
     const uint32_t tid = threadIdx.x, lane = tid % 64;
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid / 64));
-    f4 *T = reinterpret_cast<f4 *>(tab) + (size_t)wave * k;
-    uint32_t *O = reinterpret_cast<uint32_t *>(reinterpret_cast<f4 *>(tab) + (size_t)kSliceWaves * k) +
-                  (size_t)wave * k;
-    for (uint32_t i = tid; i < kSliceWaves * k; i += blockDim.x)
-        reinterpret_cast<f4 *>(tab)[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    __syncthreads();
     const uint32_t f0 = blockIdx.y * kSliceF;
     const uint32_t nf = min(kSliceF, d - f0);
+    const AccHead h = acc_head(lds_sl, kSliceF, k, f0, d, grid, tid, blockDim.x);
+    __syncthreads();
+    float sc[kSliceF];
+#pragma unroll
+    for (uint32_t j = 0; j < kSliceF; ++j)
+        sc[j] = h.scl[j];
     const uint64_t tiles = (n + 63) / 64;
     const uint64_t tstride = (uint64_t)gridDim.x * kSliceWaves;
     // Every load is unconditional and in bounds (a tile past the end reads
     // the last tile, a feature past the slice reads its last feature; such
-    // values are never added or are zeroed at use), so no branch separates
-    // a load from its use and the loads of later tiles stay in flight.
+    // values are never added), so no branch separates a load from its use
+    // and the loads of later tiles stay in flight.
     auto load = [&](uint64_t tile, f4 &v, uint32_t &L) {
         const uint64_t p = min(min(tile, tiles - 1) * 64 + lane, n - 1);
 #pragma unroll
@@ -181,9 +273,8 @@ cu_slice_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t 
     };
     // A ring of kSlicePrefetch tiles in registers: the loads of the next
     // kSlicePrefetch tiles of this wave are in flight while one is added
-    // (one tile alone leaves too few bytes in flight to cover HBM latency
-    // at the few waves per CU the tables allow).  The ring is indexed by
-    // the unrolled slot, so it stays in registers.
+    // (one tile alone leaves too few bytes in flight to cover HBM latency).
+    // The ring is indexed by the unrolled slot, so it stays in registers.
     f4 vr[kSlicePrefetch];
     uint32_t Lr[kSlicePrefetch];
     const uint64_t first = (uint64_t)blockIdx.x * kSliceWaves + wave;
@@ -194,20 +285,17 @@ cu_slice_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t 
     const uint64_t rounds = mine / kSlicePrefetch;
     const uint32_t rest = (uint32_t)(mine % kSlicePrefetch);
     auto add = [&](uint64_t tile, f4 v, uint32_t L) {
+        const bool valid = tile * 64 + lane < n && L < k;
+        if (valid) {
 #pragma unroll
-        for (uint32_t j = 0; j < kSliceF; ++j)
-            v[j] = j < nf ? v[j] : 0.0f;
-        const bool valid = tile * 64 + lane < n;
-        bool pending = valid && L < k;
-        while (__ballot(pending) != 0) {
-            if (pending)
-                O[L] = lane;
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            const bool win = pending && O[L] == lane;
-            if (win)
-                T[L] = T[L] + v;
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            pending = pending && !win;
+            for (uint32_t j = 0; j < kSliceF; ++j)
+                if (j < nf) {
+                    bool ok;
+                    const int32_t q = grid_q(v[j], sc[j], ok);
+                    lds_add64(h.acc + j * k + L, (long long)q);
+                    if (__builtin_expect(!ok, 0))
+                        lds_or(h.flg + j * k + L, special_flag(v[j]));
+                }
         }
     };
 #pragma unroll
@@ -228,17 +316,14 @@ cu_slice_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t 
         if (s < rest)
             add(base + s * tstride, vr[s], Lr[s]);
     __syncthreads();
-    const f4 *all = reinterpret_cast<const f4 *>(tab);
     for (uint32_t i = tid; i < k * nf; i += blockDim.x) {
-        const uint32_t c = i / nf, j = i % nf;
-        double s = 0.0;
-        for (uint32_t w = 0; w < kSliceWaves; ++w)
-            s += (double)all[(size_t)w * k + c][j];
-        part[(uint64_t)blockIdx.x * ((uint64_t)d * k) + (uint64_t)(f0 + j) * k + c] = s;
+        const uint32_t j = i / k, c = i % k;
+        part[(uint64_t)blockIdx.x * ((uint64_t)d * k) + (uint64_t)(f0 + j) * k + c] =
+            cell_value(h.acc[i], grid_unit(h.scl[j]), h.flg[i]);
     }
 }
 
-using CmKernel = void (*)(const float *, uint64_t, uint32_t, uint32_t, const uint32_t *, double *);
+using CmKernel = void (*)(const float *, uint64_t, uint32_t, uint32_t, const uint32_t *, GridRef, double *);
 
 // Points per work-item at most (VEC x DMAX values in registers, no spills).
 constexpr int cm_vec_cap(int dmax) { return dmax >= 64 ? 1 : dmax >= 32 ? 2 : 4; }
@@ -297,6 +382,10 @@ int run_centroid_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t
     const uint64_t dk = d * k;
     const int dmax = dmax_bucket(d);
 
+    GridRef sg;
+    CLK_TRY(resolve_grid(ctx, d, n, points, stream, &sg));
+    const uint64_t head = (uint64_t)acc_head_words((uint32_t)d, (uint32_t)k) * 4;
+
     const void *kern = nullptr;
     const char *name = nullptr;
     size_t lds = 0;
@@ -304,19 +393,22 @@ int run_centroid_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t
     uint32_t tf = 1;
     uint64_t gx = 0;
 
-    if (strategy == CLK_CENTROID_CLUSTER_MERGE && dmax && dk * B * 4 <= ctx->lds_per_block) {
+    if (strategy == CLK_CENTROID_CLUSTER_MERGE && dmax && dk < (1ull << 24) &&
+        head + dk * B * 4 <= ctx->lds_per_block) {
         const int vec = std::min(pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels),
                                  cm_vec_cap(dmax));
         CmKernel kc = pick_cm(dmax, vec, B == 256);
-        lds = dk * B * 4;
+        lds = head + dk * B * 4;
         gx = user_grid ? user_grid : auto_grid(ctx, (const void *)kc, (int)B, lds, n / vec);
         gx = std::max<uint64_t>(1, std::min<uint64_t>(gx, 1u << 20));
+        if (lds > 65536)
+            CLK_HIP_CHECK(hipFuncSetAttribute((const void *)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         CLK_TRY(ensure_scratch(ctx->part_sums, gx * dk * sizeof(double)));
         double *ps = static_cast<double *>(ctx->part_sums.ptr);
         {
             KernelTimer t(ctx, "CentroidUpdateClusterMerge", stream);
             hipLaunchKernelGGL(kc, dim3((unsigned)gx), dim3((unsigned)B), lds, stream, points, n,
-                               (uint32_t)d, (uint32_t)k, labels, ps);
+                               (uint32_t)d, (uint32_t)k, labels, sg, ps);
             CLK_HIP_CHECK(hipGetLastError());
         }
         return launch_reduce_f64(ctx, ps, gx, dk, sums, true, stream);
@@ -327,8 +419,8 @@ int run_centroid_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t
         lf = std::min<uint64_t>(lf, d);
         uint64_t bx = std::max<uint64_t>(1, B / lf);
         uint64_t fs = lf * tf;
-        lds = k * fs * bx * 4;
-        if (lds <= ctx->lds_per_block && bx * lf <= (uint64_t)ctx->max_threads_per_block) {
+        lds = (uint64_t)acc_head_words((uint32_t)fs, (uint32_t)k) * 4 + k * fs * bx * 4;
+        if (k * fs < (1ull << 24) && lds <= ctx->lds_per_block && bx * lf <= (uint64_t)ctx->max_threads_per_block) {
             kern = (const void *)cu_pardim_kernel;
             block = dim3((unsigned)bx, (unsigned)lf);
             uint64_t gy = div_up(d, fs);
@@ -340,9 +432,9 @@ int run_centroid_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t
     }
     if (!kern && (strategy == CLK_CENTROID_FEATURE_SUM || strategy == CLK_CENTROID_CLUSTER_MERGE ||
                   strategy == CLK_CENTROID_FEATURE_SUM_PARDIM) &&
-        dk * 4 <= ctx->lds_per_block) {
+        dk < (1ull << 24) && head <= ctx->lds_per_block) {
         kern = (const void *)cu_feature_sum_kernel;
-        lds = dk * 4;
+        lds = head;
         block = dim3((unsigned)B);
         gx = user_grid ? user_grid : auto_grid(ctx, kern, (int)B, lds, n * d);
         gx = std::max<uint64_t>(1, std::min<uint64_t>(gx, 1u << 20));
@@ -356,8 +448,8 @@ int run_centroid_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t
             CLK_HIP_CHECK(hipFuncSetAttribute((const void *)cu_slice_kernel,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl));
         const uint64_t gy = div_up(d, (uint64_t)kSliceF);
-        // about 64 points per table cell and wave, at least two resident
-        // rounds of work-groups
+        // at least four work-groups per CU over the slices, a quarter
+        // k-th of the points per work-group (partial rows stay few)
         uint64_t gxs = std::max<uint64_t>(div_up(n, 64 * k * kSliceWaves), div_up(4 * (uint64_t)ctx->num_cus, gy));
         gxs = std::max<uint64_t>(1, std::min<uint64_t>(gxs, div_up(n, 64 * kSliceWaves)));
         gxs = std::min<uint64_t>(gxs, 65535);
@@ -366,24 +458,26 @@ int run_centroid_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t
         {
             KernelTimer t(ctx, "CentroidUpdateSlices", stream);
             hipLaunchKernelGGL(cu_slice_kernel, dim3((unsigned)gxs, (unsigned)gy), dim3(64 * kSliceWaves), sl,
-                               stream, points, n, (uint32_t)d, (uint32_t)k, labels, ps);
+                               stream, points, n, (uint32_t)d, (uint32_t)k, labels, sg, ps);
             CLK_HIP_CHECK(hipGetLastError());
         }
         return launch_reduce_f64(ctx, ps, gxs, dk, sums, true, stream);
     }
     if (!kern)
-        return launch_accumulate_global(ctx, d, n, k, points, labels, sums, nullptr, stream);
+        return launch_accumulate_global(ctx, d, n, k, points, labels, sg, sums, nullptr, stream);
 
+    if (lds > 65536)
+        CLK_HIP_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     CLK_TRY(ensure_scratch(ctx->part_sums, gx * dk * sizeof(double)));
     double *ps = static_cast<double *>(ctx->part_sums.ptr);
     {
         KernelTimer t(ctx, name, stream);
         if (kern == (const void *)cu_pardim_kernel)
             hipLaunchKernelGGL(cu_pardim_kernel, grid, block, lds, stream, points, n, (uint32_t)d,
-                               (uint32_t)k, tf, labels, ps);
+                               (uint32_t)k, tf, labels, sg, ps);
         else
             hipLaunchKernelGGL(cu_feature_sum_kernel, grid, block, lds, stream, points, n,
-                               (uint32_t)d, (uint32_t)k, labels, ps);
+                               (uint32_t)d, (uint32_t)k, labels, sg, ps);
         CLK_HIP_CHECK(hipGetLastError());
     }
     return launch_reduce_f64(ctx, ps, gx, dk, sums, true, stream);

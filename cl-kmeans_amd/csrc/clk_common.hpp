@@ -68,6 +68,9 @@ struct clk_context {
     clk::Scratch mfma_tables;     // 16-bit centroid operand table, norms, bounds
     clk::Scratch mfma_queue;      // point ids queued for exact refinement
     uint32_t *mfma_qlen = nullptr;  // device queue length inside mfma_tables
+    clk::Scratch grid;            // sum grid: per-feature max |x| bits [d] (clk_grid.hpp)
+    bool grid_pinned = false;     // grid holds a caller-pinned grid for grid_d features
+    uint64_t grid_d = 0, grid_n = 0;
     int mfma_subnormals = -1;     // bit 0 FP16, bit 1 BF16: MFMA keeps subnormal inputs (-1 unknown)
     bool profiling = false;
     int32_t iteration = -1;

@@ -4,20 +4,27 @@
 //
 // Layout per work-group (LDS):
 //   cen   [k][DMAX]            centroids, zero-padded to the register bucket
-//   tile  [k*(d+1)][COLS]      row c*(d+1)+0 = mass of c, rows c*(d+1)+1+f =
-//                              sum of feature f of c; COLS = work-group size
-//                              (cluster_merge: one column per work-item, so
-//                              ds_add never collides) or 1 (feature_sum: one
-//                              tile shared by the work-group).
+//   thr   [k]                  keep thresholds (bounded pass)
+//   scl   [DMAX]               grid scales of the features (clk_grid.hpp)
+//   acc   [d][k] int64         exact grid-integer sums of the work-group
+//   flg   [d][k]               off-grid flags
+//   tile  cluster_merge: [k*(d+1)][B] uint32 -- row c*(d+1) = mass of c,
+//                        rows c*(d+1)+1+f = int32 grid sums of feature f,
+//                        one column per work-item (no collisions);
+//         feature_sum:   [k] uint32 masses (the sums go to acc directly).
 // Each work-item labels VEC consecutive points per step with 128-bit loads
-// (one per feature row), writes the labels, and adds the points into its
-// column (cluster_merge: plain read-add-write, the column is the work-item's
-// own; feature_sum: ds_add_f32 / ds_add_u32 atomics on the shared tile).  At the end the
-// tile columns are summed in FP64 in a fixed order and written as per-group
-// partials; a fixed-order column reduction adds the partials into the
-// caller's FP64 sums and uint32 masses.
+// (one per feature row), writes the labels, rounds the coordinates to their
+// grid integers and adds them: cluster_merge into its own int32 column
+// (ds_add_u32), flushed into acc (ds_add_u64) every 64 values a cell can
+// take; feature_sum straight into acc.  All additions are exact integer
+// additions, so the sums do not depend on the grid or the order.  At the end
+// acc (times the unit) and the summed mass columns are written as
+// per-work-group FP64 / uint32 partials; a column reduction adds the
+// partials into the caller's sums and masses (exact: every partial is a
+// multiple of the unit below 2^53 units).
 #include "clk_common.hpp"
 #include "clk_finalize.hpp"
+#include "clk_grid.hpp"
 #include "clk_kernels.hpp"
 
 #include <algorithm>
@@ -25,34 +32,85 @@
 namespace clk {
 namespace {
 
+__host__ __device__ constexpr uint32_t align4(uint32_t v) { return (v + 3u) & ~3u; }
+
+// LDS floats before the tile (host and device agree on the layout).
+__host__ __device__ inline uint32_t fused_head_words(uint32_t d, uint32_t k, uint32_t dmax, bool bounded)
+{
+    return align4(k * dmax) + (bounded ? align4(k) : 0u) + align4(dmax) + 2u * d * k + align4(d * k);
+}
+
 template <int DMAX, int VEC, int BLOCK, bool PRIVATE, bool EXACT, bool BOUNDED>
 __global__ void __launch_bounds__(BLOCK > 0 ? BLOCK : (DMAX >= 64 ? 512 : 1024))
 fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
-                  const float *__restrict__ cen, uint32_t *__restrict__ labels,
+                  const float *__restrict__ cen, uint32_t *__restrict__ labels, GridRef grid,
                   double *__restrict__ part_sums, uint32_t *__restrict__ part_masses)
 {
     extern __shared__ __align__(16) float smem[];
     const uint32_t B = BLOCK > 0 ? BLOCK : blockDim.x;
     const uint32_t tid = threadIdx.x;
     const uint32_t cols = PRIVATE ? B : 1;
-    const uint32_t rows = k * (d + 1);
+    const uint32_t dk = d * k;
+    const uint32_t rows = PRIVATE ? k * (d + 1) : k;
     float *cen_lds = smem;
-    float *thr_lds = smem + ((k * DMAX + 3u) & ~3u);  // BOUNDED: keep thresholds [k]
-    float *tile = thr_lds + (BOUNDED ? ((k + 3u) & ~3u) : 0u);
+    float *thr_lds = smem + align4(k * DMAX);  // BOUNDED: keep thresholds [k]
+    float *scl = thr_lds + (BOUNDED ? align4(k) : 0u);
+    long long *acc = reinterpret_cast<long long *>(scl + align4(DMAX));  // 16-byte aligned
+    uint32_t *flg = reinterpret_cast<uint32_t *>(acc + dk);
+    uint32_t *tile = flg + align4(dk);
 
     stage_centroids<DMAX>(cen, d, k, cen_lds);
+    for (uint32_t f = tid; f < DMAX; f += B)
+        scl[f] = f < d ? grid_scale(grid.gmax[f], grid.bits) : 1.0f;
+    for (uint32_t i = tid; i < dk; i += B) {
+        acc[i] = 0;
+        flg[i] = 0u;
+    }
     for (uint32_t i = tid; i < rows * cols; i += B)
-        tile[i] = 0.0f;
+        tile[i] = 0u;
     __syncthreads();
     if constexpr (BOUNDED) {
         stage_keep_thresholds<DMAX>(cen_lds, k, thr_lds);
         __syncthreads();
     }
+    // grid scales in registers for small buckets, read from LDS otherwise
+    constexpr bool kRegScale = DMAX <= 16;
+    float sc_reg[kRegScale ? DMAX : 1];
+    if constexpr (kRegScale) {
+#pragma unroll
+        for (int f = 0; f < DMAX; ++f)
+            sc_reg[f] = scl[f];
+    }
+    auto scale_of = [&](int f) -> float {
+        if constexpr (kRegScale)
+            return sc_reg[f];
+        else
+            return scl[f];
+    };
 
     const uint64_t groups = n / VEC;
     const uint64_t stride = (uint64_t)gridDim.x * B;
-    float *mine = tile + (PRIVATE ? tid : 0);
+    uint32_t *mine = tile + (PRIVATE ? tid : 0);
     const uint32_t row_stride = (d + 1) * cols;  // < 2^24: the tile fits in LDS
+    // A private cell takes at most VEC values per step and holds 64 values of
+    // |q| < 2^25 without int32 overflow: flush every 64 / VEC steps.
+    constexpr uint32_t kFlushSteps = 64 / VEC;
+    uint32_t since = 0;
+    auto flush = [&]() {
+        // clusters in a rotated order so the lanes of a wave hit different
+        // acc cells
+        uint32_t c = tid % k;
+        for (uint32_t i = 0; i < k; ++i) {
+            uint32_t *cell = mine + __umul24(c, row_stride) + cols;
+#pragma unroll 1
+            for (uint32_t f = 0; f < d; ++f, cell += cols) {
+                const int32_t v = (int32_t)*cell;
+                *cell = 0u;
+                lds_add64(acc + f * k + c, (long long)v);
+            }
+            c = (c + 1 == k) ? 0u : c + 1;
+        }
+    };
     // Software pipeline, two register buffers: one step's points are in
     // flight while the other step's are labelled and accumulated.
     auto step = [&](const float (&x)[DMAX][VEC], const uint32_t (&prev)[VEC], uint64_t g) {
@@ -90,24 +148,40 @@ fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
             argmin_lds<DMAX, VEC>(x, cen_lds, k, lab);
             store_labels<VEC>(labels, p, lab);
         }
+        bool off = false;
 #pragma unroll
         for (int v = 0; v < VEC; ++v) {
-            float *t = mine + __umul24(lab[v], row_stride);
-            if (PRIVATE) {
-                // u32 LDS atomics are full speed (unlike ds_add_f32): one
-                // ds_add_u32, no VALU, for the mass counter.
-                lds_add(reinterpret_cast<uint32_t *>(t), 1u);
+            uint32_t *t = mine + __umul24(lab[v], row_stride);
+            // u32 LDS atomics run at full speed (ds_add_f32 does not)
+            lds_add(PRIVATE ? t : tile + lab[v], 1u);
 #pragma unroll
-                for (int f = 0; f < DMAX; ++f)
-                    if (EXACT || f < (int)d)
-                        lds_acc(t + (f + 1) * cols, x[f][v]);
-            } else {
-                lds_add(reinterpret_cast<uint32_t *>(t), 1u);
+            for (int f = 0; f < DMAX; ++f)
+                if (EXACT || f < (int)d) {
+                    bool ok;
+                    const int32_t q = grid_q(x[f][v], scale_of(f), ok);
+                    off = off || !ok;
+                    if (PRIVATE)
+                        lds_add(t + (f + 1) * cols, (uint32_t)q);
+                    else
+                        lds_add64(acc + (uint32_t)f * k + lab[v], (long long)q);
+                }
+        }
+        if (__builtin_expect(__ballot(off) != 0, 0)) {
+            // values off the grid (NaN, inf, outside a pinned grid) were
+            // added as 0; their cells are flagged
+            if (off) {
 #pragma unroll
-                for (int f = 0; f < DMAX; ++f)
-                    if (EXACT || f < (int)d)
-                        lds_add(t + (f + 1) * cols, x[f][v]);
+                for (int v = 0; v < VEC; ++v)
+#pragma unroll
+                    for (int f = 0; f < DMAX; ++f)
+                        if ((EXACT || f < (int)d) &&
+                            !(__builtin_fabsf(x[f][v] * scale_of(f)) < kGridLimit))
+                            lds_or(flg + (uint32_t)f * k + lab[v], special_flag(x[f][v]));
             }
+        }
+        if (PRIVATE && ++since == kFlushSteps) {
+            flush();
+            since = 0;
         }
     };
     // Loads are unconditional (index clamped to the last group, always in
@@ -136,25 +210,24 @@ fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
         step(xb, lb, g1);
         g = g2;
     }
+    if (PRIVATE && since != 0)
+        flush();
     __syncthreads();
 
-    const uint64_t dk = (uint64_t)d * k;
-    for (uint32_t r = tid; r < rows; r += B) {
-        const uint32_t c = r / (d + 1), j = r % (d + 1);
-        const float *row = tile + (uint64_t)r * cols;
-        if (j == 0) {
-            part_masses[(uint64_t)blockIdx.x * k + c] =
-                PRIVATE ? lds_row_sum_u32(reinterpret_cast<const uint32_t *>(row), cols, r)
-                        : reinterpret_cast<const uint32_t *>(row)[0];
+    for (uint32_t r = tid; r < k * (d + 1); r += B) {
+        // r < k: masses; then the d*k sums in [d][k] order
+        if (r < k) {
+            part_masses[(uint64_t)blockIdx.x * k + r] =
+                PRIVATE ? lds_row_sum_u32(tile + (uint64_t)r * (d + 1) * cols, cols, r) : tile[r];
         } else {
-            part_sums[(uint64_t)blockIdx.x * dk + (uint64_t)(j - 1) * k + c] =
-                PRIVATE ? lds_row_sum_f32(row, cols, r) : (double)row[0];
+            const uint32_t i = r - k, f = i / k;
+            part_sums[(uint64_t)blockIdx.x * dk + i] = cell_value(acc[i], grid_unit(scl[f]), flg[i]);
         }
     }
 }
 
 using FusedKernel = void (*)(const float *, uint64_t, uint32_t, uint32_t, const float *,
-                             uint32_t *, double *, uint32_t *);
+                             uint32_t *, GridRef, double *, uint32_t *);
 
 // Points per work-item at most (VEC x DMAX values in registers, no spills).
 constexpr int fused_vec_cap(int dmax) { return dmax >= 32 ? 1 : dmax >= 16 ? 2 : 4; }
@@ -162,7 +235,9 @@ constexpr int fused_vec_cap(int dmax) { return dmax >= 32 ? 1 : dmax >= 16 ? 2 :
 template <int DMAX, int BLOCK, bool PRIVATE, bool EXACT, bool BOUNDED>
 FusedKernel pick_vec_block(int vec)
 {
-    constexpr int cap = fused_vec_cap(DMAX);
+    // work-groups of any size (up to 1024 work-items, 128 VGPRs): half the
+    // points per work-item from 8 features on (no scratch)
+    constexpr int cap = BLOCK > 0 ? fused_vec_cap(DMAX) : fused_vec_cap(2 * DMAX);
     if constexpr (cap >= 4)
         if (vec >= 4)
             return fused_tile_kernel<DMAX, 4, BLOCK, PRIVATE, EXACT, BOUNDED>;
@@ -252,17 +327,18 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
     // Variant by LDS fit (the reference picks its GLOBAL_MEM variant the same
     // way, fused_feature_sum.hpp:315-337): private tile, shared tile, or the
     // separate labeling, mass and centroid-update stages for k*d beyond LDS.
-    const uint64_t cen_bytes = dmax ? ((k * dmax + 3) & ~3ull) * 4 : 0;
     const uint64_t rows = k * (d + 1);
-    const uint64_t priv_plain = cen_bytes + rows * (uint64_t)B * 4;
-    const uint64_t shared_bytes = cen_bytes + rows * 4;
-    bool use_private = dmax && strategy != CLK_FUSED_FEATURE_SUM && priv_plain <= ctx->lds_per_block;
+    const uint64_t head_plain = dmax ? (uint64_t)fused_head_words((uint32_t)d, (uint32_t)k, dmax, false) * 4 : 0;
+    const uint64_t thr_bytes = ((k + 3) & ~3ull) * 4;
+    const uint64_t priv_plain = head_plain + rows * (uint64_t)B * 4;
+    const uint64_t shared_bytes = head_plain + thr_bytes;  // masses [k]
+    bool use_private = dmax && strategy != CLK_FUSED_FEATURE_SUM && k * d < (1ull << 24) &&
+                       priv_plain <= ctx->lds_per_block;
     // The bounded pass needs k keep thresholds beside the private tile; where
     // they do not fit it runs as "cluster_merge" (the same results).
-    const uint64_t thr_bytes = ((k + 3) & ~3ull) * 4;
     const bool run_bounded = bounded && use_private && priv_plain + thr_bytes <= ctx->lds_per_block;
     const uint64_t priv_bytes = priv_plain + (run_bounded ? thr_bytes : 0);
-    bool use_shared = dmax && !use_private && shared_bytes <= ctx->lds_per_block;
+    bool use_shared = dmax && !use_private && k * d < (1ull << 24) && shared_bytes <= ctx->lds_per_block;
 
     if ((!use_private && !use_shared) || n == 0) {
         if (mode == 2) {
@@ -289,14 +365,13 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
                                      : pick_fused<true, false>(dmax, exact, vec, b256);
     const size_t lds = use_private ? priv_bytes : shared_bytes;
     const uint64_t groups = n / vec;
-    // The bounded pass takes the grid of the exhaustive private-tile kernel:
-    // the same points per work-item in the same order, so bit-identical sums.
-    const void *grid_kern = run_bounded ? (const void *)pick_fused<true, false>(dmax, exact, vec, b256)
-                                        : (const void *)kern;
-    const size_t grid_lds = run_bounded ? lds - thr_bytes : lds;
     uint64_t grid = (cfg && cfg->global_size[0]) ? div_up(cfg->global_size[0], (uint64_t)B)
-                                                 : auto_grid(ctx, grid_kern, B, grid_lds, groups);
+                                                 : auto_grid(ctx, (const void *)kern, B, lds, groups);
     grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, 1u << 20));
+    if (lds > 65536)
+        CLK_HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    GridRef sgrid;
+    CLK_TRY(resolve_grid(ctx, d, n, points, stream, &sgrid));
 
     CLK_TRY(ensure_scratch(ctx->part_sums, grid * d * k * sizeof(double)));
     CLK_TRY(ensure_scratch(ctx->part_masses, grid * k * sizeof(uint32_t)));
@@ -319,7 +394,7 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
                                    : "FusedClusterMerge",
                       stream);
         hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(B), lds, stream, points, n, (uint32_t)d,
-                           (uint32_t)k, old_centroids, labels, static_cast<double *>(ctx->part_sums.ptr),
+                           (uint32_t)k, old_centroids, labels, sgrid, static_cast<double *>(ctx->part_sums.ptr),
                            static_cast<uint32_t *>(ctx->part_masses.ptr));
         CLK_HIP_CHECK(hipGetLastError());
     }

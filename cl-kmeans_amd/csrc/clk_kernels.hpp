@@ -50,8 +50,6 @@ int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, ui
                          const float *points, const float *centroids, uint32_t *labels,
                          hipStream_t stream);
 
-// sums[f,c] += x_f and masses[c] += 1 for every point, with FP64 / u32
-// global atomics (any k*d; summation order not fixed).  masses may be null.
 // The mass and centroid-update stages (accumulate into masses / sums); a
 // null cfg selects part_private / cluster_merge.
 int run_mass_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t n, uint64_t k,
@@ -60,8 +58,12 @@ int run_centroid_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t
                         uint64_t k, const float *points, double *sums, const uint32_t *labels,
                         hipStream_t stream);
 
+// sums[f,c] += x_f (the grid value of x_f, clk_grid.hpp) and masses[c] += 1
+// for every point, with FP64 / u32 global atomics (any k*d; exact, so the
+// order of the atomics does not matter).  masses may be null.
+struct GridRef;
 int launch_accumulate_global(clk_context *ctx, uint64_t d, uint64_t n, uint64_t k,
-                             const float *points, const uint32_t *labels, double *sums,
-                             uint32_t *masses, hipStream_t stream);
+                             const float *points, const uint32_t *labels, const GridRef &grid,
+                             double *sums, uint32_t *masses, hipStream_t stream);
 
 }  // namespace clk

@@ -1,6 +1,7 @@
 // Labeling pass (LabelingUnrollVector / lloyd_labeling_vp_clcp.cl) and the
 // global-accumulation pass used when k*d exceeds the LDS tiles.
 #include "clk_common.hpp"
+#include "clk_grid.hpp"
 #include "clk_kernels.hpp"
 
 #include <algorithm>
@@ -168,7 +169,7 @@ label_global_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint3
 
 __global__ void __launch_bounds__(256)
 accumulate_global_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
-                         const uint32_t *__restrict__ labels, double *__restrict__ sums,
+                         const uint32_t *__restrict__ labels, GridRef grid, double *__restrict__ sums,
                          uint32_t *__restrict__ masses)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -178,8 +179,16 @@ accumulate_global_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, 
             continue;
         if (masses)
             atomicAdd(&masses[c], 1u);
-        for (uint32_t f = 0; f < d; ++f)
-            unsafeAtomicAdd(&sums[(uint64_t)f * k + c], (double)pts[(uint64_t)f * n + p]);
+        for (uint32_t f = 0; f < d; ++f) {
+            // the grid value (a multiple of the unit below 2^53 units: every
+            // partial sum is exact); off the grid the special value
+            const float x = pts[(uint64_t)f * n + p];
+            const float sc = grid_scale(grid.gmax[f], grid.bits);
+            bool ok;
+            const int32_t q = grid_q(x, sc, ok);
+            const double v = ok ? (double)q * grid_unit(sc) : special_value(special_flag(x));
+            unsafeAtomicAdd(&sums[(uint64_t)f * k + c], v);
+        }
     }
 }
 
@@ -319,15 +328,15 @@ int launch_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, 
 }
 
 int launch_accumulate_global(clk_context *ctx, uint64_t d, uint64_t n, uint64_t k,
-                             const float *points, const uint32_t *labels, double *sums,
-                             uint32_t *masses, hipStream_t stream)
+                             const float *points, const uint32_t *labels, const GridRef &sgrid,
+                             double *sums, uint32_t *masses, hipStream_t stream)
 {
     if (n == 0)
         return CLK_OK;
     uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(div_up(n, 256), (uint64_t)ctx->num_cus * 8));
     KernelTimer t(ctx, "AccumulateGlobal", stream);
     hipLaunchKernelGGL(accumulate_global_kernel, dim3((unsigned)grid), dim3(256), 0, stream,
-                       points, n, (uint32_t)d, (uint32_t)k, labels, sums, masses);
+                       points, n, (uint32_t)d, (uint32_t)k, labels, sgrid, sums, masses);
     CLK_HIP_CHECK(hipGetLastError());
     return CLK_OK;
 }

@@ -26,8 +26,10 @@
  *    thread at a time, like one reference operator object.
  *
  * Accumulator type.  The reference accumulates centroid sums in PointT
- * (float).  This library accumulates them in FP64 (double[d][k]) and divides
- * in FP64, rounding once to float; masses stay uint32.
+ * (float).  This library accumulates them exactly (integers on a
+ * per-feature grid, see "exact sums" below), hands them over as FP64
+ * (double[d][k]) and divides in FP64, rounding once to float; masses stay
+ * uint32.
  */
 #ifndef CLKMEANS_H
 #define CLKMEANS_H
@@ -252,6 +254,44 @@ int clk_fill_zero(clk_context *ctx, void *ptr, uint64_t bytes, void *stream);
  * (kmeans_initializer.cpp:29-39). */
 int clk_init_first_x(clk_context *ctx, uint64_t d, uint64_t n, uint64_t k,
                      const float *points, float *centroids, void *stream);
+
+/* ---- exact sums: the sum grid ----------------------------------------- */
+/*
+ * No reference counterpart: the reference's FP32 sums follow its kernel
+ * geometry (per-work-item tiles, lloyd_fused_cluster_merge.cl, merged by
+ * reduce_vector_parcol.cl's tree), so its centroid bits depend on the
+ * launch.  Here every op that accumulates centroid sums (clk_fused,
+ * clk_lloyd_step, clk_centroid_update and the batch engine) first rounds
+ * each coordinate to nearest-even on a per-feature power-of-two grid and
+ * adds the grid integers exactly: sums, centroids and therefore labels do
+ * not depend on the strategy, the launch geometry, chunking, the batch tile
+ * size or the other jobs of a batch.
+ *   unit of feature f = 2^(E_f - b), where M_f = max |x_f| over the finite
+ *   coordinates, E_f = the least integer with M_f < 2^E_f (at least -100)
+ *   and b = min(25, 53 - ceil(log2(n + 1)));
+ * a coordinate moves by at most half a unit (<= 2^-26 M_f).  A NaN / +-inf
+ * coordinate makes its cell's sum NaN / +-inf (NaN for both infinities).
+ * Without a pinned grid each op uses the grid of its own (points, n): one
+ * extra read of the points per call.  A pinned grid serves every later op of
+ * the context with the same d until cleared or replaced: the strategies
+ * (clk_kmeans_run, the C++ classes) pin the grid of the whole data set once,
+ * so their chunked (_buffered) pipelines give the unchunked sums.  A point
+ * outside a pinned grid's range (|x_f| >= 2^E_f) makes its cell's sum NaN.
+ */
+/* Compute the grid of device points[d][n] (on stream) and pin it. */
+int clk_sum_grid_from_points(clk_context *ctx, uint64_t d, uint64_t n,
+                             const float *points, void *stream);
+/* Pin the grid of n points whose per-feature max |x| the caller knows
+ * (host_max_abs[d]; e.g. host-resident data streamed in chunks). */
+int clk_sum_grid_from_max(clk_context *ctx, uint64_t d, uint64_t n,
+                          const float *host_max_abs);
+/* Unpin: later ops compute the grid of their own points again. */
+int clk_sum_grid_clear(clk_context *ctx);
+/* The units (host_units[d]) an op of this context would use for
+ * (d, n, points): the pinned grid's, or the grid of these points.
+ * Synchronizes the stream. */
+int clk_sum_grid_units(clk_context *ctx, uint64_t d, uint64_t n,
+                       const float *points, double *host_units, void *stream);
 
 /* ---- the double / uint64 / uint64 type triple ------------------------- */
 /*

@@ -18,6 +18,11 @@
  *   ora_mass             integer invariants (sum == n, bincount).
  *   ora_sum_f64          math.fsum per cell, k == 1 closed form.
  *   ora_sum_kahan_f32    math.fsum within the Kahan bound.
+ *   ora_grid_scales,     Python exact rationals (fractions): the grid
+ *   ora_sum_grid         exponents by their definition, round-half-even grid
+ *                        integers summed exactly; the half-unit bound against
+ *                        math.fsum; special values; planted mistakes
+ *                        (truncation, an off-by-one exponent) must fail.
  *   ora_divide_*         IEEE division, 0/0 -> NaN case.
  *   ora_lloyd_*          sklearn KMeans (lloyd, given init), the R stats
  *                        kmeans_Lloyd built from the reference's
@@ -192,6 +197,90 @@ void ora_sum_kahan_f32(const float *pts, uint64_t n, uint64_t d,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Exact grid sums                                                           */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * The library's summation (include/clkmeans.h, "exact sums"); the reference
+ * has no counterpart (its FP32 sums follow its kernel geometry,
+ * lloyd_fused_cluster_merge.cl:188-241 + reduce_vector_parcol.cl).  Each
+ * coordinate is rounded to nearest-even on a per-feature power-of-two grid
+ * and the grid integers are added exactly, so the sum does not depend on
+ * the order.  Grid of feature f for n points:
+ *   M_f = max |x_f| over finite values; E_f = least integer with
+ *   M_f < 2^E_f, at least -100 (-100 for 0 or subnormal M_f);
+ *   b = min(25, 53 - bitlength(n)); scale_f = 2^(b - E_f); unit = 1/scale.
+ * A value with |x * scale| >= 2^25 (NaN, inf, or outside a grid computed
+ * from other data) is not added; its cell becomes NaN (NaN or finite),
+ * +inf / -inf (one sign of infinity) or NaN (both signs).
+ */
+uint32_t ora_grid_bits(uint64_t n)
+{
+    uint32_t lg = 0;
+    while (lg < 64 && (n >> lg) != 0)
+        ++lg;
+    int b = 53 - (int)lg;
+    return (uint32_t)(b < 25 ? b : 25);
+}
+
+void ora_grid_scales(const float *pts, uint64_t n, uint64_t d, uint64_t n_grid, float *scales)
+{
+    const int b = (int)ora_grid_bits(n_grid);
+    for (uint64_t f = 0; f < d; ++f) {
+        float m = 0.0f;
+        for (uint64_t p = 0; p < n; ++p) {
+            float a = fabsf(pts[f * n + p]);
+            if (a <= FLT_MAX && a > m)
+                m = a;
+        }
+        int e;
+        if (m < FLT_MIN) {
+            e = -100;  /* zero or subnormal */
+        } else {
+            (void)frexpf(m, &e);  /* m = fr * 2^e, fr in [0.5, 1): m < 2^e, m >= 2^(e-1) */
+            if (e < -100)
+                e = -100;
+        }
+        scales[f] = ldexpf(1.0f, b - e);
+    }
+}
+
+void ora_sum_grid(const float *pts, uint64_t n, uint64_t d, const uint32_t *labels,
+                  uint64_t k, const float *scales, double *sums)
+{
+    int64_t *acc = (int64_t *)calloc(k * d, sizeof(int64_t));
+    uint32_t *flg = (uint32_t *)calloc(k * d, sizeof(uint32_t));
+    for (uint64_t p = 0; p < n; ++p) {
+        uint32_t c = labels[p];
+        if (c >= k)
+            continue;
+        for (uint64_t f = 0; f < d; ++f) {
+            float x = pts[f * n + p];
+            float y = x * scales[f];  /* exact: a power-of-two scale */
+            if (fabsf(y) < 33554432.0f) {
+                acc[f * k + c] += (int64_t)rintf(y);  /* nearest, ties to even */
+            } else {
+                flg[f * k + c] |= x == INFINITY ? 2u : x == -INFINITY ? 4u : 1u;
+            }
+        }
+    }
+    for (uint64_t f = 0; f < d; ++f)
+        for (uint64_t c = 0; c < k; ++c) {
+            uint32_t fl = flg[f * k + c];
+            double v;
+            if (fl == 0)
+                v = (double)acc[f * k + c] * (1.0 / (double)scales[f]);
+            else if ((fl & 1u) || (fl & 6u) == 6u)
+                v = NAN;
+            else
+                v = (fl & 2u) ? INFINITY : -INFINITY;
+            sums[f * k + c] = v;
+        }
+    free(acc);
+    free(flg);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Division                                                                  */
 /* ------------------------------------------------------------------------ */
 
@@ -232,15 +321,17 @@ void ora_first_x(const float *pts, uint64_t n, uint64_t d, uint64_t k, float *ce
 /* Full Lloyd loop                                                           */
 /* ------------------------------------------------------------------------ */
 
-enum { ORA_MODE_FP64_MERGE = 0, ORA_MODE_NAIVE_KAHAN = 1 };
+enum { ORA_MODE_FP64_MERGE = 0, ORA_MODE_NAIVE_KAHAN = 1, ORA_MODE_GRID = 2 };
 
 /*
  * Fixed-iteration Lloyd loop: kmeans_single_stage.hpp:92-151 /
  * kmeans_three_stage.hpp:116-222 protocol (label, zero, mass, sum, divide,
  * swap), no convergence test (kmeans.converge is never read,
  * kmeans_three_stage.hpp:140).
- *   mode 0: centroid = (float)(FP64 sum / mass)  -- the GPU product's form.
+ *   mode 0: centroid = (float)(FP64 sum / mass), FP64 sums in point order.
  *   mode 1: Kahan FP32 sums / (float)mass        -- kmeans_naive.cpp:31-106.
+ *   mode 2: centroid = (float)(exact grid sum / mass) with the grid of all
+ *           n points (ora_sum_grid) -- the GPU product's form.
  * cen is in/out (k*d), labels (n) and masses (k) are out.
  */
 void ora_lloyd_f32(const float *pts, uint64_t n, uint64_t d, uint64_t k,
@@ -249,12 +340,18 @@ void ora_lloyd_f32(const float *pts, uint64_t n, uint64_t d, uint64_t k,
 {
     double *s64 = (double *)malloc(k * d * sizeof(double));
     float *s32 = (float *)malloc(k * d * sizeof(float));
+    float *sc = (float *)malloc(d * sizeof(float));
+    if (mode == ORA_MODE_GRID)
+        ora_grid_scales(pts, n, d, n, sc);
     for (uint32_t it = 0; it < iterations; ++it) {
         ora_label_f32(pts, n, d, cen, k, labels);
         ora_mass(labels, n, k, masses);
         if (mode == ORA_MODE_NAIVE_KAHAN) {
             ora_sum_kahan_f32(pts, n, d, labels, k, s32);
             ora_divide_f32(s32, masses, d, k, cen);
+        } else if (mode == ORA_MODE_GRID) {
+            ora_sum_grid(pts, n, d, labels, k, sc, s64);
+            ora_divide_from_f64(s64, masses, d, k, cen);
         } else {
             ora_sum_f64(pts, n, d, labels, k, s64);
             ora_divide_from_f64(s64, masses, d, k, cen);
@@ -262,6 +359,7 @@ void ora_lloyd_f32(const float *pts, uint64_t n, uint64_t d, uint64_t k,
     }
     free(s64);
     free(s32);
+    free(sc);
 }
 
 /* Double / uint64 type triple, FP64 throughout. */

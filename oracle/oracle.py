@@ -15,6 +15,7 @@ R_REF_PATH = os.path.join(HERE, "_ref", "libr_stats_kmeans.so")
 
 MODE_FP64_MERGE = 0
 MODE_NAIVE_KAHAN = 1
+MODE_GRID = 2
 
 _L = None
 
@@ -42,12 +43,16 @@ def lib():
             "ora_lloyd_f64": [vp, u64, u64, u64, vp, ctypes.c_uint32, vp, vp],
             "ora_lloyd_mt_f32": [vp, u64, u64, u64, vp, ctypes.c_uint32, vp, vp, i32],
             "ora_reduce_u32": [vp, u64, u64, vp],
+            "ora_grid_scales": [vp, u64, u64, u64, vp],
+            "ora_sum_grid": [vp, u64, u64, vp, u64, vp, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = None
         L.ora_lloyd_mt_f32.restype = i32
+        L.ora_grid_bits.argtypes = [u64]
+        L.ora_grid_bits.restype = ctypes.c_uint32
         L.ora_partition_matrix.argtypes = [vp, vp, u64, u64, u64]
         L.ora_partition_matrix.restype = ctypes.c_int64
         _L = L
@@ -130,6 +135,31 @@ def sums_kahan(points, labels, k):
     d, n = p.shape
     out = np.empty((d, k), np.float32)
     lib().ora_sum_kahan_f32(pp, n, d, lp, k, out.ctypes.data)
+    return out
+
+
+def grid_bits(n):
+    return int(lib().ora_grid_bits(n))
+
+
+def grid_scales(points, n_grid=None):
+    """Per-feature grid scales 2^(b - E_f) of points [d][n] (b from n_grid,
+    default n); the units are their reciprocals."""
+    p, pp = _c(points, np.float32)
+    d, n = p.shape
+    out = np.empty(d, np.float32)
+    lib().ora_grid_scales(pp, n, d, n if n_grid is None else n_grid, out.ctypes.data)
+    return out
+
+
+def sums_grid(points, labels, k, scales=None):
+    """Exact grid sums [d][k] (FP64) of float32 points with uint32 labels."""
+    p, pp = _c(points, np.float32)
+    l, lp = _c(labels, np.uint32)
+    d, n = p.shape
+    sc = grid_scales(p) if scales is None else np.ascontiguousarray(scales, np.float32)
+    out = np.empty((d, k), np.float64)
+    lib().ora_sum_grid(pp, n, d, lp, k, sc.ctypes.data, out.ctypes.data)
     return out
 
 

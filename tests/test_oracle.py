@@ -270,3 +270,216 @@ def test_partition_matrix_layout():
         for v in range(d):
             got = dst[b * buffer_size + v * real: b * buffer_size + (v + 1) * real]
             np.testing.assert_array_equal(got, s[v, b * per: b * per + real])
+
+
+# ---------------------------------------------------------------------------
+# The double / uint64 triple: ora_label_f64 pinned independently of its code
+
+
+def exact_label_f64(points, cen, fma=True, strict=True):
+    """Exact-rational restatement of the FP64 labeling: t = RN(x - c),
+    dist = RN(t*t + dist) (one rounding: the fma) from 0, strict '<' against
+    DBL_MAX, label 0 when nothing is smaller.  fma=False / strict=False are
+    the planted mistakes the checks below must catch."""
+    d, n = points.shape
+    k = cen.shape[1]
+    dbl_max = Fraction(np.finfo(np.float64).max.item())
+    rnd = lambda q: Fraction(float(q))  # Fraction -> float rounds to nearest even
+    out = []
+    for p in range(n):
+        best, lab = dbl_max, 0
+        for c in range(k):
+            dist = Fraction(0)
+            for f in range(d):
+                t = rnd(Fraction(float(points[f, p])) - Fraction(float(cen[f, c])))
+                dist = rnd(t * t + dist) if fma else rnd(rnd(t * t) + dist)
+            if dist < best or (not strict and dist == best):
+                best, lab = dist, c
+        out.append(lab)
+    return np.array(out, np.uint64)
+
+
+def near_tie_case_f64(rng, d, n, k):
+    """Points on (or within ~1e-16 relative of) the bisector of two centroids,
+    and a duplicated centroid (exact ties): the rounding sequence and the tie
+    rule decide their labels."""
+    cen = rng.standard_normal((d, k))
+    cen[:, k - 1] = cen[:, 0]  # exact duplicate: ties between 0 and k-1
+    pts = np.empty((d, n))
+    for p in range(n):
+        a, b = rng.choice(k, 2, replace=False)
+        mid = (cen[:, a] + cen[:, b]) / 2  # exact: equal true distances
+        noise = 0.0 if p % 2 == 0 else 1e-16
+        pts[:, p] = mid + rng.standard_normal(d) * noise * (1 + np.abs(mid))
+    return pts, cen
+
+
+def _check_label_f64(fn, seeds=range(30)):
+    """The checks that pin ora_label_f64: near-tie cases against the exact
+    restatement, plus R's double kmeans_Lloyd (reference tree) after one
+    iteration on random data with duplicated centroids."""
+    for s in seeds:
+        rng = np.random.default_rng(1000 + s)
+        d, k = int(rng.integers(1, 5)), int(rng.integers(3, 6))
+        pts, cen = near_tie_case_f64(rng, d, 10, k)
+        if not np.array_equal(fn(pts, cen), exact_label_f64(pts, cen)):
+            return False
+    if O.r_ref() is not None:
+        for s in seeds:
+            rng = np.random.default_rng(2000 + s)
+            d, n, k = int(rng.integers(1, 6)), 150, int(rng.integers(2, 9))
+            pts = rng.standard_normal((d, n)) * 5
+            cen = rng.standard_normal((d, k)) * 5
+            cen[:, -1] = cen[:, 0]
+            pts[:, :k] = cen  # points on duplicated centroids: exact ties
+            _, rl, _, _ = O.r_lloyd(pts, cen, 1, double=True)
+            if not np.array_equal(fn(pts, cen), rl.astype(np.uint64)):
+                return False
+    return True
+
+
+def test_label_f64_pinned_by_exact_rationals_and_r():
+    assert _check_label_f64(O.label_f64)
+
+
+def test_label_f64_checks_catch_planted_mistakes():
+    # the same checks run on the exact restatement with a planted '<=' and
+    # with the fma split into a rounded product and a rounded sum: both fail
+    assert not _check_label_f64(lambda p, c: exact_label_f64(p, c, strict=False), seeds=range(8))
+    assert not _check_label_f64(lambda p, c: exact_label_f64(p, c, fma=False), seeds=range(30))
+
+
+@pytest.mark.skipif(not os.path.exists(O.R_REF_PATH), reason="oracle/_ref not built (reference tree absent)")
+def test_lloyd_f64_against_r_double_lloyd():
+    """ora_lloyd_f64 against R's double kmeans_Lloyd from the reference tree:
+    labels and masses equal over 30 iterations, centroids to FP64 rounding."""
+    pts, _ = clkmeans.generate_blobs(20000, 5, 8, seed=11, sigma=2.0, box=10.0)
+    p64 = pts.astype(np.float64)
+    init = np.ascontiguousarray(p64[:, :8])
+    rc, rl, rn, _ = O.r_lloyd(p64, init, 30, double=True)
+    oc, ol, om = O.lloyd_f64(p64, init, 30)
+    np.testing.assert_array_equal(ol, rl.astype(np.uint64))
+    np.testing.assert_array_equal(om, rn.astype(np.uint64))
+    np.testing.assert_allclose(oc, rc, rtol=1e-12, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# Exact grid sums (the library's summation): pinned by their definition in
+# exact rational arithmetic
+
+
+def grid_scales_by_definition(points, n_grid):
+    b = min(25, 53 - int(n_grid).bit_length())
+    out = []
+    for row in np.asarray(points, np.float32):
+        fin = np.abs(row[np.isfinite(row)]).astype(np.float64)
+        m = Fraction(float(fin.max())) if fin.size else Fraction(0)
+        if m < Fraction(2) ** -126:
+            e = -100
+        else:
+            e = -100
+            while Fraction(2) ** e <= m:  # least e with m < 2^e
+                e += 1
+        out.append(Fraction(2) ** (b - max(e, -100)))
+    return out
+
+
+def round_half_even(q):
+    lo = math.floor(q)
+    rem = q - lo
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and lo % 2 == 1):
+        lo += 1
+    return lo
+
+
+def grid_sums_exact(points, labels, k, scales, trunc=False):
+    d, n = points.shape
+    out = np.zeros((d, k))
+    for f in range(d):
+        sc = Fraction(float(scales[f]))
+        for c in range(k):
+            vals = points[f, labels == c]
+            tot = 0
+            for x in vals:
+                y = Fraction(float(x)) * sc
+                tot += int(y) if trunc else round_half_even(y)
+            out[f, c] = float(Fraction(tot) / sc)
+    return out
+
+
+def test_grid_scales_match_definition():
+    rng = np.random.default_rng(31)
+    cases = []
+    for e in (-140, -60, -3, 0, 5, 40, 120):
+        cases.append(((rng.standard_normal((3, 50)) * 2.0 ** e).astype(np.float32), 50))
+    z = np.zeros((2, 9), np.float32)
+    z[1, 3] = 1e-45  # subnormal maximum
+    cases.append((z, 9))
+    sp = rng.standard_normal((2, 40)).astype(np.float32)
+    sp[0, 5], sp[1, 7], sp[1, 8] = np.nan, np.inf, -np.inf  # ignored by the maximum
+    cases.append((sp, 40))
+    pw = np.array([[1.0, -0.5, 0.25], [4.0, 3.9999998, 2.0]], np.float32)  # maxima on powers of two
+    cases.append((pw, 3))
+    for n_grid in (1, 2 ** 27, 2 ** 28 - 1, 2 ** 28, 2 ** 31):
+        cases.append((sp, n_grid))
+    for pts, n_grid in cases:
+        got = O.grid_scales(pts, n_grid)
+        want = grid_scales_by_definition(pts, n_grid)
+        assert [Fraction(float(g)) for g in got] == want
+
+
+def test_grid_sums_match_exact_integers_and_bound():
+    rng = np.random.default_rng(32)
+    for trial in range(12):
+        d, n, k = int(rng.integers(1, 4)), int(rng.integers(50, 400)), int(rng.integers(1, 6))
+        pts = (rng.standard_normal((d, n)) * 10.0 ** rng.integers(-3, 4)).astype(np.float32)
+        sc = O.grid_scales(pts)
+        # values exactly half a unit off the grid: round-half-even decides them
+        for f in range(d):
+            unit = 1.0 / float(sc[f])
+            pts[f, :8] = (np.arange(8) - 3.5) * unit * 3.0
+        labels = rng.integers(0, k + 1, n).astype(np.uint32)  # label k: ignored
+        got = O.sums_grid(pts, labels, k, sc)
+        want = grid_sums_exact(pts, labels, k, sc)
+        np.testing.assert_array_equal(got, want)
+        for f in range(d):
+            unit = 1.0 / float(sc[f])
+            for c in range(k):
+                cell = pts[f, labels == c]
+                assert abs(got[f, c] - math.fsum(float(x) for x in cell)) <= cell.size * unit / 2
+
+
+def test_grid_sums_special_values_and_planted_mistakes():
+    pts = np.array([[1.0, np.nan, 2.0, 3.0, np.inf, -np.inf, 5.0, np.inf]], np.float32)
+    labels = np.array([0, 0, 1, 2, 2, 3, 3, 4], np.uint32)
+    sc = O.grid_scales(pts)
+    s = O.sums_grid(pts, labels, 5, sc)[0]
+    assert np.isnan(s[0]) and s[1] == 2.0 and s[2] == np.inf and s[3] == -np.inf and s[4] == np.inf
+    both = np.array([[np.inf, -np.inf]], np.float32)
+    assert np.isnan(O.sums_grid(both, np.zeros(2, np.uint32), 1, np.ones(1, np.float32))[0, 0])
+    # a finite value outside the grid it is summed on: NaN
+    assert np.isnan(O.sums_grid(np.array([[1e6]], np.float32), np.zeros(1, np.uint32), 1,
+                                np.array([2.0 ** 30], np.float32))[0, 0])
+    # planted mistakes: truncation instead of round-half-even, and a grid one
+    # exponent too coarse, give other sums on the same data
+    rng = np.random.default_rng(33)
+    pts = rng.standard_normal((2, 300)).astype(np.float32)
+    labels = rng.integers(0, 3, 300).astype(np.uint32)
+    sc = O.grid_scales(pts)
+    got = O.sums_grid(pts, labels, 3, sc)
+    assert not np.array_equal(got, grid_sums_exact(pts, labels, 3, sc, trunc=True))
+    assert not np.array_equal(got, grid_sums_exact(pts, labels, 3, sc / 2))
+
+
+def test_grid_lloyd_close_to_fp64_merge_and_order_free():
+    pts, _ = clkmeans.generate_blobs(30000, 4, 10, seed=6)
+    init = O.first_x(pts, 10)
+    a = O.lloyd(pts, init, 10)
+    g = O.lloyd(pts, init, 10, mode=O.MODE_GRID)
+    np.testing.assert_array_equal(a[1], g[1])
+    np.testing.assert_allclose(a[0], g[0], rtol=1e-6)
+    # the sums of a permutation of the points are the same bits
+    perm = np.random.default_rng(7).permutation(pts.shape[1])
+    sc = O.grid_scales(pts)
+    np.testing.assert_array_equal(O.sums_grid(pts, g[1], 10, sc),
+                                  O.sums_grid(pts[:, perm], g[1][perm], 10, sc))
