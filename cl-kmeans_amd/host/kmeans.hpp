@@ -7,10 +7,13 @@
 // per-stage platform/device placement collapses to one queue.
 #pragma once
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <functional>
 #include <memory>
+#include <thread>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -18,6 +21,51 @@
 #include "operators.hpp"
 
 namespace Clustering {
+
+// The sum grid of a run's data set (clkmeans.h, "exact sums"), pinned on the
+// context for the run and cleared after it: every op of the run sums
+// exactly on the grid of all n points, so the chunked strategies give the
+// unchunked sums and no op re-reads the points to find a grid.  The double
+// triple sums in FP64 and has no grid.
+template <typename PointT>
+void pin_sum_grid(const ContextHandle &ctx, const Queue &q, size_t d, size_t n, const PointT *device_points)
+{
+    if constexpr (std::is_same<PointT, float>::value)
+        check_status(clk_sum_grid_from_points(ctx.get(), d, n, device_points, q.get()), "clk_sum_grid_from_points");
+}
+
+// The same from host-resident points [d][n]: per-feature max |x| over the
+// finite values, on the host cores.
+template <typename PointT>
+void pin_sum_grid_host(const ContextHandle &ctx, size_t d, size_t n, const std::vector<PointT> &hp)
+{
+    if constexpr (std::is_same<PointT, float>::value) {
+        std::vector<float> mx(d, 0.0f);
+        const size_t nt = std::max<size_t>(1, std::min<size_t>(std::thread::hardware_concurrency(), 64));
+        std::vector<std::vector<float>> part(nt, std::vector<float>(d, 0.0f));
+        std::vector<std::thread> th;
+        const size_t per = (n + nt - 1) / nt;
+        for (size_t t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                const size_t p0 = std::min(n, t * per), p1 = std::min(n, p0 + per);
+                for (size_t f = 0; f < d; ++f) {
+                    float m = 0.0f;
+                    const float *row = hp.data() + f * n;
+                    for (size_t p = p0; p < p1; ++p) {
+                        const float a = std::fabs(row[p]);
+                        m = (a <= 3.4028234663852886e38f && a > m) ? a : m;
+                    }
+                    part[t][f] = m;
+                }
+            });
+        for (auto &x : th)
+            x.join();
+        for (size_t t = 0; t < nt; ++t)
+            for (size_t f = 0; f < d; ++f)
+                mx[f] = std::max(mx[f], part[t][f]);
+        check_status(clk_sum_grid_from_max(ctx.get(), d, n, mx.data()), "clk_sum_grid_from_max");
+    }
+}
 
 template <typename PointT, typename LabelT, typename MassT, bool ColMajor = true>
 class AbstractKmeans {
@@ -161,6 +209,7 @@ public:
         buf.setup(queue, d, n, k, *this->host_points, *this->host_centroids, m);
         if (this->centroids_initializer)
             this->centroids_initializer(queue, d, n, k, buf.points.data(), buf.centroids.data());
+        pin_sum_grid(context, queue, d, n, buf.points.data());
         queue.finish();
 
         auto t0 = std::chrono::steady_clock::now();
@@ -180,6 +229,7 @@ public:
         auto ns = std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0);
         m.add_datapoint().set_name("TotalTime").add_value() = (uint64_t)ns.count();
         buf.download(queue, d, n, k, *this->host_centroids, *this->host_labels, *this->host_masses);
+        check_status(clk_sum_grid_clear(context.get()), "clk_sum_grid_clear");
     }
 
 private:
@@ -223,6 +273,7 @@ public:
         matrix_divide.prepare(context, MatrixBinaryOp<PointT, MassT>::Divide);
         if (this->centroids_initializer)
             this->centroids_initializer(queue, d, n, k, buf.points.data(), buf.centroids.data());
+        pin_sum_grid(context, queue, d, n, buf.points.data());
         queue.finish();
 
         auto t0 = std::chrono::steady_clock::now();
@@ -253,6 +304,7 @@ public:
         auto ns = std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0);
         m.add_datapoint().set_name("TotalTime").add_value() = (uint64_t)ns.count();
         buf.download(queue, d, n, k, *this->host_centroids, *this->host_labels, *this->host_masses);
+        check_status(clk_sum_grid_clear(context.get()), "clk_sum_grid_clear");
     }
 
 private:
@@ -308,6 +360,7 @@ public:
         if (this->centroids_initializer)
             this->centroids_initializer(queue, d, std::min(n, cache.chunk_points()), k, cache.first_chunk(queue),
                                         st.centroids.data());
+        pin_sum_grid_host(context, d, n, *this->host_points);
         queue.finish();
         this->measurement->set_parameter("BufferSize", std::to_string(buffer_size));
 
@@ -341,6 +394,7 @@ public:
         st.masses.copy_to_host(this->host_masses->data(), k, queue);
         scheduler.finish();
         cache.copy_labels_out(*this->host_labels);
+        check_status(clk_sum_grid_clear(context.get()), "clk_sum_grid_clear");
     }
 
 private:
@@ -400,6 +454,7 @@ public:
         if (this->centroids_initializer)
             this->centroids_initializer(queue, d, std::min(n, cache.chunk_points()), k, cache.first_chunk(queue),
                                         st.centroids.data());
+        pin_sum_grid_host(context, d, n, *this->host_points);
         queue.finish();
         this->measurement->set_parameter("BufferSize", std::to_string(buffer_size));
 
@@ -436,6 +491,7 @@ public:
         st.masses.copy_to_host(this->host_masses->data(), k, queue);
         scheduler.finish();
         cache.copy_labels_out(*this->host_labels);
+        check_status(clk_sum_grid_clear(context.get()), "clk_sum_grid_clear");
     }
 
 private:

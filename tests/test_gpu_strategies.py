@@ -29,7 +29,7 @@ def conf(pipeline, iterations=30, k=10, **stage):
 def data():
     pts, _ = clkmeans.generate_blobs(1 << 18, 4, 10, sigma=1.0, box=50.0, seed=0)
     init = O.first_x(pts, 10)
-    return pts, init, O.lloyd(pts, init, 30)
+    return pts, init, O.lloyd(pts, init, 30), O.lloyd(pts, init, 30, mode=O.MODE_GRID)
 
 
 CASES = [
@@ -53,7 +53,7 @@ CASES = [
 
 @pytest.mark.parametrize("pipeline,stages", CASES, ids=["%s-%d" % (c[0], i) for i, c in enumerate(CASES)])
 def test_strategy_matches_oracle(data, pipeline, stages):
-    pts, init, (rc, rl, rm) = data
+    pts, init, (rc, rl, rm), (gc, gl, gm) = data
     # 3 chunks plus a ragged tail for the buffered pipelines
     c, l, m, it, ns = clkmeans.kmeans_run(conf(pipeline, **stages), pts, init, buffer_size=4 * 4 * 100000)
     assert it == 30 and ns > 0
@@ -61,6 +61,10 @@ def test_strategy_matches_oracle(data, pipeline, stages):
     np.testing.assert_array_equal(m, rm)
     ref64 = O.sums_f64(pts, rl, 10) / rm.astype(np.float64)
     assert centroid_rel_err(c, ref64, pts, rl) <= 1e-5
+    # every strategy, chunked or not: the exact grid sums, so the oracle's
+    # grid loop bit for bit
+    np.testing.assert_array_equal(l, gl)
+    np.testing.assert_array_equal(c, gc)
 
 
 DOUBLE = "types.point = double\ntypes.label = uint64\ntypes.mass = uint64\n"
@@ -108,7 +112,7 @@ def test_dense_three_stage_matches_oracle(labeling):
 
 
 def test_convergence_tolerance_stops_early(data):
-    pts, init, _ = data
+    pts, init = data[0], data[1]
     c, l, m, it, _ = clkmeans.kmeans_run(conf("single_stage", iterations=50), pts, init, tolerance=1e-4)
     assert 1 <= it < 50
     ref_c, ref_l, ref_m = O.lloyd(pts, init, it)
@@ -116,7 +120,7 @@ def test_convergence_tolerance_stops_early(data):
 
 
 def test_unknown_strategy_raises_invalid_argument(data):
-    pts, init, _ = data
+    pts, init = data[0], data[1]
     with pytest.raises(ValueError):
         clkmeans.kmeans_run(conf("single_stage", fused={"strategy": "no_such"}), pts, init)
     with pytest.raises(ValueError):
