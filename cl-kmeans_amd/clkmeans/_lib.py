@@ -136,6 +136,7 @@ def load(path=None):
         "clk_fused": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp, vp, vp]),
         "clk_labeling": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp]),
         "clk_lloyd_step": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "clk_lloyd_step_noalloc": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp, vp, vp, vp, vp]),
         "clk_mass_update": (i32, [vp, cfgp, u64, u64, vp, vp, vp]),
         "clk_centroid_update": (i32, [vp, cfgp, u64, u64, u64, vp, vp, vp, vp, vp]),
         "clk_matrix_divide_row": (i32, [vp, u64, u64, vp, vp, vp, vp, vp, vp]),
@@ -321,6 +322,13 @@ class Context:
         check(fn(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),
                                     _p(old_centroids), _p(new_centroids), _p(sums), _p(labels), _p(masses),
                                     _p(max_shift), _s(stream)))
+
+    def lloyd_step_noalloc(self, cfg, d, n, k, points, old_centroids, new_centroids, sums, labels, masses,
+                           max_shift=None, stream=None):
+        """clk_lloyd_step_noalloc: the same iteration with no device allocation."""
+        check(self.L.clk_lloyd_step_noalloc(self.handle, ctypes.byref(cfg) if cfg else None, d, n, k, _p(points),
+                                            _p(old_centroids), _p(new_centroids), _p(sums), _p(labels),
+                                            _p(masses), _p(max_shift), _s(stream)))
 
     def labeling_refined_points(self):
         """Points of the last MFMA labeling call that went through the exact refinement."""

@@ -70,6 +70,7 @@ struct clk_context {
     uint32_t *mfma_qlen = nullptr;  // device queue length inside mfma_tables
     clk::Scratch grid;            // sum grid: per-feature max |x| bits [d] (clk_grid.hpp)
     bool grid_pinned = false;     // grid holds a caller-pinned grid for grid_d features
+    char *noalloc_scratch = nullptr;  // this device's static scratch (clk_lloyd_step_noalloc)
     uint64_t grid_d = 0, grid_n = 0;
     int mfma_subnormals = -1;     // bit 0 FP16, bit 1 BF16: MFMA keeps subnormal inputs (-1 unknown)
     bool profiling = false;

@@ -28,6 +28,7 @@
 #include "clk_kernels.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace clk {
 namespace {
@@ -300,10 +301,18 @@ static int block_size(clk_context *ctx, const clk_kernel_config *cfg)
 
 // mode 1: sums/masses accumulate (clk_fused).  mode 2: sums/masses are
 // overwritten and new_centroids = divide (clk_lloyd_step).
+// Caller-provided device scratch (clk_lloyd_step_noalloc): partial rows and
+// the per-call sum grid are carved from it instead of the context's
+// allocations.
+struct ExtScratch {
+    char *ptr;
+    size_t bytes;
+};
+
 int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64_t n, uint64_t k,
               const float *points, const float *old_centroids, double *sums, uint32_t *labels,
               uint32_t *masses, hipStream_t stream, uint32_t mode, float *new_centroids,
-              double *max_shift)
+              double *max_shift, const ExtScratch *ext = nullptr)
 {
     CLK_TRY(check_sizes(d, n, k));
     CLK_REQUIRE(points && old_centroids && sums && labels && masses, CLK_ERR_INVALID_ARGUMENT,
@@ -340,6 +349,8 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
     const uint64_t priv_bytes = priv_plain + (run_bounded ? thr_bytes : 0);
     bool use_shared = dmax && !use_private && k * d < (1ull << 24) && shared_bytes <= ctx->lds_per_block;
 
+    CLK_REQUIRE(!ext || use_private || use_shared || n == 0, CLK_ERR_UNSUPPORTED,
+                "lloyd_step_noalloc: k*d beyond the work-group LDS tiles (the separate stages allocate)");
     if ((!use_private && !use_shared) || n == 0) {
         if (mode == 2) {
             CLK_TRY(clk_fill_zero(ctx, sums, d * k * sizeof(double), stream));
@@ -371,13 +382,37 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
     if (lds > 65536)
         CLK_HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     GridRef sgrid;
-    CLK_TRY(resolve_grid(ctx, d, n, points, stream, &sgrid));
-
-    CLK_TRY(ensure_scratch(ctx->part_sums, grid * d * k * sizeof(double)));
-    CLK_TRY(ensure_scratch(ctx->part_masses, grid * k * sizeof(uint32_t)));
+    double *psums = nullptr;
+    uint32_t *pmasses = nullptr;
+    if (ext) {
+        // [per-call grid d words, 256-aligned] [partial sums] [partial masses]:
+        // as many work-groups as the rows that fit (grid-stride: any count
+        // gives the same results)
+        const size_t head = ctx->grid_pinned ? 0 : (d * sizeof(uint32_t) + 255) / 256 * 256;
+        const size_t row = d * k * sizeof(double) + k * sizeof(uint32_t);
+        const size_t fit = ext->bytes > head ? (ext->bytes - head) / row : 0;
+        CLK_REQUIRE(fit >= 1, CLK_ERR_SIZE, "lloyd_step_noalloc: k*(8d+4) exceeds the static scratch");
+        grid = std::min<uint64_t>(grid, fit);
+        if (ctx->grid_pinned) {
+            CLK_TRY(resolve_grid(ctx, d, n, points, stream, &sgrid));
+        } else {
+            uint32_t *gmax = reinterpret_cast<uint32_t *>(ext->ptr);
+            CLK_TRY(launch_grid_max(ctx, d, n, points, gmax, stream));
+            sgrid.gmax = gmax;
+            sgrid.bits = grid_bits(n);
+        }
+        psums = reinterpret_cast<double *>(ext->ptr + head);
+        pmasses = reinterpret_cast<uint32_t *>(psums + grid * d * k);
+    } else {
+        CLK_TRY(resolve_grid(ctx, d, n, points, stream, &sgrid));
+        CLK_TRY(ensure_scratch(ctx->part_sums, grid * d * k * sizeof(double)));
+        CLK_TRY(ensure_scratch(ctx->part_masses, grid * k * sizeof(uint32_t)));
+        psums = static_cast<double *>(ctx->part_sums.ptr);
+        pmasses = static_cast<uint32_t *>(ctx->part_masses.ptr);
+    }
     FinalizeArgs fa;
-    fa.part_sums = static_cast<double *>(ctx->part_sums.ptr);
-    fa.part_masses = static_cast<uint32_t *>(ctx->part_masses.ptr);
+    fa.part_sums = psums;
+    fa.part_masses = pmasses;
     fa.rows = (uint32_t)grid;
     fa.sums = sums;
     fa.masses = masses;
@@ -394,8 +429,7 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
                                    : "FusedClusterMerge",
                       stream);
         hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(B), lds, stream, points, n, (uint32_t)d,
-                           (uint32_t)k, old_centroids, labels, sgrid, static_cast<double *>(ctx->part_sums.ptr),
-                           static_cast<uint32_t *>(ctx->part_masses.ptr));
+                           (uint32_t)k, old_centroids, labels, sgrid, psums, pmasses);
         CLK_HIP_CHECK(hipGetLastError());
     }
     const size_t fin_lds = d * k * sizeof(double) + k * sizeof(uint32_t) + 12 * 1024 + 16;
@@ -411,6 +445,39 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
     if (mode == 2)
         return clk_matrix_divide_row(ctx, d, k, sums, masses, new_centroids,
                                      max_shift ? old_centroids : nullptr, max_shift, stream);
+    return CLK_OK;
+}
+
+// ---- the entry point that allocates nothing -----------------------------
+
+namespace {
+
+// Scratch of clk_lloyd_step_noalloc: partial rows and the per-call sum grid.
+// Every GPU the library runs on holds one copy until the process ends.  In
+// host code the name is a host-side stand-in: kernels and fills take the
+// device address hipGetSymbolAddress returns (scratch_dev below).
+static __device__ __align__(256) unsigned char scratch[CLK_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_dev[kMaxDevices] = {};
+
+}  // namespace
+
+// Once per device, with that device current (clk_context_create calls it):
+// the device address of this device's copy of `scratch`.
+int noalloc_setup(clk_context *ctx)
+{
+    CLK_REQUIRE(ctx->device >= 0 && ctx->device < kMaxDevices, CLK_ERR_UNSUPPORTED,
+                "noalloc scratch: device index beyond the table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[ctx->device]) {
+        void *p = nullptr;
+        CLK_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        CLK_REQUIRE(p != nullptr, CLK_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_dev[ctx->device] = static_cast<char *>(p);
+    }
+    ctx->noalloc_scratch = g_scratch_dev[ctx->device];
     return CLK_OK;
 }
 
@@ -437,4 +504,24 @@ extern "C" int clk_lloyd_step(clk_context *ctx, const clk_kernel_config *cfg, ui
                 "clk_lloyd_step: config is not a fused-stage config");
     return clk::run_fused(ctx, cfg, d, n, k, points, old_centroids, sums, labels, masses,
                           clk::as_stream(stream), 2, new_centroids, max_shift);
+}
+
+extern "C" int clk_lloyd_step_noalloc(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
+                                      uint64_t n, uint64_t k, const float *points,
+                                      const float *old_centroids, float *new_centroids, double *sums,
+                                      uint32_t *labels, uint32_t *masses, double *max_shift,
+                                      void *stream)
+{
+    CLK_TRY(clk::enter(ctx));
+    CLK_REQUIRE(!cfg || cfg->stage == CLK_STAGE_FUSED, CLK_ERR_INVALID_ARGUMENT,
+                "clk_lloyd_step_noalloc: config is not a fused-stage config");
+    CLK_REQUIRE(ctx->noalloc_scratch != nullptr, CLK_ERR_HIP, "clk_lloyd_step_noalloc: no scratch address");
+    const clk::ExtScratch ext{ctx->noalloc_scratch, CLK_NOALLOC_SCRATCH_BYTES};
+    // no timing records: taking an event may create one
+    const bool prof = ctx->profiling;
+    ctx->profiling = false;
+    const int rc = clk::run_fused(ctx, cfg, d, n, k, points, old_centroids, sums, labels, masses,
+                                  clk::as_stream(stream), 2, new_centroids, max_shift, &ext);
+    ctx->profiling = prof;
+    return rc;
 }

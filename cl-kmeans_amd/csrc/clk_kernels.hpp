@@ -66,4 +66,8 @@ int launch_accumulate_global(clk_context *ctx, uint64_t d, uint64_t n, uint64_t 
                              const float *points, const uint32_t *labels, const GridRef &grid,
                              double *sums, uint32_t *masses, hipStream_t stream);
 
+// Device address of this device's static scratch of clk_lloyd_step_noalloc
+// into ctx->noalloc_scratch (hipGetSymbolAddress once per device).
+int noalloc_setup(clk_context *ctx);
+
 }  // namespace clk

@@ -446,6 +446,11 @@ extern "C" int clk_context_create(int device, clk_context **out)
     ctx->num_cus = prop.multiProcessorCount;
     ctx->lds_per_block = prop.sharedMemPerBlock;
     ctx->max_threads_per_block = prop.maxThreadsPerBlock;
+    const int rc = noalloc_setup(ctx);
+    if (rc != CLK_OK) {
+        delete ctx;
+        return rc;
+    }
     *out = ctx;
     return CLK_OK;
 }

@@ -172,6 +172,27 @@ int clk_lloyd_step(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
                    uint32_t *labels, uint32_t *masses, double *max_shift, void *stream);
 
 /*
+ * No counterpart in the reference: clk_lloyd_step for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and stream
+ * semantics as clk_lloyd_step; it calls no hipMalloc / hipFree.  Its scratch
+ * (per-work-group partial sums and masses, and the sum grid of the call's
+ * points unless one is pinned) lives in one static device array of
+ * CLK_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; the
+ * persistent grid is clamped to the partial rows that fit.  CLK_ERR_SIZE when
+ * one row, k*(8d+4) bytes, does not fit; CLK_ERR_UNSUPPORTED when k*d needs
+ * the separate labeling / mass / centroid stages (beyond the LDS tiles).
+ * Every call on one GPU shares the array and leaves its contents there:
+ * calls that overlap in time on one GPU race (order them on one stream).
+ * No profiling records are taken.
+ */
+#define CLK_NOALLOC_SCRATCH_BYTES (8u << 20)
+int clk_lloyd_step_noalloc(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
+                           uint64_t n, uint64_t k, const float *points,
+                           const float *old_centroids, float *new_centroids, double *sums,
+                           uint32_t *labels, uint32_t *masses, double *max_shift, void *stream);
+
+/*
  * Labeling only.  Replaces LabelingFactory::LabelingFunction
  * (labeling_factory.hpp:33-48) / LabelingUnrollVector
  * (labeling_unroll_vector.hpp:168-272): same arithmetic as clk_fused.
