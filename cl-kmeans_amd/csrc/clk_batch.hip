@@ -433,7 +433,9 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
 #pragma unroll
         for (int f = 0; f < DMAX; ++f)
             xn[f] = pts[(uint64_t)min((uint32_t)f, d - 1) * n + pc];
-        ln = prev_labels[pc];
+        // previous labels only matter to jobs with the keep test; the
+        // others read one cached word instead of streaming the labels
+        ln = prev_labels[keep_ok ? pc : first];
     };
     fetch(first);
     for (uint64_t base = first; base < end; base += blockDim.x) {
@@ -861,7 +863,7 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
                 const float v = gpts[(uint64_t)min((uint32_t)f, d - 1) * n + pc];
                 xc[f] = f < (int)d ? v : 0.0f;
             }
-            prev = glabels[pc];
+            prev = glabels[keep_ok ? pc : first];  // one cached word without the keep test
         }
         // The keep test runs before the next batch's loads are issued: loads
         // complete in order, so a load issued after them (the centroid row,

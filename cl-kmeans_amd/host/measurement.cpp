@@ -110,11 +110,43 @@ std::string Measurement::format_filename(const std::string &base, const std::str
     return dir + get_datetime() + "_" + id + "_" + stem + suffix + ext;
 }
 
-std::deque<const DataPoint *> Measurement::flattened() const
+// The rows of *_mnts.csv: every top-level data point, preceded, when it
+// has children (the chunk pipeline's per-chunk copies and kernels), by a
+// "<name>#Sub" point holding the sum of the children's values
+// (measurement/measurement.cpp:369-390).
+std::deque<DataPoint> Measurement::flattened() const
 {
-    std::deque<const DataPoint *> out;
+    std::deque<DataPoint> out;
+    for (const DataPoint &dp : data_points_) {
+        if (!dp.children_.empty()) {
+            DataPoint sub = dp.is_iterative() ? DataPoint(dp.get_iteration()) : DataPoint();
+            sub.set_name(dp.get_name() + "#Sub");
+            uint64_t v = 0;
+            for (const DataPoint &c : dp.children_)
+                v += c.get_value();
+            sub.add_value() = v;
+            out.push_back(sub);
+        }
+        out.push_back(dp);
+    }
+    return out;
+}
+
+// Every data point with events, children included, depth first
+// (measurement/measurement.cpp:392-412).
+std::deque<const DataPoint *> Measurement::with_events() const
+{
+    std::deque<const DataPoint *> out, stack;
     for (const DataPoint &dp : data_points_)
-        out.push_back(&dp);
+        stack.push_back(&dp);
+    while (!stack.empty()) {
+        const DataPoint *dp = stack.front();
+        stack.pop_front();
+        if (dp->num_events() > 0)
+            out.push_back(dp);
+        for (const DataPoint &c : dp->children_)
+            stack.push_front(&c);
+    }
     return out;
 }
 
@@ -145,11 +177,11 @@ std::vector<std::string> Measurement::write_csv(const std::string &filename)
     {
         std::ofstream mf(mf_name, std::ios_base::out | std::ios::trunc);
         mf << "ExperimentID,Run,TypeName,Iteration,Value\n";
-        for (const DataPoint *dp : flattened()) {
-            mf << id << ',' << run_ << ',' << dp->get_name() << ',';
-            if (dp->is_iterative())
-                mf << dp->get_iteration();
-            mf << ',' << dp->get_value() << '\n';
+        for (const DataPoint &dp : flattened()) {
+            mf << id << ',' << run_ << ',' << dp.get_name() << ',';
+            if (dp.is_iterative())
+                mf << dp.get_iteration();
+            mf << ',' << dp.get_value() << '\n';
         }
     }
     files.push_back(mf_name);
@@ -160,19 +192,20 @@ std::vector<std::string> Measurement::write_csv(const std::string &filename)
     {
         std::ofstream ef(ev_name, std::ios_base::out | std::ios::trunc);
         ef << "ExperimentID,Run,TypeName,Iteration,CommandQueueID,Queued,Submit,Start,End\n";
+        const std::deque<const DataPoint *> evs = with_events();
         hipEvent_t epoch = nullptr;
-        for (const DataPoint &dp : data_points_)
-            if (!dp.events_.empty() && dp.events_.front().start) {
-                epoch = dp.events_.front().start.get();
+        for (const DataPoint *dp : evs)
+            if (dp->events_.front().start) {
+                epoch = dp->events_.front().start.get();
                 break;
             }
-        for (const DataPoint &dp : data_points_) {
-            for (const TimedEvent &e : dp.events_) {
+        for (const DataPoint *dp : evs) {
+            for (const TimedEvent &e : dp->events_) {
                 uint64_t s = (epoch && e.start) ? elapsed_ns(epoch, e.start.get()) : 0;
                 uint64_t t = (epoch && e.end) ? elapsed_ns(epoch, e.end.get()) : 0;
-                ef << id << ',' << run_ << ',' << dp.get_name() << ',';
-                if (dp.is_iterative())
-                    ef << dp.get_iteration();
+                ef << id << ',' << run_ << ',' << dp->get_name() << ',';
+                if (dp->is_iterative())
+                    ef << dp->get_iteration();
                 ef << ",0," << s << ',' << s << ',' << s << ',' << t << '\n';
             }
         }

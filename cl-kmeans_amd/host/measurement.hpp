@@ -103,7 +103,8 @@ private:
     std::string get_datetime() const;
     std::string format_filename(const std::string &base, const std::string &id,
                                 const std::string &suffix) const;
-    std::deque<const DataPoint *> flattened() const;
+    std::deque<DataPoint> flattened() const;
+    std::deque<const DataPoint *> with_events() const;
 
     int run_ = 0;
     std::deque<DataPoint> data_points_;

@@ -12,9 +12,16 @@
  * cen[f * k + c]  (lloyd_fused_feature_sum.cl:230-232).
  *
  * Pinning status of each function (what checks it besides itself):
- *   ora_label_*,         exact-rational brute force on tiny inputs with
+ *   ora_label_f32,       exact-rational brute force on tiny inputs with
  *   ora_dist_f32         adversarial near-ties (tests/test_oracle.py), hand
  *                        ties / NaN / inf cases, sklearn on separated blobs.
+ *   ora_label_f64        exact-rational FP64 restatement on points on the
+ *                        bisectors of centroid pairs and on duplicated
+ *                        centroids; the reference tree's R double
+ *                        kmeans_Lloyd (oracle/_ref) after one iteration on
+ *                        30 random cases with exact ties; a planted '<=' and
+ *                        a split fma both fail those checks.
+ *   ora_lloyd_f64        the R double kmeans_Lloyd over 30 iterations.
  *   ora_mass             integer invariants (sum == n, bincount).
  *   ora_sum_f64          math.fsum per cell, k == 1 closed form.
  *   ora_sum_kahan_f32    math.fsum within the Kahan bound.
@@ -32,7 +39,8 @@
  *   ora_first_x          closed form of kmeans_initializer.cpp:34-38.
  *   ora_partition_matrix byte-exact chunk layout of buffer_helper.cpp:58-84,
  *                        checked against a slicing restatement in numpy.
- *   ora_lloyd_mt_f32     equality with ora_lloyd_f32 (mode 0).
+ *   ora_lloyd_mt_f32,    equality with ora_lloyd_f32 (mode 0): labels and
+ *   ora_lloyd3_mt_f32    masses exact, centroids to FP64 summation order.
  *
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off): fmaf() is called
  * explicitly where the reference's kernels call fma(); nothing else may be
@@ -440,6 +448,95 @@ int ora_lloyd_mt_f32(const float *pts, uint64_t n, uint64_t d, uint64_t k,
         }
         for (int t = 0; t < threads; ++t)
             pthread_join(tid[t], NULL);
+        memset(tot, 0, sizeof(double) * k * d);
+        for (uint64_t c = 0; c < k; ++c)
+            masses[c] = 0;
+        for (int t = 0; t < threads; ++t) {
+            for (uint64_t i = 0; i < k * d; ++i)
+                tot[i] += sums[(size_t)t * k * d + i];
+            for (uint64_t c = 0; c < k; ++c)
+                masses[c] += (uint32_t)ms[(size_t)t * k + c];
+        }
+        ora_divide_from_f64(tot, masses, d, k, cen);
+    }
+    free(tid);
+    free(jobs);
+    free(sums);
+    free(ms);
+    free(tot);
+    return 0;
+}
+
+/*
+ * The three-stage CPU strategy, the reference's
+ * configurations/intel_core_i7-6700K_three_stage.conf:15-41 (unroll_vector
+ * labeling, part_private mass update, cluster_merge centroid update), one
+ * pass over the points per stage, as kmeans_three_stage.hpp:116-222 runs
+ * them: each stage splits the points into contiguous per-thread blocks; the
+ * mass stage keeps a private histogram per thread (mass_update_part_private
+ * merged in thread order), the centroid stage a private k*d FP64 tile per
+ * thread (lloyd_cluster_merge.cl, merged in thread order).  Results equal
+ * ora_lloyd_f32 (mode 0) up to the FP64 summation order.
+ */
+typedef struct {
+    const float *pts;
+    const float *cen;
+    uint64_t n, d, k, p0, p1;
+    uint32_t *labels;
+    double *sums;     /* k*d */
+    uint64_t *masses; /* k */
+    int stage;        /* 0 label, 1 mass, 2 centroid */
+} ora_t3_job;
+
+static void *ora_t3_worker(void *arg)
+{
+    ora_t3_job *j = (ora_t3_job *)arg;
+    if (j->stage == 0) {
+        for (uint64_t p = j->p0; p < j->p1; ++p)
+            j->labels[p] = label_point_f32(j->pts, j->n, j->d, j->cen, j->k, p);
+    } else if (j->stage == 1) {
+        memset(j->masses, 0, j->k * sizeof(uint64_t));
+        for (uint64_t p = j->p0; p < j->p1; ++p)
+            j->masses[j->labels[p]] += 1;
+    } else {
+        memset(j->sums, 0, j->k * j->d * sizeof(double));
+        for (uint64_t f = 0; f < j->d; ++f) {
+            const float *row = j->pts + f * j->n;
+            double *s = j->sums + f * j->k;
+            for (uint64_t p = j->p0; p < j->p1; ++p)
+                s[j->labels[p]] += (double)row[p];
+        }
+    }
+    return NULL;
+}
+
+int ora_lloyd3_mt_f32(const float *pts, uint64_t n, uint64_t d, uint64_t k,
+                      float *cen, uint32_t iterations, uint32_t *labels,
+                      uint32_t *masses, int threads)
+{
+    if (threads < 1)
+        threads = 1;
+    pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+    ora_t3_job *jobs = (ora_t3_job *)malloc(sizeof(ora_t3_job) * threads);
+    double *sums = (double *)malloc(sizeof(double) * k * d * threads);
+    uint64_t *ms = (uint64_t *)malloc(sizeof(uint64_t) * k * threads);
+    double *tot = (double *)malloc(sizeof(double) * k * d);
+    uint64_t per = (n + threads - 1) / threads;
+    for (uint32_t it = 0; it < iterations; ++it) {
+        for (int stage = 0; stage < 3; ++stage) {
+            for (int t = 0; t < threads; ++t) {
+                uint64_t p0 = per * t, p1 = p0 + per;
+                if (p0 > n) p0 = n;
+                if (p1 > n) p1 = n;
+                ora_t3_job jb = {pts, cen, n, d, k, p0, p1, labels,
+                                 sums + (size_t)t * k * d, ms + (size_t)t * k, stage};
+                jobs[t] = jb;
+                if (pthread_create(&tid[t], NULL, ora_t3_worker, &jobs[t]) != 0)
+                    return -1;
+            }
+            for (int t = 0; t < threads; ++t)
+                pthread_join(tid[t], NULL);
+        }
         memset(tot, 0, sizeof(double) * k * d);
         for (uint64_t c = 0; c < k; ++c)
             masses[c] = 0;

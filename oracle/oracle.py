@@ -44,6 +44,7 @@ def lib():
             "ora_lloyd_mt_f32": [vp, u64, u64, u64, vp, ctypes.c_uint32, vp, vp, i32],
             "ora_reduce_u32": [vp, u64, u64, vp],
             "ora_grid_scales": [vp, u64, u64, u64, vp],
+            "ora_lloyd3_mt_f32": [vp, u64, u64, u64, vp, ctypes.c_uint32, vp, vp, i32],
             "ora_sum_grid": [vp, u64, u64, vp, u64, vp, vp],
         }
         for name, args in sig.items():
@@ -51,6 +52,7 @@ def lib():
             fn.argtypes = args
             fn.restype = None
         L.ora_lloyd_mt_f32.restype = i32
+        L.ora_lloyd3_mt_f32.restype = i32
         L.ora_grid_bits.argtypes = [u64]
         L.ora_grid_bits.restype = ctypes.c_uint32
         L.ora_partition_matrix.argtypes = [vp, vp, u64, u64, u64]
@@ -222,6 +224,20 @@ def lloyd_mt(points, centroids, iterations, threads):
     masses = np.zeros(k, np.uint32)
     rc = lib().ora_lloyd_mt_f32(pp, n, d, k, c.ctypes.data, iterations, labels.ctypes.data,
                                 masses.ctypes.data, int(threads))
+    assert rc == 0
+    return c, labels, masses
+
+
+def lloyd3_mt(points, centroids, iterations, threads):
+    """Multi-threaded three-stage CPU Lloyd (label, mass, centroid passes)."""
+    p, pp = _c(points, np.float32)
+    c = np.array(centroids, dtype=np.float32, copy=True, order="C")
+    d, n = p.shape
+    k = c.shape[1]
+    labels = np.zeros(n, np.uint32)
+    masses = np.zeros(k, np.uint32)
+    rc = lib().ora_lloyd3_mt_f32(pp, n, d, k, c.ctypes.data, iterations, labels.ctypes.data,
+                                 masses.ctypes.data, int(threads))
     assert rc == 0
     return c, labels, masses
 

@@ -116,11 +116,17 @@ def test_bounded_keep_boundary(gpu_ctx):
     cols, prev = [], []
     for a in (0, 1, 2, 6, 7):
         r = np.sqrt(keep_radius2(cen, a))
-        others = [j for j in range(k) if j != a and np.isfinite(c64[:, j]).all()]
+        # the nearest other finite centroid at a nonzero distance (centroid
+        # 1's duplicate, centroid 3, gives no direction)
+        others = [j for j in range(k) if j != a and np.isfinite(c64[:, j]).all()
+                  and np.sum((c64[:, a] - c64[:, j]) ** 2) > 0]
         near = min(others, key=lambda j: np.sum((c64[:, a] - c64[:, j]) ** 2))
         u = (c64[:, near] - c64[:, a]) / np.linalg.norm(c64[:, near] - c64[:, a])
+        # a keep radius of 0 (a duplicated centroid: never keep) has no
+        # boundary; the points along u then lie at the centroid
+        rb = r if r > 0 else np.sqrt(np.sum((c64[:, near] - c64[:, a]) ** 2)) / 2
         for f in (1 - 1e-6, 1 - 1e-7, 1.0, 1 + 1e-7, 1 + 1e-6, 1.01, 2.2):
-            cols.append(c64[:, a] + u * r * f)
+            cols.append(c64[:, a] + u * rb * f)
             prev.append(a)
         cols.append(0.5 * (c64[:, a] + c64[:, near]))  # equidistant (in real arithmetic)
         prev.append(max(a, near))
@@ -131,6 +137,7 @@ def test_bounded_keep_boundary(gpu_ctx):
             cols.append(c64[:, a] + v / np.linalg.norm(v) * r * rng.uniform(0.9, 1.1))
             prev.append(a)
     pts = np.ascontiguousarray(np.array(cols).T.astype(np.float32))
+    assert np.isfinite(pts).all()  # NaN / inf only in the centroids
     n = pts.shape[1]
     prev = np.array(prev, np.uint32)
     ref = O.label(pts, cen)

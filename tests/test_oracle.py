@@ -226,10 +226,10 @@ def test_lloyd_mt_equals_single_threaded():
     pts, _ = clkmeans.generate_blobs(50000, 4, 10, seed=2, sigma=4.0, box=10.0)
     init = pts[:, ::7][:, :10].copy()
     a = O.lloyd(pts, init, 8)
-    b = O.lloyd_mt(pts, init, 8, 4)
-    np.testing.assert_array_equal(a[1], b[1])
-    np.testing.assert_array_equal(a[2], b[2])
-    np.testing.assert_allclose(a[0], b[0], rtol=1e-6)
+    for b in (O.lloyd_mt(pts, init, 8, 4), O.lloyd3_mt(pts, init, 8, 3)):
+        np.testing.assert_array_equal(a[1], b[1])
+        np.testing.assert_array_equal(a[2], b[2])
+        np.testing.assert_allclose(a[0], b[0], rtol=1e-6)
 
 
 def test_naive_kahan_mode_close_to_fp64():
