@@ -567,9 +567,10 @@ def batch(args):
     # the plain per-job loop on the same GPU
     stream.synchronize()
     w0 = time.perf_counter()
-    ser.run_serial(s)
+    if not args.no_serial:
+        ser.run_serial(s)
     stream.synchronize()
-    serial_ms = (time.perf_counter() - w0) * 1e3
+    serial_ms = (time.perf_counter() - w0) * 1e3 if not args.no_serial else None
 
     # parity: every job's bits equal the per-job loop's (exact sums); a
     # sample against the oracle
@@ -579,6 +580,8 @@ def batch(args):
         ca, la, ma, ita, _ = eng.take(a, s)
         if j in checks:
             sample_out[j] = (ca, la, ma)
+        if args.no_serial:
+            continue
         cb, lb, mb, itb, _ = ser.take(b_, s)
         lab_equal += int(np.array_equal(la, lb))
         mass_equal += int(np.array_equal(ma, mb))
@@ -600,8 +603,9 @@ def batch(args):
         "jobs": len(mix), "points": points, "labeling": args.batch_labeling, "tile_work": tuned,
         "dtype": "f32" if args.batch_labeling == "unroll_vector" else
                  args.batch_labeling.split("_")[1] + "-mfma+f32-exact",
-        "serial_per_job_loop": {"solves_per_s": len(mix) / (serial_ms * 1e-3), "total_ms": serial_ms},
-        "speedup_vs_serial": serial_ms / batch_ms,
+        "serial_per_job_loop": {"solves_per_s": len(mix) / (serial_ms * 1e-3), "total_ms": serial_ms}
+                               if serial_ms else None,
+        "speedup_vs_serial": serial_ms / batch_ms if serial_ms else None,
         "roofline": {"bound": "mfma", "achieved": ach_flops, "peak": MFMA16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": ach_flops / MFMA16_PEAK_TFLOPS, "traffic": None,
                      "kernel": "one batch iteration (size-class tile kernels, merge, finish)",
@@ -790,6 +794,8 @@ def main():
     ap.add_argument("--large-frac", type=float, default=0.02,
                     help="serve workload: share of large jobs (N=2^24, d=32, k=256)")
     ap.add_argument("--jobs", type=int, default=4096, help="batch workload: number of jobs")
+    ap.add_argument("--no-serial", action="store_true",
+                    help="batch workload: skip the plain per-job loop (for profiling the batch kernels)")
     ap.add_argument("--tile-work", type=int, default=0,
                     help="serve workload: points per cluster in a tile (0: the engine's default)")
     ap.add_argument("--batch-labeling", default="mfma_bf16", help="batch and serve: labeling strategy")

@@ -425,6 +425,7 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
     // its first point; past d: feature d-1, zeroed at use), so no branch
     // separates the loads from their use.
     const bool keep_ok = J.keep_ok != 0;
+    const bool special = J.gmax[d] != 0;  // the job holds NaN / inf
     float xn[DMAX];
     uint32_t ln = 0;
     auto fetch = [&](uint64_t b0) {
@@ -482,22 +483,28 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
                 J.labels[p] = lab[0];
         }
         // grid integers summed along the runs; each run's last lane adds
-        // its sums to the work-group's accumulator
+        // its sums to the work-group's accumulator (a job without NaN / inf
+        // is on its grid: no range checks)
         const WaveRuns r = wave_runs(lab[0], valid, lane);
         bool off = false;
 #pragma unroll
         for (int f = 0; f < DMAX; ++f)
             if (f < (int)d) {
-                bool ok;
-                const int32_t q = grid_q(x[f][0], acc.scl[f], ok);
-                off = off || (valid && !ok);
+                int32_t q;
+                if (special) {
+                    bool ok;
+                    q = grid_q(x[f][0], acc.scl[f], ok);
+                    off = off || (valid && !ok);
+                } else {
+                    q = grid_q_on(x[f][0], acc.scl[f]);
+                }
                 const int32_t v = run_sum((int32_t)__shfl((int)q, (int)r.src), r);
                 if (r.last)
                     lds_add64(acc.A + (uint32_t)f * k + r.label, (long long)v);
             }
         if (r.last)
             lds_add(acc.cnt + r.label, r.count);
-        if (__builtin_expect(__ballot(off) != 0, 0)) {
+        if (special && __builtin_expect(__ballot(off) != 0, 0)) {
             // off-grid values (added as 0): flag their cells for the finish
             if (off) {
 #pragma unroll
@@ -690,6 +697,7 @@ __device__ void batch_finalize(BJob &J, double *red)
     const float *old = J.cen[J.iter & 1];
     float *nxt = J.cen[(J.iter & 1) ^ 1];
     double local = 0.0;
+    const bool special = J.gmax[d] != 0;  // the job holds NaN / inf: flags may be set
     for (uint32_t c = tid; c < k; c += blockDim.x) {
         const double m = tot[(size_t)k * d + c];
         double sq = 0.0;
@@ -697,7 +705,7 @@ __device__ void batch_finalize(BJob &J, double *red)
             // exact sums, or NaN / inf where off-grid values were flagged
             // (the flags are cleared for the next iteration)
             const size_t i = (size_t)f * k + c;
-            const uint32_t fl = J.flags[i];
+            const uint32_t fl = special ? J.flags[i] : 0u;
             if (fl)
                 J.flags[i] = 0u;
             const float v = (float)((fl ? special_value(fl) : tot[i]) / m);
@@ -800,6 +808,7 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
     // points all pass skips the scores, one with one or two failures rescans
     // those exactly from the rows, others take the MFMA path
     const bool keep_ok = J.keep_ok != 0;
+    const bool special = J.gmax[d] != 0;  // the job holds NaN / inf
     const uint32_t w4 = J.cen_t_w / 4;
     // global-memory views of the job's arrays (global, not flat, loads)
     const gf32v4 *rows = (const gf32v4 *)J.cen_t;
@@ -1033,14 +1042,21 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         // of the points sorted by label
         r = wave_runs(mine, mp < n, lane);
         bool off = false;
+        if (special) {
 #pragma unroll
-        for (int f = 0; f < (int)P; ++f)
-            if (f < (int)d) {
-                bool ok;
-                xc[f] = __int_as_float(grid_q(xc[f], acc.scl[f], ok));
-                off = off || (mp < n && !ok);
-            }
-        if (__builtin_expect(__ballot(off) != 0, 0)) {
+            for (int f = 0; f < (int)P; ++f)
+                if (f < (int)d) {
+                    bool ok;
+                    xc[f] = __int_as_float(grid_q(xc[f], acc.scl[f], ok));
+                    off = off || (mp < n && !ok);
+                }
+        } else {
+#pragma unroll
+            for (int f = 0; f < (int)P; ++f)
+                if (f < (int)d)
+                    xc[f] = __int_as_float(grid_q_on(xc[f], acc.scl[f]));
+        }
+        if (special && __builtin_expect(__ballot(off) != 0, 0)) {
             // off-grid values (added as 0): flag their cells for the finish
             if (off)
                 for (uint32_t f = 0; f < d; ++f) {
@@ -1238,6 +1254,11 @@ struct clk_batch {
     uint32_t stable_steps = 0;  // steps since the job set last changed
     bool dry = false;           // autotuning: iterations label and accumulate only
     BClassState cls[clk::kClasses];
+    // the size classes' tile kernels run concurrently: forked from the
+    // step's stream onto these (default priority) and joined before the merge
+    static constexpr int kSide = 3;
+    hipStream_t side[kSide] = {};
+    hipEvent_t fork = nullptr, join[kSide] = {};
     BJob *h_rec = nullptr;  // pinned staging of job records for activation
     BJob *h_rec_dev = nullptr;  // its device address
 };
@@ -1259,7 +1280,7 @@ uint64_t job_bytes(uint64_t n, uint32_t d, uint32_t k, uint32_t tile_pts, bool p
     b += clk::align_up((uint64_t)k * 4, 256);             // keep thresholds
     b += clk::align_up((uint64_t)k * clk::keep_row_width(d) * 4, 256);  // centroid rows
     b += clk::align_up(nt * (uint64_t)k * (d + 1) * 8, 256);  // partial rows
-    b += clk::align_up((uint64_t)d * 4, 256);                  // grid (max |x| bits)
+    b += clk::align_up((uint64_t)(d + 1) * 4, 256);            // grid (max |x| bits, NaN/inf seen)
     b += clk::align_up((uint64_t)d * k * 4, 256);              // off-grid flags
     return b;
 }
@@ -1487,13 +1508,32 @@ int rebuild(clk_batch *b, hipStream_t stream)
 }
 
 // The tile kernel of every non-empty class, then the merge and finish
-// kernels (the finish resets the tile counters).
-int launch_step(clk_batch *b, hipStream_t stream)
+// kernels (the finish resets the tile counters).  The classes are
+// independent (their own jobs, tile counters and partial rows): they run
+// concurrently on the step's stream and the batch's side streams, forked
+// and joined by events (inside a graph capture these become its edges), so
+// the tail of one class overlaps the others.
+int launch_step(clk_batch *b, hipStream_t stream_main)
 {
+    uint32_t used = 0;  // side streams forked so far
+    uint32_t nlaunched = 0;
     for (uint32_t c = 0; c < clk::kClasses; ++c) {
         const BClassState &C = b->cls[c];
         if (C.ntiles == 0)
             continue;
+        // round robin: main stream, then the side streams
+        const uint32_t slot = nlaunched++ % (clk_batch::kSide + 1);
+        hipStream_t stream = stream_main;
+        if (slot != 0) {
+            const uint32_t si = slot - 1;
+            if (si >= used) {
+                if (used == 0)
+                    CLK_HIP_CHECK(hipEventRecord(b->fork, stream_main));
+                CLK_HIP_CHECK(hipStreamWaitEvent(b->side[si], b->fork, 0));
+                used = si + 1;
+            }
+            stream = b->side[si];
+        }
         const BTile *tl = b->d_tiles + C.tile0;
         BCtl *ctl = b->d_ctl + c;
         const dim3 g(C.grid), t(64 * C.waves);
@@ -1523,6 +1563,11 @@ int launch_step(clk_batch *b, hipStream_t stream)
         }
         CLK_HIP_CHECK(hipGetLastError());
     }
+    for (uint32_t si = 0; si < used; ++si) {
+        CLK_HIP_CHECK(hipEventRecord(b->join[si], b->side[si]));
+        CLK_HIP_CHECK(hipStreamWaitEvent(stream_main, b->join[si], 0));
+    }
+    hipStream_t stream = stream_main;
     if (b->nunits > 0) {
         hipLaunchKernelGGL(clk::batch_merge_kernel, dim3(b->nunits), dim3(clk::kMergeThreads), 0, stream, b->d_jobs,
                            b->d_units, b->d_ctl);
@@ -1600,6 +1645,15 @@ extern "C" int clk_batch_create(clk_context *ctx, const clk_batch_config *cfg, c
     }
     CLK_HIP_CHECK(hipMemset(b->d_jobs, 0, cfg->max_jobs * sizeof(BJob)));
     CLK_HIP_CHECK(hipMemset(b->d_ctl, 0, clk::kClasses * sizeof(BCtl)));
+    bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; i < clk_batch::kSide && ok; ++i)
+        ok = hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        clk_batch_destroy(b);
+        clk::set_error("clk_batch_create: stream / event creation failed");
+        return CLK_ERR_HIP;
+    }
     *out = b;
     return CLK_OK;
 }
@@ -1628,6 +1682,14 @@ extern "C" int clk_batch_destroy(clk_batch *b)
         (void)hipHostFree(b->h_rec);
     if (b->arena)
         (void)hipFree(b->arena);
+    for (int i = 0; i < clk_batch::kSide; ++i) {
+        if (b->side[i])
+            (void)hipStreamDestroy(b->side[i]);
+        if (b->join[i])
+            (void)hipEventDestroy(b->join[i]);
+    }
+    if (b->fork)
+        (void)hipEventDestroy(b->fork);
     delete b;
     return CLK_OK;
 }
@@ -1712,7 +1774,7 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
     J.part = reinterpret_cast<double *>(p);
     p += clk::align_up(clk::div_up(n, row_tile_points(b->tile_work, d, k)) * (uint64_t)k * (d + 1) * 8, 256);
     J.gmax = reinterpret_cast<uint32_t *>(p);
-    p += clk::align_up((uint64_t)d * 4, 256);
+    p += clk::align_up((uint64_t)(d + 1) * 4, 256);
     J.flags = reinterpret_cast<uint32_t *>(p);
     J.n = n;
     J.d = d;

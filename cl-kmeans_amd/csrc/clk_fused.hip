@@ -388,7 +388,7 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
         // [per-call grid d words, 256-aligned] [partial sums] [partial masses]:
         // as many work-groups as the rows that fit (grid-stride: any count
         // gives the same results)
-        const size_t head = ctx->grid_pinned ? 0 : (d * sizeof(uint32_t) + 255) / 256 * 256;
+        const size_t head = ctx->grid_pinned ? 0 : ((d + 1) * sizeof(uint32_t) + 255) / 256 * 256;
         const size_t row = d * k * sizeof(double) + k * sizeof(uint32_t);
         const size_t fit = ext->bytes > head ? (ext->bytes - head) / row : 0;
         CLK_REQUIRE(fit >= 1, CLK_ERR_SIZE, "lloyd_step_noalloc: k*(8d+4) exceeds the static scratch");

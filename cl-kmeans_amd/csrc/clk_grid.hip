@@ -15,18 +15,21 @@ namespace {
 // A wave's maximum goes to gmax[f] with one atomicMax (bits of non-negative
 // floats order like their values).  NaN and +-inf are skipped.
 __global__ void __launch_bounds__(256)
-grid_max_kernel(const float *__restrict__ pts, uint64_t n, uint32_t *__restrict__ gmax)
+grid_max_kernel(const float *__restrict__ pts, uint64_t n, uint32_t *__restrict__ gmax, uint32_t d)
 {
     const uint32_t f = blockIdx.y;
     const float *row = pts + (uint64_t)f * n;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     uint32_t m = 0;
+    bool special = false;  // a NaN or an infinity seen
     const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool vec4 = (n % 4 == 0) && (reinterpret_cast<uintptr_t>(row) % 16 == 0);
     auto take = [&](float x) {
         const float a = __builtin_fabsf(x);
         const uint32_t u = __float_as_uint(a);
-        m = (a <= FLT_MAX && u > m) ? u : m;
+        const bool fin = a <= FLT_MAX;
+        m = (fin && u > m) ? u : m;
+        special = special || !fin;
     };
     if (vec4) {
         const f32x4 *r4 = reinterpret_cast<const f32x4 *>(row);
@@ -47,6 +50,8 @@ grid_max_kernel(const float *__restrict__ pts, uint64_t n, uint32_t *__restrict_
     }
     if (threadIdx.x % 64 == 0 && m != 0)
         atomicMax(&gmax[f], m);
+    if (__ballot(special) != 0 && threadIdx.x % 64 == 0)
+        atomicOr(&gmax[d], 1u);
 }
 
 }  // namespace
@@ -54,7 +59,7 @@ grid_max_kernel(const float *__restrict__ pts, uint64_t n, uint32_t *__restrict_
 int launch_grid_max(clk_context *ctx, uint64_t d, uint64_t n, const float *points, uint32_t *gmax,
                     hipStream_t stream)
 {
-    CLK_TRY(clk_fill_zero(ctx, gmax, d * sizeof(uint32_t), stream));
+    CLK_TRY(clk_fill_zero(ctx, gmax, (d + 1) * sizeof(uint32_t), stream));
     if (n == 0)
         return CLK_OK;
     CLK_REQUIRE(d <= 65535, CLK_ERR_SIZE, "sum grid: d too large");
@@ -62,7 +67,7 @@ int launch_grid_max(clk_context *ctx, uint64_t d, uint64_t n, const float *point
     const uint64_t gx = std::min<uint64_t>(per_row, div_up(n, 256 * 4));
     KernelTimer t(ctx, "SumGrid", stream);
     hipLaunchKernelGGL(grid_max_kernel, dim3((unsigned)std::max<uint64_t>(gx, 1), (unsigned)d), dim3(256),
-                       0, stream, points, n, gmax);
+                       0, stream, points, n, gmax, (uint32_t)d);
     CLK_HIP_CHECK(hipGetLastError());
     return CLK_OK;
 }
@@ -77,7 +82,7 @@ int resolve_grid(clk_context *ctx, uint64_t d, uint64_t n, const float *points, 
         out->bits = grid_bits(ctx->grid_n);
         return CLK_OK;
     }
-    CLK_TRY(ensure_scratch(ctx->grid, d * sizeof(uint32_t)));
+    CLK_TRY(ensure_scratch(ctx->grid, (d + 1) * sizeof(uint32_t)));
     CLK_TRY(launch_grid_max(ctx, d, n, points, static_cast<uint32_t *>(ctx->grid.ptr), stream));
     out->gmax = static_cast<const uint32_t *>(ctx->grid.ptr);
     out->bits = grid_bits(n);
@@ -95,7 +100,7 @@ extern "C" int clk_sum_grid_from_points(clk_context *ctx, uint64_t d, uint64_t n
     CLK_REQUIRE(d >= 1 && d <= 65535, CLK_ERR_SIZE, "clk_sum_grid_from_points: bad d");
     CLK_REQUIRE(points || n == 0, CLK_ERR_INVALID_ARGUMENT, "clk_sum_grid_from_points: null points");
     ctx->grid_pinned = false;
-    CLK_TRY(ensure_scratch(ctx->grid, d * sizeof(uint32_t)));
+    CLK_TRY(ensure_scratch(ctx->grid, (d + 1) * sizeof(uint32_t)));
     CLK_TRY(launch_grid_max(ctx, d, n, points, static_cast<uint32_t *>(ctx->grid.ptr), as_stream(stream)));
     ctx->grid_pinned = true;
     ctx->grid_d = d;
@@ -108,7 +113,7 @@ extern "C" int clk_sum_grid_from_max(clk_context *ctx, uint64_t d, uint64_t n, c
     CLK_TRY(enter(ctx));
     CLK_REQUIRE(d >= 1 && d <= 65535, CLK_ERR_SIZE, "clk_sum_grid_from_max: bad d");
     CLK_REQUIRE(host_max_abs != nullptr, CLK_ERR_INVALID_ARGUMENT, "clk_sum_grid_from_max: null maxima");
-    std::vector<uint32_t> bits(d);
+    std::vector<uint32_t> bits(d + 1, 0u);
     for (uint64_t f = 0; f < d; ++f) {
         const float a = std::fabs(host_max_abs[f]);
         uint32_t u = 0;
@@ -116,9 +121,10 @@ extern "C" int clk_sum_grid_from_max(clk_context *ctx, uint64_t d, uint64_t n, c
             std::memcpy(&u, &a, 4);
         bits[f] = u;
     }
+    bits[d] = 1u;  // whether the data hold NaN / inf is not known: check
     ctx->grid_pinned = false;
-    CLK_TRY(ensure_scratch(ctx->grid, d * sizeof(uint32_t)));
-    CLK_HIP_CHECK(hipMemcpy(ctx->grid.ptr, bits.data(), d * sizeof(uint32_t), hipMemcpyHostToDevice));
+    CLK_TRY(ensure_scratch(ctx->grid, (d + 1) * sizeof(uint32_t)));
+    CLK_HIP_CHECK(hipMemcpy(ctx->grid.ptr, bits.data(), (d + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
     ctx->grid_pinned = true;
     ctx->grid_d = d;
     ctx->grid_n = n;

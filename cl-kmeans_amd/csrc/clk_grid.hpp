@@ -44,7 +44,8 @@ constexpr int kGridMaxBits = 25;
 constexpr int kGridMinExp = -100;
 
 // Kernel argument: per-feature max |x| bit patterns (finite values; the bits
-// of a non-negative float order like its value) and b.
+// of a non-negative float order like its value) and b.  gmax[d] is non-zero
+// when the grid's data hold a NaN or an infinity (or when it is not known).
 struct GridRef {
     const uint32_t *gmax;
     uint32_t bits;
@@ -93,6 +94,13 @@ __device__ __forceinline__ int32_t grid_q(float x, float scale, bool &ok)
     return ok ? q : 0;
 }
 
+// The grid integer of a value known to be on the grid (data whose grid was
+// computed from themselves and hold no NaN / inf).
+__device__ __forceinline__ int32_t grid_q_on(float x, float scale)
+{
+    return (int32_t)__builtin_rintf(x * scale);
+}
+
 // Flag bits of a value off the grid: 1 NaN (or finite but off the grid),
 // 2 +inf, 4 -inf.
 __host__ __device__ inline uint32_t special_flag(float x)
@@ -126,8 +134,8 @@ __device__ __forceinline__ void lds_or(uint32_t *p, uint32_t v)
     __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Host side: the grid of (points [d][n]) into gmax[d] (device), on stream:
-// zero, then the max-|x| pass.
+// Host side: the grid of (points [d][n]) into gmax[d + 1] (device), on
+// stream: zero, then the max-|x| pass (gmax[d]: NaN / inf seen).
 int launch_grid_max(clk_context *ctx, uint64_t d, uint64_t n, const float *points, uint32_t *gmax,
                     hipStream_t stream);
 
