@@ -106,6 +106,7 @@ class ServeStats(ctypes.Structure):
         ("mean_latency", ctypes.c_double),
         ("steps", ctypes.c_uint64),
         ("max_active", ctypes.c_uint32),
+        ("solo_steps", ctypes.c_uint64),
     ]
 
 

@@ -181,11 +181,20 @@ inline BLayout mfma_layout_for(uint32_t waves, uint32_t kmax, uint32_t kdmax, ui
     return L;
 }
 
+// Floats between the staged centroid rows of the plain kernel: rows of 8 and
+// 16 features get 4 floats of padding, so the lanes of a wave reading
+// different centroids (the keep test: each lane its previous label's row;
+// the wave-wide rescan: each lane another centroid) hit different LDS banks
+// with their 128-bit reads, where unpadded rows start on the same few banks.
+// (Rows of 32 features stay unpadded: the d <= 32 kernel is held to 168
+// VGPRs, and the padded addressing spills.)
+__host__ __device__ constexpr uint32_t cen_stride(uint32_t dm) { return (dm == 8 || dm == 16) ? dm + 4 : dm; }
+
 // LDS of the plain kernel: the exact accumulator (largest k, largest k*d) |
-// centroids [k][DMAX] | W rescan lists [64]
+// centroids [k][cen_stride(DMAX)] | W rescan lists [64]
 __host__ __device__ inline size_t batch_lds_bytes(uint32_t waves, uint32_t k, uint32_t kd, uint32_t dm)
 {
-    return acc_bytes(k, kd, dm) + (size_t)k * dm * 4 + (size_t)waves * 64 * 4;
+    return acc_bytes(k, kd, dm) + (size_t)k * cen_stride(dm) * 4 + (size_t)waves * 64 * 4;
 }
 
 // Labels of the lanes in `fail` (at most 16 of the wave's lanes: points that
@@ -196,7 +205,7 @@ __host__ __device__ inline size_t batch_lds_bytes(uint32_t waves, uint32_t k, ui
 // distances -- the label the sequential scan returns (0 when no distance is
 // below FLT_MAX).  Returns the label of the calling lane's own point (lanes
 // outside `fail` get a value to ignore).  slot: 64 words of LDS of the wave.
-template <int DMAX>
+template <int DMAX, int S>
 __device__ __forceinline__ uint32_t sparse_scan(const float (&x)[DMAX][1], const float *__restrict__ cen_s,
                                                 uint32_t d, uint32_t k, unsigned long long fail, uint32_t lane,
                                                 uint32_t *slot)
@@ -217,7 +226,7 @@ __device__ __forceinline__ uint32_t sparse_scan(const float (&x)[DMAX][1], const
     float best = FLT_MAX;
     uint32_t bi = 0xFFFFFFFFu;
     for (uint32_t c = r; c < k; c += G) {
-        const float *cc = cen_s + (size_t)c * DMAX;
+        const float *cc = cen_s + (size_t)c * S;
         float t = xs[0] - cc[0];
         float dist = t * t;
 #pragma unroll
@@ -404,11 +413,12 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
     (void)waves;
     const uint32_t tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     const uint32_t d = J.d, k = J.k;
+    constexpr int S = (int)cen_stride(DMAX);
     const BAcc acc = acc_carve(reinterpret_cast<char *>(lds), J, DMAX);
     float *cen_s = reinterpret_cast<float *>(reinterpret_cast<char *>(lds) + acc_bytes(k, k * d, DMAX));
-    uint32_t *slot = reinterpret_cast<uint32_t *>(cen_s + (size_t)k * DMAX) + wave * 64;
+    uint32_t *slot = reinterpret_cast<uint32_t *>(cen_s + (size_t)k * S) + wave * 64;
 
-    stage_centroids<DMAX>(J.cen[J.iter & 1], d, k, cen_s);
+    stage_centroids<DMAX, S>(J.cen[J.iter & 1], d, k, cen_s);
     __syncthreads();
 
     const uint64_t first = (uint64_t)tindex * J.tile_pts;
@@ -457,7 +467,7 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
         uint32_t lab[1] = {0u};
         bool keep = false;
         if (keep_ok) {
-            const float *ca = cen_s + (size_t)a * DMAX;
+            const float *ca = cen_s + (size_t)a * S;
             float t = x[0][0] - ca[0];
             float dist = t * t;
 #pragma unroll
@@ -473,11 +483,11 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
             // few failures: the whole wave rescans those points; many: each
             // failing lane scans its own
             if (__builtin_popcountll(fail) <= 16) {
-                const uint32_t l = sparse_scan<DMAX>(x, cen_s, d, k, fail, lane, slot);
+                const uint32_t l = sparse_scan<DMAX, S>(x, cen_s, d, k, fail, lane, slot);
                 if (valid && !keep)
                     lab[0] = l;
             } else if (valid && !keep) {
-                argmin_lds<DMAX, 1>(x, cen_s, k, lab);
+                argmin_lds<DMAX, 1, S>(x, cen_s, k, lab);
             }
             if (valid && !keep)
                 J.labels[p] = lab[0];
@@ -807,7 +817,6 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
     // centroid's keep radius keeps its label (keep_threshold); a wave whose
     // points all pass skips the scores, one with one or two failures rescans
     // those exactly from the rows, others take the MFMA path
-    const bool keep_ok = J.keep_ok != 0;
     const bool special = J.gmax[d] != 0;  // the job holds NaN / inf
     const uint32_t w4 = J.cen_t_w / 4;
     // global-memory views of the job's arrays (global, not flat, loads)
@@ -826,9 +835,12 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
     const uint64_t first = (uint64_t)tindex * J.tile_pts;
     const uint64_t end = std::min<uint64_t>(first + J.tile_pts, n);
     const uint64_t nb = (end - first + 63) / 64;
-    // the next batch's points are loaded (MFMA lane layout) while this batch
-    // is labeled; this batch's points are loaded again one point per lane
-    // (cache hits, for the accumulation) at the start of its labeling
+    // The points are loaded one point per lane (xc: keep test, exact rescans,
+    // accumulation).  The MFMA lane layout (raw) is needed only by batches
+    // that take the MFMA path: it is prefetched for the next batch while
+    // this one is labeled when this one took that path (or the job has no
+    // keep test yet), else loaded on demand -- once the labels settle, most
+    // batches pass the keep test and read each point once.
     auto batch_ids = [&](uint64_t b0, uint64_t (&pid)[kGroups]) {
 #pragma unroll
         for (int g = 0; g < kGroups; ++g) {
@@ -842,7 +854,9 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         batch_ids(b0, pid);
         fetch_points<NKB>(raw, J.pts, n, d, pid, h, &kZeroWord);
     };
-    if (wave < nb)
+    const bool keep_ok = J.keep_ok != 0;
+    bool pre = !keep_ok && wave < nb;  // raw holds this batch's points
+    if (pre)
         fetch(first + (uint64_t)wave * 64);
     // Rounds: in round rd wave w labels batch rd*W + w; the run sums go to
     // the work-group's accumulator (exact integer atomics, any order).
@@ -858,9 +872,6 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         const uint64_t base = first + bt * 64;
         uint64_t pid[kGroups];
         batch_ids(base, pid);
-        PointOps<NKB> x;
-        float X2[kGroups], RX2[kGroups];
-        build_operands<FP16, NKB>(raw, x, X2, RX2);
         const uint64_t pme = base + lane;  // lane l holds point base + l in xc
         const bool vme = pme < end;
         uint32_t prev = 0;
@@ -894,12 +905,22 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
                 }
             keep = prev < k && dist <= gkeep[a];
         }
-        fetch(first + (bt + waves) * 64);  // past the tile: every point missing, zeros read
         const unsigned long long fail = keep_ok ? __ballot(vme && !keep) : ~0ull;
         const uint32_t nfail = (uint32_t)__builtin_popcountll(fail);
+        const bool scores = !(keep_ok && nfail <= 2);  // wave-uniform
+        PointOps<NKB> x;
+        float X2[kGroups];
+        if (scores) {
+            if (!pre)
+                fetch(base);
+            build_operands<FP16, NKB>(raw, x, X2);
+        }
+        pre = scores;
+        if (scores)
+            fetch(first + (bt + waves) * 64);  // past the tile: every point missing, zeros read
         uint32_t mine;
         uint64_t mp;
-        if (keep_ok && nfail <= 2) {
+        if (!scores) {
             // kept labels stand; one or two failing points are rescanned
             // exactly: G lanes per point over centroids r, r+G, ..., the
             // (distance, index) minima combined with the lowest index on
@@ -955,7 +976,7 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
                 }
             }
         } else {
-        const WaveBounds w = wave_bounds<NKB>(X2, RX2, J.stats, J.prm);
+        const WaveBounds w = wave_bounds<NKB, FP16>(X2, J.stats, J.prm);
         uint32_t best[kGroups], second[kGroups];
         rank_all<FP16, NKB>(ltab, lnorm, nt16, lane, h, w.K, kmask, x, best, second);
         Verdict v[kGroups];
@@ -1122,9 +1143,17 @@ batch_merge_kernel(const BJob *__restrict__ jobs, const BMergeUnit *__restrict__
 // One work-group per active job: divide, advance, and for a job with the MFMA
 // labeling the next iteration's 16-bit table.  Work-group 0 resets the tile
 // counters of every class (the tile kernels have finished).
+// The status fields that lead a job record (iter, state, shift), mirrored
+// by the finish kernel into pinned host memory after every iteration.
+struct BStatus {
+    uint32_t iter, state;
+    double shift;
+};
+
 template <bool FP16>
 __global__ void __launch_bounds__(256)
-batch_finish_kernel(BJob *__restrict__ jobs, const uint32_t *__restrict__ active, BCtl *__restrict__ ctl)
+batch_finish_kernel(BJob *__restrict__ jobs, const uint32_t *__restrict__ active, BCtl *__restrict__ ctl,
+                    BStatus *__restrict__ status)
 {
     __shared__ double red[256];
     __shared__ float cs[kKeepStage];
@@ -1139,6 +1168,8 @@ batch_finish_kernel(BJob *__restrict__ jobs, const uint32_t *__restrict__ active
     const bool table = J.kind == 1;
     batch_finalize(J, red);
     __syncthreads();
+    if (threadIdx.x == 0)  // host-visible copy of the job's status
+        status[active[blockIdx.x]] = BStatus{J.iter, J.state, J.shift};
     if (table)
         mfma::build_table_block<FP16>(J.cen[nxt], J.d, J.k, J.nt16, J.nkb, J.table, J.norm, J.stats, red);
     // FP32-scan jobs (state, shift and keep_ok are uniform: written before
@@ -1193,13 +1224,33 @@ activate_kernel(const uint32_t *__restrict__ slots, const BJob *__restrict__ rec
         mfma::build_table_block<FP16>(J.cen[0], J.d, J.k, J.nt16, J.nkb, J.table, J.norm, J.stats, red);
 }
 
-// The table of a newly admitted job (from its initial centroids).
+// The table of a job's current centroids (a newly admitted job: its initial
+// ones; a job that iterated alone: those of its last iteration).
 template <bool FP16>
 __global__ void __launch_bounds__(256) batch_build_table_kernel(BJob *__restrict__ jobs, uint32_t slot)
 {
     __shared__ double red[256];
     BJob &J = jobs[slot];
-    mfma::build_table_block<FP16>(J.cen[0], J.d, J.k, J.nt16, J.nkb, J.table, J.norm, J.stats, red);
+    mfma::build_table_block<FP16>(J.cen[J.iter & 1], J.d, J.k, J.nt16, J.nkb, J.table, J.norm, J.stats, red);
+}
+
+// The record update after an iteration of a job alone (the batch finish's
+// bookkeeping for lloyd_step's result): shift, iteration count, stopping
+// rule, status mirror.  The keep thresholds and centroid rows are now stale:
+// the next batch finish recomputes them before any keep test reads them.
+__global__ void __launch_bounds__(64)
+solo_finish_kernel(BJob *__restrict__ jobs, uint32_t slot, const double *__restrict__ shift,
+                   BStatus *__restrict__ status)
+{
+    if (threadIdx.x != 0)
+        return;
+    BJob &J = jobs[slot];
+    J.shift = *shift;
+    J.iter += 1;
+    if (J.iter >= J.max_iter || (J.tol > 0.0 && J.shift < J.tol))
+        J.state = CLK_JOB_DONE;
+    J.keep_ok = 0;
+    status[slot] = BStatus{J.iter, J.state, J.shift};
 }
 
 uint32_t bucket_of(uint32_t d)
@@ -1227,6 +1278,8 @@ struct BClassState {
     size_t lds = 0;
     clk::BLayout layout{};
 };
+
+constexpr uint32_t kGraphAfter = 8;
 
 struct clk_batch {
     clk_context *ctx = nullptr;
@@ -1261,6 +1314,9 @@ struct clk_batch {
     hipEvent_t fork = nullptr, join[kSide] = {};
     BJob *h_rec = nullptr;  // pinned staging of job records for activation
     BJob *h_rec_dev = nullptr;  // its device address
+    // status of every slot after its last iteration, written by the finish
+    // kernel (pinned, device-mapped): the serving loop reads it in place
+    clk::BStatus *h_status = nullptr, *h_status_dev = nullptr;
 };
 
 namespace {
@@ -1575,9 +1631,11 @@ int launch_step(clk_batch *b, hipStream_t stream_main)
     }
     const dim3 fg(std::max<uint32_t>(b->nactive, 1)), fb(256);
     if (b->fp16)
-        hipLaunchKernelGGL(clk::batch_finish_kernel<true>, fg, fb, 0, stream, b->d_jobs, b->d_active, b->d_ctl);
+        hipLaunchKernelGGL(clk::batch_finish_kernel<true>, fg, fb, 0, stream, b->d_jobs, b->d_active, b->d_ctl,
+                           b->h_status_dev);
     else
-        hipLaunchKernelGGL(clk::batch_finish_kernel<false>, fg, fb, 0, stream, b->d_jobs, b->d_active, b->d_ctl);
+        hipLaunchKernelGGL(clk::batch_finish_kernel<false>, fg, fb, 0, stream, b->d_jobs, b->d_active, b->d_ctl,
+                           b->h_status_dev);
     CLK_HIP_CHECK(hipGetLastError());
     return CLK_OK;
 }
@@ -1638,7 +1696,10 @@ extern "C" int clk_batch_create(clk_context *ctx, const clk_batch_config *cfg, c
     hipError_t e2 = hipMalloc(&b->d_ctl, clk::kClasses * sizeof(BCtl));
     hipError_t e3 = hipMalloc(&b->arena, cfg->arena_bytes);
     hipError_t e4 = hipMalloc(&b->d_active, cfg->max_jobs * sizeof(uint32_t));
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+    hipError_t e5 = hipHostMalloc((void **)&b->h_status, cfg->max_jobs * sizeof(clk::BStatus), hipHostMallocDefault);
+    if (e5 == hipSuccess)
+        e5 = hipHostGetDevicePointer((void **)&b->h_status_dev, b->h_status, 0);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess || e5 != hipSuccess) {
         clk_batch_destroy(b);
         clk::set_error("clk_batch_create: device allocation failed");
         return CLK_ERR_NOMEM;
@@ -1680,6 +1741,8 @@ extern "C" int clk_batch_destroy(clk_batch *b)
         (void)hipFree(b->d_units);
     if (b->h_rec)
         (void)hipHostFree(b->h_rec);
+    if (b->h_status)
+        (void)hipHostFree(b->h_status);
     if (b->arena)
         (void)hipFree(b->arena);
     for (int i = 0; i < clk_batch::kSide; ++i) {
@@ -1879,9 +1942,12 @@ extern "C" int clk_batch_step(clk_batch *b, void *stream)
     }
     if (b->nactive == 0)
         return CLK_OK;
-    // a graph pays off once the job set repeats: while jobs come and go every
-    // iteration (serving), launch directly
-    if (++b->stable_steps < 2) {
+    // a graph pays off once the job set repeats: while jobs come and go
+    // (serving), launch directly.  Capture + instantiation cost more than a
+    // few direct launches, so a graph is made after kGraphAfter steps with
+    // the same jobs (serving at 300 jobs/s: p50 0.58 ms capturing at the
+    // second step, 0.43 ms at the eighth)
+    if (++b->stable_steps < kGraphAfter) {
         clk::KernelTimer t(b->ctx, "BatchStep", st);
         return launch_step(b, st);
     }
@@ -2111,14 +2177,7 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
     hipStream_t cs = nullptr;  // admission copies, concurrent with the iterations
     hipStream_t rs = nullptr;  // result copies, concurrent with the next iteration
     hipEvent_t step_done = nullptr;
-    struct JobStatus {  // the first 16 bytes of a job record
-        uint32_t iter, state;
-        double shift;
-    };
-    static_assert(offsetof(BJob, iter) == 0 && offsetof(BJob, state) == 4 && offsetof(BJob, shift) == 8,
-                  "job status leads the record");
-    JobStatus *h_status = nullptr;  // pinned status of every slot
-    std::vector<JobStatus> done_status(slots);
+    std::vector<clk::BStatus> done_status(slots);
     char *res = nullptr, *res_dev = nullptr;  // pinned, device-mapped result staging
     clk::GatherDesc *descs = nullptr, *descs_dev = nullptr;
     size_t res_cap = 0, desc_cap = 0;
@@ -2134,16 +2193,31 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
     }
     int rc = CLK_OK;
     std::vector<int64_t> slot_job(slots, -1);
+    // A lone small job (nothing else resident, staged or arrived) iterates
+    // through the plain per-job fused pass on its own buffers and grid: the
+    // same results bit for bit (exact labels, exact sums on the job's grid),
+    // one streaming launch plus its merge instead of the batch's tile, merge
+    // and finish launches, whose few long tiles leave most of the GPU idle
+    // at low load.  host_iter tracks each job's iteration count (which
+    // centroid buffer is current); solo_ran marks jobs whose MFMA table and
+    // keep data must be rebuilt before they rejoin a batch step.
+    constexpr uint64_t kSoloMaxPoints = 1ull << 20;
+    clk_kernel_config solo_cfg{};
+    double *solo_shift = nullptr;
+    std::vector<uint32_t> host_iter(slots, 0);
+    std::vector<uint8_t> in_step(slots, 0), solo_ran(slots, 0);
     std::deque<Staged> staged;
     auto fail = [&](int code) {
         rc = code;
         return code;
     };
     if (hipEventCreateWithFlags(&step_done, hipEventDisableTiming) != hipSuccess ||
-        hipHostMalloc((void **)&h_status, slots * sizeof(JobStatus), hipHostMallocDefault) != hipSuccess) {
-        clk::set_error("clk_serve_run: event / pinned buffer allocation failed");
+        hipMalloc(&solo_shift, sizeof(double)) != hipSuccess) {
+        clk::set_error("clk_serve_run: event / device allocation failed");
         fail(CLK_ERR_HIP);
     }
+    if (rc == CLK_OK && clk_default_config(b->ctx, CLK_STAGE_FUSED, CLK_FUSED_CLUSTER_MERGE, &solo_cfg) != CLK_OK)
+        fail(CLK_ERR_INVALID_ARGUMENT);
     if (rc == CLK_OK && ensure_pinned_records(b) != CLK_OK)
         fail(CLK_ERR_HIP);
     const auto t0 = std::chrono::steady_clock::now();
@@ -2151,6 +2225,7 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
     bool inflight = false;
     uint64_t steps = 0;
     uint32_t max_active = 0;
+    uint64_t solo_steps = 0;
     while (rc == CLK_OK && finished < count) {
         const double now = seconds_since(t0);
         // 1. stage the arrived jobs in arrival order while slots and arena allow
@@ -2193,16 +2268,15 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
                 break;
             }
             inflight = false;
-            if (hipMemcpy2DAsync(h_status, sizeof(JobStatus), b->d_jobs, sizeof(BJob), sizeof(JobStatus), slots,
-                                 hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess) {
-                clk::set_error("clk_serve_run: job status read failed");
-                fail(CLK_ERR_HIP);
-                break;
-            }
-            // finished jobs leave the tile list now; their results are copied
-            // while the next iteration runs (no kernel touches a DONE job)
+            // the finish kernel of that iteration wrote every active job's
+            // status to b->h_status (pinned): read in place.  Finished jobs
+            // leave the tile list now; their results are copied while the
+            // next iteration runs (no kernel touches a DONE job)
+            const clk::BStatus *h_status = b->h_status;
             done_slots.clear();
+            for (uint32_t s = 0; s < slots; ++s)
+                if (in_step[s])
+                    host_iter[s] = h_status[s].iter;
             for (uint32_t s = 0; s < slots; ++s)
                 if (b->jobs[s].state == CLK_JOB_ACTIVE && h_status[s].state == CLK_JOB_DONE) {
                     b->jobs[s].state = CLK_JOB_DONE;
@@ -2234,6 +2308,9 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
             J.state = b->staged_state[g.slot];
             b->h_rec[g.slot] = J;
             act_slots.push_back((uint32_t)g.slot);
+            host_iter[g.slot] = 0;
+            in_step[g.slot] = 0;
+            solo_ran[g.slot] = 0;
             if (J.state == CLK_JOB_ACTIVE)
                 ++active;
             else  // zero iterations: done at once
@@ -2286,7 +2363,48 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
         // 4. the next iteration of every resident job
         if (active > 0) {
             max_active = std::max(max_active, active);
-            const int r = clk_batch_step(b, st);
+            int32_t lone = -1;
+            if (active == 1 && staged.empty() && (next >= count || jobs[next].arrival > seconds_since(t0)))
+                for (uint32_t s = 0; s < slots && lone < 0; ++s)
+                    if (b->jobs[s].state == CLK_JOB_ACTIVE && b->jobs[s].n <= kSoloMaxPoints)
+                        lone = (int32_t)s;
+            int r = CLK_ERR_UNSUPPORTED;
+            if (lone >= 0) {
+                const BJob &J = b->jobs[lone];  // host record: buffers and sizes
+                const uint32_t it = host_iter[lone];
+                const clk::GridRef g{J.gmax, clk::grid_bits(J.n)};
+                r = clk::lloyd_step_with_grid(b->ctx, &solo_cfg, J.d, J.n, J.k, J.pts, J.cen[it & 1],
+                                              J.cen[(it & 1) ^ 1], J.part, J.labels, J.masses, solo_shift, g, st);
+                if (r == CLK_OK) {
+                    hipLaunchKernelGGL(clk::solo_finish_kernel, dim3(1), dim3(64), 0, st, b->d_jobs, (uint32_t)lone,
+                                       solo_shift, b->h_status_dev);
+                    r = hipGetLastError() == hipSuccess ? CLK_OK : CLK_ERR_HIP;
+                    solo_ran[lone] = 1;
+                    ++solo_steps;
+                }
+            }
+            if (r == CLK_ERR_UNSUPPORTED) {  // the batch step
+                r = CLK_OK;
+                for (uint32_t s = 0; s < slots && r == CLK_OK; ++s) {
+                    if (!solo_ran[s])
+                        continue;
+                    solo_ran[s] = 0;
+                    if (b->jobs[s].state == CLK_JOB_ACTIVE && b->jobs[s].kind == 1) {
+                        if (b->fp16)
+                            hipLaunchKernelGGL(clk::batch_build_table_kernel<true>, dim3(1), dim3(256), 0, st,
+                                               b->d_jobs, s);
+                        else
+                            hipLaunchKernelGGL(clk::batch_build_table_kernel<false>, dim3(1), dim3(256), 0, st,
+                                               b->d_jobs, s);
+                        if (hipGetLastError() != hipSuccess)
+                            r = CLK_ERR_HIP;
+                    }
+                }
+                if (r == CLK_OK)
+                    r = clk_batch_step(b, st);
+            }
+            for (uint32_t s = 0; s < slots; ++s)
+                in_step[s] = b->jobs[s].state == CLK_JOB_ACTIVE;
             if (r != CLK_OK || hipEventRecord(step_done, st) != hipSuccess) {
                 fail(r != CLK_OK ? r : CLK_ERR_HIP);
                 break;
@@ -2391,8 +2509,8 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
     (void)hipStreamDestroy(rs);
     if (step_done)
         (void)hipEventDestroy(step_done);
-    if (h_status)
-        (void)hipHostFree(h_status);
+    if (solo_shift)
+        (void)hipFree(solo_shift);
     if (res)
         (void)hipHostFree(res);
     if (descs)
@@ -2404,6 +2522,7 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
         latency_stats(jobs, count, stats);
         stats->steps = steps;
         stats->max_active = max_active;
+        stats->solo_steps = solo_steps;
     }
     return CLK_OK;
 }
@@ -2510,6 +2629,7 @@ extern "C" int clk_serve_serial(clk_context *ctx, clk_serve_job *jobs, uint32_t 
         latency_stats(jobs, count, stats);
         stats->steps = 0;
         stats->max_active = 1;
+        stats->solo_steps = 0;
     }
     return CLK_OK;
 }

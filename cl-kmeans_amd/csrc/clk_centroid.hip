@@ -24,6 +24,7 @@
 #include "clk_kernels.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace clk {
 namespace {
@@ -109,14 +110,15 @@ cu_cluster_merge_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, u
             c = (c + 1 == k) ? 0u : c + 1;
         }
     };
-    for (uint64_t g = (uint64_t)blockIdx.x * B + tid; g < groups; g += stride) {
-        const uint64_t p = g * VEC;
-        float x[DMAX][VEC];
-        load_points<DMAX, VEC>(pts, n, d, p, x);
-        uint32_t lab[VEC];
-        load_labels<VEC>(labels, p, lab);
+    // V points p .. p+V-1 (V = VEC, or 1 for the n % VEC tail points)
+    auto add_points = [&](auto vtag, uint64_t p) {
+        constexpr int V = decltype(vtag)::value;
+        float x[DMAX][V];
+        load_points<DMAX, V>(pts, n, d, p, x);
+        uint32_t lab[V];
+        load_labels<V>(labels, p, lab);
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) {
+        for (int v = 0; v < V; ++v) {
             if (lab[v] >= k)
                 continue;
             uint32_t *t = mine + __umul24(lab[v], row_stride);
@@ -129,7 +131,13 @@ cu_cluster_merge_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, u
             flush();
             since = 0;
         }
-    }
+    };
+    const uint64_t id = (uint64_t)blockIdx.x * B + tid;
+    for (uint64_t g = id; g < groups; g += stride)
+        add_points(std::integral_constant<int, VEC>{}, g * VEC);
+    if (VEC > 1 && id == groups % stride)
+        for (uint64_t p = groups * VEC; p < n; ++p)
+            add_points(std::integral_constant<int, 1>{}, p);
     if (since != 0)
         flush();
     __syncthreads();
@@ -395,8 +403,7 @@ int run_centroid_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t
 
     if (strategy == CLK_CENTROID_CLUSTER_MERGE && dmax && dk < (1ull << 24) &&
         head + dk * B * 4 <= ctx->lds_per_block) {
-        const int vec = std::min(pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels),
-                                 cm_vec_cap(dmax));
+        const int vec = std::min(pick_vector_width(cfg ? cfg->vector_length : 4), cm_vec_cap(dmax));
         CmKernel kc = pick_cm(dmax, vec, B == 256);
         lds = head + dk * B * 4;
         gx = user_grid ? user_grid : auto_grid(ctx, (const void *)kc, (int)B, lds, n / vec);

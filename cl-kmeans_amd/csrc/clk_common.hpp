@@ -100,11 +100,12 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 
 __host__ __device__ inline uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
-// Largest vector width <= requested (1,2,4,8) such that every feature row of
-// `n` floats starting at `p` stays aligned to width*4 bytes, and `labels`
-// too.  Rows start at f*n, so n must be a multiple of the width.
-int pick_vector_width(uint64_t requested, uint64_t n, const void *points,
-                      const void *labels);
+// Points per work-item step of the streaming passes: the requested width
+// (1, 2 or 4; 8 and above give 4, anything else 4).  Any n and any 4-byte
+// aligned buffers take it: the vector accesses need only 4-byte alignment
+// (rows start at f*n), and the kernels finish the n % width tail points one
+// at a time.
+int pick_vector_width(uint64_t requested);
 
 // Persistent-style grid: blocks_per_cu resident blocks on every CU, capped by
 // the work available.
@@ -120,6 +121,16 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
+// The same vectors at 4-byte alignment, for point rows and labels in global
+// memory: a row f starts at f*n floats, 16-byte aligned only when 4 divides
+// n.  gfx950 global loads and stores of 2 or 4 dwords take any 4-byte
+// aligned address (the HSA runtime runs the GPU in unaligned access mode),
+// so the accesses stay 64 or 128 bits wide for every n.
+typedef float gf32x4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef float gf32x2 __attribute__((ext_vector_type(2), aligned(4)));
+typedef uint32_t gu32x4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef uint32_t gu32x2 __attribute__((ext_vector_type(2), aligned(4)));
+
 // Streaming (read-once / write-once) accesses: non-temporal loads and stores
 // keep the points and labels from displacing reused data in L2 / MALL; on
 // gfx950 they measured ~8% faster for the fused pass's read+write mix.
@@ -134,7 +145,9 @@ __device__ __forceinline__ void stream_store(V v, V *p) { __builtin_nontemporal_
 // from 0 (lloyd_fused_feature_sum.cl:213-239); padded features add
 // fma(0, 0, dist) == dist exactly.  Strict '<' against a running minimum
 // starting at FLT_MAX; label 0 when nothing is smaller (kmeans_naive.cpp:49-63).
-template <int DMAX, int VEC>
+// STRIDE: floats between consecutive centroid rows (DMAX, or more where
+// rows are padded against LDS bank conflicts).
+template <int DMAX, int VEC, int STRIDE = DMAX>
 __device__ __forceinline__ void argmin_lds(const float (&x)[DMAX][VEC],
                                            const float *__restrict__ cen,
                                            uint32_t k, uint32_t (&lab)[VEC])
@@ -149,7 +162,7 @@ __device__ __forceinline__ void argmin_lds(const float (&x)[DMAX][VEC],
         float cc[DMAX];
 #pragma unroll
         for (int f = 0; f < DMAX; ++f)
-            cc[f] = cen[c * DMAX + f];
+            cc[f] = cen[c * STRIDE + f];
 #pragma unroll
         for (int v = 0; v < VEC; ++v) {
             float t = x[0][v] - cc[0];
@@ -197,7 +210,7 @@ __device__ __forceinline__ void argmin_lds_chunk(const float (&x)[DMAX][VEC],
     }
 }
 
-// Load VEC consecutive points (p % VEC == 0, rows aligned) of DMAX features;
+// Load VEC consecutive points p .. p+VEC-1 (all < n) of DMAX features;
 // features >= d are 0.  EXACT: d == DMAX (no per-feature guards).
 template <int DMAX, int VEC, bool EXACT = false>
 __device__ __forceinline__ void load_points(const float *__restrict__ pts,
@@ -216,14 +229,14 @@ __device__ __forceinline__ void load_points(const float *__restrict__ pts,
             if constexpr (VEC == 1) {
                 x[f][0] = stream_load(row);
             } else if constexpr (VEC == 2) {
-                f32x2 a = stream_load(reinterpret_cast<const f32x2 *>(row));
+                gf32x2 a = stream_load(reinterpret_cast<const gf32x2 *>(row));
                 x[f][0] = a.x; x[f][1] = a.y;
             } else if constexpr (VEC == 4) {
-                f32x4 a = stream_load(reinterpret_cast<const f32x4 *>(row));
+                gf32x4 a = stream_load(reinterpret_cast<const gf32x4 *>(row));
                 x[f][0] = a.x; x[f][1] = a.y; x[f][2] = a.z; x[f][3] = a.w;
             } else {
-                f32x4 a = stream_load(reinterpret_cast<const f32x4 *>(row));
-                f32x4 b = stream_load(reinterpret_cast<const f32x4 *>(row + 4));
+                gf32x4 a = stream_load(reinterpret_cast<const gf32x4 *>(row));
+                gf32x4 b = stream_load(reinterpret_cast<const gf32x4 *>(row + 4));
                 x[f][0] = a.x; x[f][1] = a.y; x[f][2] = a.z; x[f][3] = a.w;
                 x[f][4] = b.x; x[f][5] = b.y; x[f][6] = b.z; x[f][7] = b.w;
             }
@@ -243,12 +256,12 @@ __device__ __forceinline__ void store_labels(uint32_t *__restrict__ labels,
     if constexpr (VEC == 1) {
         stream_store(lab[0], dst);
     } else if constexpr (VEC == 2) {
-        stream_store(u32x2{lab[0], lab[1]}, reinterpret_cast<u32x2 *>(dst));
+        stream_store(gu32x2{lab[0], lab[1]}, reinterpret_cast<gu32x2 *>(dst));
     } else if constexpr (VEC == 4) {
-        stream_store(u32x4{lab[0], lab[1], lab[2], lab[3]}, reinterpret_cast<u32x4 *>(dst));
+        stream_store(gu32x4{lab[0], lab[1], lab[2], lab[3]}, reinterpret_cast<gu32x4 *>(dst));
     } else {
-        stream_store(u32x4{lab[0], lab[1], lab[2], lab[3]}, reinterpret_cast<u32x4 *>(dst));
-        stream_store(u32x4{lab[4], lab[5], lab[6], lab[7]}, reinterpret_cast<u32x4 *>(dst + 4));
+        stream_store(gu32x4{lab[0], lab[1], lab[2], lab[3]}, reinterpret_cast<gu32x4 *>(dst));
+        stream_store(gu32x4{lab[4], lab[5], lab[6], lab[7]}, reinterpret_cast<gu32x4 *>(dst + 4));
     }
 }
 
@@ -260,14 +273,14 @@ __device__ __forceinline__ void load_labels(const uint32_t *__restrict__ labels,
     if constexpr (VEC == 1) {
         lab[0] = src[0];
     } else if constexpr (VEC == 2) {
-        uint2 a = *reinterpret_cast<const uint2 *>(src);
+        gu32x2 a = *reinterpret_cast<const gu32x2 *>(src);
         lab[0] = a.x; lab[1] = a.y;
     } else if constexpr (VEC == 4) {
-        uint4 a = *reinterpret_cast<const uint4 *>(src);
+        gu32x4 a = *reinterpret_cast<const gu32x4 *>(src);
         lab[0] = a.x; lab[1] = a.y; lab[2] = a.z; lab[3] = a.w;
     } else {
-        uint4 a = *reinterpret_cast<const uint4 *>(src);
-        uint4 b = *reinterpret_cast<const uint4 *>(src + 4);
+        gu32x4 a = *reinterpret_cast<const gu32x4 *>(src);
+        gu32x4 b = *reinterpret_cast<const gu32x4 *>(src + 4);
         lab[0] = a.x; lab[1] = a.y; lab[2] = a.z; lab[3] = a.w;
         lab[4] = b.x; lab[5] = b.y; lab[6] = b.z; lab[7] = b.w;
     }
@@ -283,10 +296,10 @@ __device__ __forceinline__ void stream_load_labels(const uint32_t *__restrict__ 
     if constexpr (VEC == 1) {
         lab[0] = stream_load(src);
     } else if constexpr (VEC == 2) {
-        u32x2 a = stream_load(reinterpret_cast<const u32x2 *>(src));
+        gu32x2 a = stream_load(reinterpret_cast<const gu32x2 *>(src));
         lab[0] = a.x; lab[1] = a.y;
     } else {
-        u32x4 a = stream_load(reinterpret_cast<const u32x4 *>(src));
+        gu32x4 a = stream_load(reinterpret_cast<const gu32x4 *>(src));
         lab[0] = a.x; lab[1] = a.y; lab[2] = a.z; lab[3] = a.w;
     }
 }
@@ -322,14 +335,14 @@ __device__ __forceinline__ float keep_threshold(double s2)
 }
 
 // Stage centroids [d][k] (global) into LDS as [c][DMAX], zero-padded.
-template <int DMAX>
+template <int DMAX, int STRIDE = DMAX>
 __device__ __forceinline__ void stage_centroids(const float *__restrict__ cen,
                                                 uint32_t d, uint32_t k,
                                                 float *__restrict__ lds)
 {
     for (uint32_t i = threadIdx.x; i < k * DMAX; i += blockDim.x) {
         uint32_t c = i / DMAX, f = i % DMAX;
-        lds[i] = (f < d) ? cen[(uint64_t)f * k + c] : 0.0f;
+        lds[c * STRIDE + f] = (f < d) ? cen[(uint64_t)f * k + c] : 0.0f;
     }
 }
 

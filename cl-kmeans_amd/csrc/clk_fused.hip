@@ -113,17 +113,18 @@ fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
         }
     };
     // Software pipeline, two register buffers: one step's points are in
-    // flight while the other step's are labelled and accumulated.
-    auto step = [&](const float (&x)[DMAX][VEC], const uint32_t (&prev)[VEC], uint64_t g) {
-        const uint64_t p = g * VEC;
-        uint32_t lab[VEC];
+    // flight while the other step's are labelled and accumulated.  A step
+    // takes V = VEC points p .. p+V-1 (V = 1 for the n % VEC tail points).
+    auto step = [&](const auto &x, const auto &prev, uint64_t p) {
+        constexpr int V = (int)(sizeof(prev) / sizeof(uint32_t));
+        uint32_t lab[V];
         if constexpr (BOUNDED) {
             // Keep the previous label where the point is inside its centroid's
             // keep radius (keep_threshold: the full scan provably returns it);
             // otherwise scan every centroid.  Only changed labels are stored.
             bool keep = true;
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) {
+            for (int v = 0; v < V; ++v) {
                 const bool valid = prev[v] < k;
                 const uint32_t a = valid ? prev[v] : 0u;
                 const float *ca = cen_lds + a * DMAX;
@@ -138,20 +139,20 @@ fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
                 lab[v] = a;
             }
             if (!keep)
-                argmin_lds<DMAX, VEC>(x, cen_lds, k, lab);
+                argmin_lds<DMAX, V>(x, cen_lds, k, lab);
             bool changed = false;
 #pragma unroll
-            for (int v = 0; v < VEC; ++v)
+            for (int v = 0; v < V; ++v)
                 changed = changed || lab[v] != prev[v];
             if (changed)
-                store_labels<VEC>(labels, p, lab);
+                store_labels<V>(labels, p, lab);
         } else {
-            argmin_lds<DMAX, VEC>(x, cen_lds, k, lab);
-            store_labels<VEC>(labels, p, lab);
+            argmin_lds<DMAX, V>(x, cen_lds, k, lab);
+            store_labels<V>(labels, p, lab);
         }
         bool off = false;
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) {
+        for (int v = 0; v < V; ++v) {
             uint32_t *t = mine + __umul24(lab[v], row_stride);
             // u32 LDS atomics run at full speed (ds_add_f32 does not)
             lds_add(PRIVATE ? t : tile + lab[v], 1u);
@@ -172,7 +173,7 @@ fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
             // added as 0; their cells are flagged
             if (off) {
 #pragma unroll
-                for (int v = 0; v < VEC; ++v)
+                for (int v = 0; v < V; ++v)
 #pragma unroll
                     for (int f = 0; f < DMAX; ++f)
                         if ((EXACT || f < (int)d) &&
@@ -203,13 +204,27 @@ fused_tile_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
     while (g < groups) {
         const uint64_t g1 = g + stride;
         load(min(g1, last), xb, lb);
-        step(xa, la, g);
+        step(xa, la, g * VEC);
         if (g1 >= groups)
             break;
         const uint64_t g2 = g1 + stride;
         load(min(g2, last), xa, la);
-        step(xb, lb, g1);
+        step(xb, lb, g1 * VEC);
         g = g2;
+    }
+    if constexpr (VEC > 1) {
+        // the n % VEC points after the last group, one at a time, by the
+        // work-item that would take group `groups` next
+        if ((uint64_t)blockIdx.x * B + tid == groups % stride) {
+            for (uint64_t p = groups * VEC; p < n; ++p) {
+                float x1[DMAX][1];
+                uint32_t l1[1] = {0u};
+                load_points<DMAX, 1, EXACT>(pts, n, d, p, x1);
+                if constexpr (BOUNDED)
+                    stream_load_labels<1>(labels, p, l1);
+                step(x1, l1, p);
+            }
+        }
     }
     if (PRIVATE && since != 0)
         flush();
@@ -309,10 +324,12 @@ struct ExtScratch {
     size_t bytes;
 };
 
+// grid_in: the sum grid to use (the batch engine's per-job grid) instead of
+// the pinned or per-call one; only the LDS-tile paths take it.
 int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64_t n, uint64_t k,
               const float *points, const float *old_centroids, double *sums, uint32_t *labels,
               uint32_t *masses, hipStream_t stream, uint32_t mode, float *new_centroids,
-              double *max_shift, const ExtScratch *ext = nullptr)
+              double *max_shift, const ExtScratch *ext = nullptr, const GridRef *grid_in = nullptr)
 {
     CLK_TRY(check_sizes(d, n, k));
     CLK_REQUIRE(points && old_centroids && sums && labels && masses, CLK_ERR_INVALID_ARGUMENT,
@@ -328,8 +345,7 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
     const int B = block_size(ctx, cfg);
     CLK_REQUIRE(B > 0, CLK_ERR_INVALID_ARGUMENT, "fused: local_size exceeds the device limit");
     const int dmax = dmax_bucket(d);
-    const int vec = std::min(pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels),
-                             fused_vec_cap(dmax));
+    const int vec = std::min(pick_vector_width(cfg ? cfg->vector_length : 4), fused_vec_cap(dmax));
     CLK_REQUIRE(dmax < 64 || B <= 512, CLK_ERR_INVALID_ARGUMENT,
                 "fused: local_size above 512 with more than 32 features");
 
@@ -351,6 +367,8 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
 
     CLK_REQUIRE(!ext || use_private || use_shared || n == 0, CLK_ERR_UNSUPPORTED,
                 "lloyd_step_noalloc: k*d beyond the work-group LDS tiles (the separate stages allocate)");
+    CLK_REQUIRE(!grid_in || use_private || use_shared || n == 0, CLK_ERR_UNSUPPORTED,
+                "fused pass with a given grid: k*d beyond the work-group LDS tiles");
     if ((!use_private && !use_shared) || n == 0) {
         if (mode == 2) {
             CLK_TRY(clk_fill_zero(ctx, sums, d * k * sizeof(double), stream));
@@ -404,7 +422,10 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
         psums = reinterpret_cast<double *>(ext->ptr + head);
         pmasses = reinterpret_cast<uint32_t *>(psums + grid * d * k);
     } else {
-        CLK_TRY(resolve_grid(ctx, d, n, points, stream, &sgrid));
+        if (grid_in)
+            sgrid = *grid_in;
+        else
+            CLK_TRY(resolve_grid(ctx, d, n, points, stream, &sgrid));
         CLK_TRY(ensure_scratch(ctx->part_sums, grid * d * k * sizeof(double)));
         CLK_TRY(ensure_scratch(ctx->part_masses, grid * k * sizeof(uint32_t)));
         psums = static_cast<double *>(ctx->part_sums.ptr);
@@ -446,6 +467,15 @@ int run_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64
         return clk_matrix_divide_row(ctx, d, k, sums, masses, new_centroids,
                                      max_shift ? old_centroids : nullptr, max_shift, stream);
     return CLK_OK;
+}
+
+int lloyd_step_with_grid(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64_t n, uint64_t k,
+                         const float *points, const float *old_centroids, float *new_centroids, double *sums,
+                         uint32_t *labels, uint32_t *masses, double *max_shift, const GridRef &grid,
+                         hipStream_t stream)
+{
+    return run_fused(ctx, cfg, d, n, k, points, old_centroids, sums, labels, masses, stream, 2, new_centroids,
+                     max_shift, nullptr, &grid);
 }
 
 // ---- the entry point that allocates nothing -----------------------------

@@ -23,7 +23,6 @@ grid_max_kernel(const float *__restrict__ pts, uint64_t n, uint32_t *__restrict_
     uint32_t m = 0;
     bool special = false;  // a NaN or an infinity seen
     const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool vec4 = (n % 4 == 0) && (reinterpret_cast<uintptr_t>(row) % 16 == 0);
     auto take = [&](float x) {
         const float a = __builtin_fabsf(x);
         const uint32_t u = __float_as_uint(a);
@@ -31,19 +30,17 @@ grid_max_kernel(const float *__restrict__ pts, uint64_t n, uint32_t *__restrict_
         m = (fin && u > m) ? u : m;
         special = special || !fin;
     };
-    if (vec4) {
-        const f32x4 *r4 = reinterpret_cast<const f32x4 *>(row);
-        for (uint64_t i = t0; i < n / 4; i += stride) {
-            const f32x4 v = stream_load(r4 + i);
-            take(v.x);
-            take(v.y);
-            take(v.z);
-            take(v.w);
-        }
-    } else {
-        for (uint64_t i = t0; i < n; i += stride)
-            take(row[i]);
+    // 128-bit loads (4-byte aligned rows), then the n % 4 tail values
+    const gf32x4 *r4 = reinterpret_cast<const gf32x4 *>(row);
+    for (uint64_t i = t0; i < n / 4; i += stride) {
+        const gf32x4 v = stream_load(r4 + i);
+        take(v.x);
+        take(v.y);
+        take(v.z);
+        take(v.w);
     }
+    if (t0 < n % 4)
+        take(row[n / 4 * 4 + t0]);
     for (int o = 32; o >= 1; o /= 2) {
         const uint32_t other = (uint32_t)__shfl_xor((int)m, o);
         m = other > m ? other : m;

@@ -66,6 +66,14 @@ int launch_accumulate_global(clk_context *ctx, uint64_t d, uint64_t n, uint64_t 
                              const float *points, const uint32_t *labels, const GridRef &grid,
                              double *sums, uint32_t *masses, hipStream_t stream);
 
+// One Lloyd iteration through the fused pass (clk_lloyd_step) with a given
+// sum grid instead of the pinned or per-call one.  CLK_ERR_UNSUPPORTED, with
+// nothing enqueued, when k*d needs the separate stages.
+int lloyd_step_with_grid(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, uint64_t n, uint64_t k,
+                         const float *points, const float *old_centroids, float *new_centroids, double *sums,
+                         uint32_t *labels, uint32_t *masses, double *max_shift, const GridRef &grid,
+                         hipStream_t stream);
+
 // Device address of this device's static scratch of clk_lloyd_step_noalloc
 // into ctx->noalloc_scratch (hipGetSymbolAddress once per device).
 int noalloc_setup(clk_context *ctx);

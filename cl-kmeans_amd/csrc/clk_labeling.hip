@@ -5,6 +5,7 @@
 #include "clk_kernels.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace clk {
 namespace {
@@ -33,19 +34,18 @@ label_lds_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t
         stage_keep_thresholds<DMAX>(cen_lds, k, thr);
         __syncthreads();
     }
-    const uint64_t groups = n / VEC;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += stride) {
-        const uint64_t p = g * VEC;
-        float x[DMAX][VEC];
-        load_points<DMAX, VEC>(pts, n, d, p, x);
-        uint32_t lab[VEC];
+    // V points p .. p+V-1 (V = VEC, or 1 for the n % VEC tail points)
+    auto label_points = [&](auto vtag, uint64_t p) {
+        constexpr int V = decltype(vtag)::value;
+        float x[DMAX][V];
+        load_points<DMAX, V>(pts, n, d, p, x);
+        uint32_t lab[V];
         if constexpr (BOUNDED) {
-            uint32_t prev[VEC];
-            stream_load_labels<VEC>(labels, p, prev);
+            uint32_t prev[V];
+            stream_load_labels<V>(labels, p, prev);
             bool keep = true;
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) {
+            for (int v = 0; v < V; ++v) {
                 const bool valid = prev[v] < k;
                 const uint32_t a = valid ? prev[v] : 0u;
                 const float *ca = cen_lds + a * DMAX;
@@ -60,18 +60,26 @@ label_lds_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t
                 lab[v] = a;
             }
             if (!keep)
-                argmin_lds<DMAX, VEC>(x, cen_lds, k, lab);
+                argmin_lds<DMAX, V>(x, cen_lds, k, lab);
             bool changed = false;
 #pragma unroll
-            for (int v = 0; v < VEC; ++v)
+            for (int v = 0; v < V; ++v)
                 changed = changed || lab[v] != prev[v];
             if (changed)
-                store_labels<VEC>(labels, p, lab);
+                store_labels<V>(labels, p, lab);
         } else {
-            argmin_lds<DMAX, VEC>(x, cen_lds, k, lab);
-            store_labels<VEC>(labels, p, lab);
+            argmin_lds<DMAX, V>(x, cen_lds, k, lab);
+            store_labels<V>(labels, p, lab);
         }
-    }
+    };
+    const uint64_t groups = n / VEC;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t g = id; g < groups; g += stride)
+        label_points(std::integral_constant<int, VEC>{}, g * VEC);
+    if (VEC > 1 && id == groups % stride)
+        for (uint64_t p = groups * VEC; p < n; ++p)
+            label_points(std::integral_constant<int, 1>{}, p);
 }
 
 // k*d beyond LDS: the centroids pass through LDS in chunks of `chunk`
@@ -87,16 +95,16 @@ label_chunked_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint
     const uint64_t per = (uint64_t)blockDim.x * VEC;
     for (uint64_t base = (uint64_t)blockIdx.x * per; base < n; base += (uint64_t)gridDim.x * per) {
         const uint64_t p = base + (uint64_t)threadIdx.x * VEC;
-        const bool valid = p < n;  // n % VEC == 0
+        const bool full = p + VEC <= n;  // else the last, partial group or none
         float x[DMAX][VEC];
-        if (valid) {
+        if (full) {
             load_points<DMAX, VEC>(pts, n, d, p, x);
         } else {
 #pragma unroll
             for (int f = 0; f < DMAX; ++f)
 #pragma unroll
                 for (int v = 0; v < VEC; ++v)
-                    x[f][v] = 0.0f;
+                    x[f][v] = (f < (int)d && p + v < n) ? pts[(uint64_t)f * n + p + v] : 0.0f;
         }
         float best[VEC];
         uint32_t lab[VEC];
@@ -115,8 +123,14 @@ label_chunked_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint
             __syncthreads();
             argmin_lds_chunk<DMAX, VEC>(x, cen_lds, c0, cn, best, lab);
         }
-        if (valid)
+        if (full) {
             store_labels<VEC>(labels, p, lab);
+        } else {
+#pragma unroll
+            for (int v = 0; v < VEC; ++v)
+                if (p + v < n)
+                    labels[p + v] = lab[v];
+        }
     }
 }
 
@@ -288,7 +302,7 @@ int launch_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, 
     bounded = bounded && dmax && cen_plain + k * 4 <= ctx->lds_per_block;
     const uint64_t cen_bytes = cen_plain + (bounded ? k * 4 : 0);
     if (dmax && cen_bytes <= ctx->lds_per_block) {
-        const int vec = std::min(pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels),
+        const int vec = std::min(pick_vector_width(cfg ? cfg->vector_length : 4),
                                  label_vec_cap(dmax));
         LabelKernel kern = bounded ? pick_label_lds<true>(dmax, vec) : pick_label_lds<false>(dmax, vec);
         CLK_REQUIRE(dmax < 64 || B <= 512, CLK_ERR_INVALID_ARGUMENT,
@@ -304,7 +318,7 @@ int launch_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d, 
         return CLK_OK;
     }
     if (dmax) {  // centroid chunks through LDS (64 KiB: two work-groups per CU)
-        const int vec = std::min(pick_vector_width(cfg ? cfg->vector_length : 4, n, points, labels),
+        const int vec = std::min(pick_vector_width(cfg ? cfg->vector_length : 4),
                                  label_vec_cap(dmax));
         ChunkKernel kern = pick_label_chunked(dmax, vec);
         const uint32_t chunk = (uint32_t)std::min<uint64_t>(k, (64u << 10) / (4u * dmax));

@@ -147,7 +147,7 @@ int run_mass_update(clk_context *ctx, const clk_kernel_config *cfg, uint64_t n, 
     const uint64_t B = (cfg && cfg->local_size[0]) ? cfg->local_size[0] : 256;
     CLK_REQUIRE(B <= (uint64_t)ctx->max_threads_per_block, CLK_ERR_INVALID_ARGUMENT,
                 "mass_update: local_size exceeds the device limit");
-    const int vec = pick_vector_width(cfg ? cfg->vector_length : 4, n, labels, labels);
+    const int vec = pick_vector_width(cfg ? cfg->vector_length : 4);
     const uint64_t user_grid = (cfg && cfg->global_size[0]) ? div_up(cfg->global_size[0], B) : 0;
 
     // LDS-fit fallbacks: private -> local -> part_global.

@@ -14,8 +14,10 @@
 //
 // Error bound of one computed score (u = 2^-24, μ = absolute input error of a
 // 16-bit subnormal the matrix core might flush, P = padded dimension,
-// X >= ||x||, C >= max ||c||, N >= max ||c||^2, and the measured rounding
-// residuals  Rx >= ||x̃ - x|| (per point, from the converted values) and
+// X >= ||x||, C >= max ||c||, N >= max ||c||^2, the rounding bound
+// Rx >= ||x̃ - x|| = u16 X + s16 √d (per point: relative 2^-11 / 2^-8 per
+// coordinate for FP16 / BF16, half the 16-bit subnormal spacing below the
+// normal range; mfma::rounding_bound) and the measured centroid residual
 // R >= max over centroids of ||c̃ - (-2c)||):
 //   inputs           (X + Rx') (R + μ√d) + Rx' 2C,  Rx' = Rx + μ√d
 //                    (Σ x̃(c̃ + 2c) + Σ (x̃ - x)(-2c), each by Cauchy-Schwarz)
@@ -204,9 +206,9 @@ label_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
         for (int g = 0; g < kGroups; ++g)
             pid[g] = bt * 64 + g * 16 + q;
         PointOps<NKB> x;
-        float X2[kGroups], RX2[kGroups];
-        load_points<FP16, NKB, FULL>(x, X2, RX2, pts, n, d, pid, q, h);
-        const WaveBounds w = wave_bounds<NKB>(X2, RX2, stats, prm);
+        float X2[kGroups];
+        load_points<FP16, NKB, FULL>(x, X2, pts, n, d, pid, q, h);
+        const WaveBounds w = wave_bounds<NKB, FP16>(X2, stats, prm);
         uint32_t best[kGroups], second[kGroups];
         rank_all<FP16, NKB>(lds_table, lds_norm, ntiles, lane, h, w.K, kmask, x, best, second);
 
@@ -295,9 +297,9 @@ refine_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32
             thr_in[g] = ok ? qthr[slot] : 0.0f;
         }
         PointOps<NKB> x;
-        float X2[kGroups], RX2[kGroups];
-        load_points<FP16, NKB, FULL>(x, X2, RX2, pts, n, d, pid, q, h, xs);
-        const WaveBounds w = wave_bounds<NKB>(X2, RX2, stats, prm);
+        float X2[kGroups];
+        load_points<FP16, NKB, FULL>(x, X2, pts, n, d, pid, q, h, xs);
+        const WaveBounds w = wave_bounds<NKB, FP16>(X2, stats, prm);
         // candidate threshold on this kernel's scores: T + K' + E', rounded up;
         // NaN or infinite -> every centroid is a candidate
         float thr[kGroups];

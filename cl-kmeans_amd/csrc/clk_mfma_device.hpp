@@ -229,11 +229,10 @@ __device__ __forceinline__ float exact_dist(const float *xrow, const float *__re
 // for group g, point pid[g] (>= n: none; zeros), features kb*32 + 8h + j
 // (0 past d).  Every load address is in bounds (a missing point reads point
 // n-1, a missing feature reads feature 0; the value is then replaced by 0).
-// X2[g] >= ||x||^2 and RX2[g] >= ||x̃ - x||^2 (FP32 sums over the 4 lanes of
-// a point: at most P/4 + 2 roundings, plus underflow; x - x̃ is exact).
+// X2[g] >= ||x||^2 (an FP32 sum over the 4 lanes of a point: at most P/4 + 2
+// roundings, plus underflow).
 template <bool FP16, int NKB, bool FULL, bool KEEP>
 __device__ __forceinline__ void load_points_core(PointOps<NKB> &x, float (&X2)[kGroups],
-                                                 float (&RX2)[kGroups],
                                                  const float *__restrict__ pts, uint64_t n,
                                                  uint32_t d, const uint64_t (&pid)[kGroups],
                                                  uint32_t q, uint32_t h, float *xs,
@@ -274,7 +273,7 @@ __device__ __forceinline__ void load_points_core(PointOps<NKB> &x, float (&X2)[k
             for (int j = 0; j < NKB * 8; ++j)
                 keep[g][j] = raw[j];
         }
-        float s = 0.0f, r = 0.0f;
+        float s = 0.0f;
 #pragma unroll
         for (int kb = 0; kb < NKB; ++kb) {
             u16x8 vb;
@@ -283,42 +282,35 @@ __device__ __forceinline__ void load_points_core(PointOps<NKB> &x, float (&X2)[k
                 const float v = raw[kb * 8 + j];
                 s = __builtin_fmaf(v, v, s);
                 vb[j] = to16<FP16>(v);
-                const float res = v - from16<FP16>(vb[j]);
-                r = __builtin_fmaf(res, res, r);
             }
             x.b[g][kb] = vb;
         }
         s += xor_swizzle<16>(s);
         s += __shfl_xor(s, 32);
-        r += xor_swizzle<16>(r);
-        r += __shfl_xor(r, 32);
         X2[g] = s * (1.0f + (float)(2 * NKB * kKBlock + 8) * kU) + 1e-37f;
-        RX2[g] = r * (1.0f + (float)(2 * NKB * kKBlock + 8) * kU) + 1e-37f;
     }
 }
 
 template <bool FP16, int NKB, bool FULL>
 __device__ __forceinline__ void load_points(PointOps<NKB> &x, float (&X2)[kGroups],
-                                            float (&RX2)[kGroups],
                                             const float *__restrict__ pts, uint64_t n, uint32_t d,
                                             const uint64_t (&pid)[kGroups], uint32_t q, uint32_t h,
                                             float *xs = nullptr)
 {
     float unused[kGroups][NKB * 8];
-    load_points_core<FP16, NKB, FULL, false>(x, X2, RX2, pts, n, d, pid, q, h, xs, unused);
+    load_points_core<FP16, NKB, FULL, false>(x, X2, pts, n, d, pid, q, h, xs, unused);
 }
 
 // As load_points, and keep[g][kb*8 + j] = the FP32 value of feature
 // kb*32 + 8h + j of group g's point (0 past n or d).
 template <bool FP16, int NKB, bool FULL>
 __device__ __forceinline__ void load_points_keep(PointOps<NKB> &x, float (&X2)[kGroups],
-                                                 float (&RX2)[kGroups],
                                                  const float *__restrict__ pts, uint64_t n,
                                                  uint32_t d, const uint64_t (&pid)[kGroups],
                                                  uint32_t q, uint32_t h,
                                                  float (&keep)[kGroups][NKB * 8])
 {
-    load_points_core<FP16, NKB, FULL, true>(x, X2, RX2, pts, n, d, pid, q, h, nullptr, keep);
+    load_points_core<FP16, NKB, FULL, true>(x, X2, pts, n, d, pid, q, h, nullptr, keep);
 }
 
 // The loads of load_points alone, for software pipelining: raw[g][kb*8 + j]
@@ -350,14 +342,14 @@ __device__ __forceinline__ void fetch_points(float (&raw)[kGroups][NKB * 8], con
     }
 }
 
-// The operands, X2 and RX2 of load_points from fetch_points' values.
+// The operands and X2 of load_points from fetch_points' values.
 template <bool FP16, int NKB>
 __device__ __forceinline__ void build_operands(const float (&raw)[kGroups][NKB * 8], PointOps<NKB> &x,
-                                               float (&X2)[kGroups], float (&RX2)[kGroups])
+                                               float (&X2)[kGroups])
 {
 #pragma unroll
     for (int g = 0; g < kGroups; ++g) {
-        float s = 0.0f, r = 0.0f;
+        float s = 0.0f;
 #pragma unroll
         for (int kb = 0; kb < NKB; ++kb) {
             u16x8 vb;
@@ -366,17 +358,12 @@ __device__ __forceinline__ void build_operands(const float (&raw)[kGroups][NKB *
                 const float v = raw[g][kb * 8 + j];
                 s = __builtin_fmaf(v, v, s);
                 vb[j] = to16<FP16>(v);
-                const float res = v - from16<FP16>(vb[j]);
-                r = __builtin_fmaf(res, res, r);
             }
             x.b[g][kb] = vb;
         }
         s += xor_swizzle<16>(s);
         s += __shfl_xor(s, 32);
-        r += xor_swizzle<16>(r);
-        r += __shfl_xor(r, 32);
         X2[g] = s * (1.0f + (float)(2 * NKB * kKBlock + 8) * kU) + 1e-37f;
-        RX2[g] = r * (1.0f + (float)(2 * NKB * kKBlock + 8) * kU) + 1e-37f;
     }
 }
 
@@ -388,9 +375,27 @@ struct WaveBounds {
     float Xn[kGroups];     // >= ||x||
 };
 
-template <int NKB>
-__device__ __forceinline__ WaveBounds wave_bounds(const float (&X2)[kGroups],
-                                                  const float (&RX2)[kGroups], const float *stats,
+// The 16-bit rounding of a point, ||x̃ - x|| <= u16 ||x|| + s16 sqrt(d):
+// each coordinate moves by at most u16 |x_f| (round to nearest, u16 = 2^-11
+// for FP16, 2^-8 for BF16) or, below the smallest normal, by half the
+// subnormal spacing s16 (2^-25 for FP16; BF16's 2^-134, bounded by 1e-38).
+// An FP16 coordinate of magnitude >= 65520 becomes infinite: a point whose
+// ||x||^2 bound reaches 65520^2 gets an infinite bound (every centroid is a
+// candidate), as does every non-finite point.
+template <bool FP16>
+__device__ __forceinline__ float rounding_bound(float X, float X2, float sqrt_d)
+{
+    constexpr float u16 = FP16 ? 4.8828125e-4f : 3.90625e-3f;  // 2^-11, 2^-8
+    constexpr float s16 = FP16 ? 2.9802322e-8f : 1e-38f;       // 2^-25, > 2^-134
+    const float r = (u16 * X + s16 * sqrt_d) * (1.0f + 4.0f * kU);
+    if constexpr (FP16)
+        return X2 < 4.29287e9f ? r : __builtin_inff();  // 65520^2 = 4.2928704e9
+    else
+        return r;
+}
+
+template <int NKB, bool FP16>
+__device__ __forceinline__ WaveBounds wave_bounds(const float (&X2)[kGroups], const float *stats,
                                                   const MfmaParams &prm)
 {
     const float Cx = stats[0], Nx = stats[1], R = stats[3];
@@ -401,7 +406,7 @@ __device__ __forceinline__ WaveBounds wave_bounds(const float (&X2)[kGroups],
 #pragma unroll
     for (int g = 0; g < kGroups; ++g) {
         const float X = sqrtf(X2[g]) * (1.0f + 4.0f * kU);
-        const float Rx = sqrtf(RX2[g]) * (1.0f + 4.0f * kU) + ms;
+        const float Rx = rounding_bound<FP16>(X, X2[g], prm.sqrt_d) + ms;
         const float in = (X + Rx) * (R + ms) + Rx * 2.0f * Cx;
         const float Pm = (X + Rx) * (2.0f * Cx + R + ms);
         w.Xn[g] = X;

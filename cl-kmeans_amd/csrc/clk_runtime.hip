@@ -74,20 +74,11 @@ KernelTimer::~KernelTimer()
         (void)hipEventRecord(ctx->records[(size_t)rec].stop, stream);
 }
 
-int pick_vector_width(uint64_t requested, uint64_t n, const void *points, const void *labels)
+int pick_vector_width(uint64_t requested)
 {
-    int w = 4;
     if (requested == 1 || requested == 2 || requested == 4)
-        w = (int)requested;
-    else if (requested >= 8)
-        w = 4;
-    for (; w > 1; w /= 2) {
-        uintptr_t a = reinterpret_cast<uintptr_t>(points);
-        uintptr_t b = reinterpret_cast<uintptr_t>(labels);
-        if (n % (uint64_t)w == 0 && a % (4u * w) == 0 && b % (4u * w) == 0)
-            break;
-    }
-    return w;
+        return (int)requested;
+    return 4;
 }
 
 uint64_t auto_grid(clk_context *ctx, const void *kernel, int block, size_t dyn_lds,

@@ -7,8 +7,12 @@
 //   pipeline: the _buffered pipelines' SimpleBufferCache +
 //     SingleDeviceScheduler (two device slots, copy and compute streams)
 //     with a zero fill of every chunk on the compute stream.
+// --verify checks every byte that moved: the device buffer after each H2D
+// mode, the host buffer after the D2H, and (the pipeline then copies each
+// chunk's points to its label slot instead of zeroing them, and writes the
+// labels back) the host label mirror; a mismatch exits with status 3.
 // Usage: clk_transfer_bench [--size MiB] [--buffer-sizes 1,4,16,64,256]
-//                           [--repeat R] [--device D]
+//                           [--repeat R] [--device D] [--verify]
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -47,6 +51,7 @@ int main(int argc, char **argv)
 {
     uint64_t mib = 2048;
     int repeat = 3, device = 0;
+    bool verify = false;
     std::vector<uint64_t> sizes = {1, 4, 16, 64, 256};
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -63,6 +68,8 @@ int main(int argc, char **argv)
             repeat = std::atoi(next().c_str());
         } else if (a == "--device") {
             device = std::atoi(next().c_str());
+        } else if (a == "--verify") {
+            verify = true;
         } else if (a == "--buffer-sizes") {
             sizes.clear();
             std::stringstream ss(next());
@@ -70,7 +77,8 @@ int main(int argc, char **argv)
             while (std::getline(ss, tok, ','))
                 sizes.push_back(std::strtoull(tok.c_str(), nullptr, 10));
         } else {
-            std::cerr << "Usage: clk_transfer_bench [--size MiB] [--buffer-sizes 1,4,16] [--repeat R] [--device D]\n";
+            std::cerr << "Usage: clk_transfer_bench [--size MiB] [--buffer-sizes 1,4,16] [--repeat R] [--device D] "
+                         "[--verify]\n";
             return a == "--help" ? 0 : 2;
         }
     }
@@ -89,6 +97,20 @@ int main(int argc, char **argv)
         std::memcpy(pinned.data(), pageable.data(), bytes);
         DeviceVector<float> dev(n);
         Queue q(device);
+        std::vector<float> back(verify ? n : 0);
+        // byte equality of `got` with the source data, or exit 3
+        auto check = [&](const void *got, const char *what, uint64_t bs_mib) {
+            if (std::memcmp(got, pageable.data(), bytes) != 0) {
+                std::fprintf(stderr, "clk_transfer_bench: %s with %llu MiB buffers: data differ\n", what,
+                             (unsigned long long)bs_mib);
+                std::exit(3);
+            }
+        };
+        auto check_device = [&](const char *what, uint64_t bs_mib) {
+            check_hip(hipMemcpy(back.data(), dev.data(), bytes, hipMemcpyDeviceToHost), "verify D2H");
+            check(back.data(), what, bs_mib);
+            check_hip(hipMemset(dev.data(), 0, bytes), "verify clear");
+        };
         std::printf("buffer_MiB,pinned_h2d_GBps,pageable_h2d_GBps,pinned_d2h_GBps,pipeline_GBps\n");
         for (uint64_t bs_mib : sizes) {
             const size_t chunk = std::max<size_t>(bs_mib << 20, sizeof(float)) / sizeof(float);
@@ -110,8 +132,17 @@ int main(int argc, char **argv)
                 });
             };
             const double t_pin = copy_all(pinned.data(), hipMemcpyHostToDevice);
+            if (verify)
+                check_device("pinned H2D", bs_mib);
             const double t_page = copy_all(pageable.data(), hipMemcpyHostToDevice);
+            if (verify) {
+                check_device("pageable H2D", bs_mib);
+                check_hip(hipMemcpy(dev.data(), pageable.data(), bytes, hipMemcpyHostToDevice), "verify H2D");
+                std::memset(pinned.data(), 0, bytes);
+            }
             const double t_d2h = copy_all(pinned.data(), hipMemcpyDeviceToHost);
+            if (verify)
+                check(pinned.data(), "pinned D2H", bs_mib);
             // the _buffered pipeline: d = 1 "points", chunks of bs bytes
             SimpleBufferCache<float, uint32_t> cache;
             cache.setup(q, pageable, 1, n, bs_mib << 20);
@@ -119,12 +150,21 @@ int main(int argc, char **argv)
                 SingleDeviceScheduler<float, uint32_t> sched(cache, q);
                 q.finish();
                 const auto t0 = std::chrono::steady_clock::now();
-                sched.run(false, [](const Queue &cq, size_t cn, float *pts, uint32_t *) {
-                    check_hip(hipMemsetAsync(pts, 0, cn * sizeof(float), cq.get()), "zero");
+                sched.run(verify, [&](const Queue &cq, size_t cn, float *pts, uint32_t *labels) {
+                    if (verify)
+                        check_hip(hipMemcpyAsync(labels, pts, cn * sizeof(float), hipMemcpyDeviceToDevice, cq.get()),
+                                  "chunk copy");
+                    else
+                        check_hip(hipMemsetAsync(pts, 0, cn * sizeof(float), cq.get()), "zero");
                 });
                 sched.finish();
                 return seconds(t0);
             });
+            if (verify) {
+                std::vector<uint32_t> mirror;
+                cache.copy_labels_out(mirror);
+                check(mirror.data(), "pipeline", bs_mib);
+            }
             const double gb = (double)bytes / 1e9;
             std::printf("%llu,%.2f,%.2f,%.2f,%.2f\n", (unsigned long long)bs_mib, gb / t_pin, gb / t_page,
                         gb / t_d2h, gb / t_pipe);

@@ -122,8 +122,9 @@ typedef struct clk_serve_stats {
     double solves_per_s;      /* jobs / makespan */
     double p50, p99;          /* latency (finished - arrival), nearest rank, seconds */
     double mean_latency;
-    uint64_t steps;           /* batch iterations launched */
+    uint64_t steps;           /* iterations launched (batch steps + lone-job steps) */
     uint32_t max_active;      /* most jobs iterating at once */
+    uint64_t solo_steps;      /* of those, iterations of a lone job through the fused pass */
 } clk_serve_stats;
 
 /* Continuous batching over an empty batch: a job is admitted once its
@@ -131,7 +132,11 @@ typedef struct clk_serve_stats {
  * order), its points are copied in on a second stream while the resident jobs
  * iterate, it joins the next iteration's launch, and it leaves (results
  * copied to its out_* buffers) when it reaches its tolerance or iteration
- * limit.  A job's results are those it gets alone in a batch. */
+ * limit.  While one job of at most 2^20 points is resident and no other has
+ * arrived, its iterations run through the per-job fused pass (as
+ * clk_lloyd_step, with the job's sum grid) instead of the batch step.  A
+ * job's results are those it gets alone in a batch, whichever path ran each
+ * of its iterations. */
 int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, clk_serve_stats *stats, void *stream);
 
 /* The plain per-job loop for the same arrivals: one job at a time, copied in

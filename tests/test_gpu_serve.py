@@ -128,3 +128,39 @@ def test_serve_rejects_bad_input(gpu_ctx, stream):
     assert e.value.code == clkmeans.CLK_ERR_NOMEM
     small.close()
     b.close()
+
+
+@pytest.mark.parametrize("labeling", ["mfma_bf16", "unroll_vector"])
+def test_lone_job_path_equals_batch_path(gpu_ctx, stream, labeling):
+    """At low load a lone job iterates through the per-job fused pass and
+    switches back to the batch step when others arrive (MFMA jobs get their
+    16-bit table rebuilt from their current centroids).  Whatever path ran
+    each iteration, every result equals the same job alone in a batch."""
+    # MFMA-class jobs (k >= 16, k*d >= 2048) and FP32-class jobs; long runs
+    # (no tolerance) so that later arrivals meet jobs that iterated alone
+    specs = [(20000, 32, 100, 3), (6000, 3, 12, 4), (30000, 16, 200, 5), (9000, 8, 40, 6), (4096, 32, 64, 7),
+             (12000, 8, 300, 8)]
+    reqs = []
+    # arrivals spread over ~5 ms, each job taking ~1-2 ms: the first runs
+    # alone for a while, later ones overlap, the last finishes alone
+    for (n, d, k, s), t in zip(specs, [0.0, 0.0004, 0.0007, 0.0025, 0.0027, 0.005]):
+        pts, _ = clkmeans.generate_blobs(n, d, k, sigma=2.0, box=10.0, seed=s)
+        reqs.append(clkmeans.ServeRequest(pts, O.first_x(pts, k), t, max_iterations=40, tolerance=0.0))
+    b = clkmeans.Batch(gpu_ctx, max_jobs=4, arena_bytes=256 << 20, max_d=32, max_k=300, labeling=labeling)
+    stats = clkmeans.serve(b, reqs, stream.handle)
+    b.close()
+    assert stats["solo_steps"] > 0 and stats["steps"] > stats["solo_steps"]
+    for r in reqs:
+        c, l, m, it, shift = run_alone(gpu_ctx, r, stream, labeling, max_k=300)
+        assert r.iterations == it == 40
+        assert r.shift == shift
+        np.testing.assert_array_equal(r.out_centroids.view(np.uint32), c.view(np.uint32))
+        np.testing.assert_array_equal(r.out_labels, l)
+        np.testing.assert_array_equal(r.out_masses, m)
+    # and the plain per-job loop
+    sreqs = [clkmeans.ServeRequest(r.points, r.centroids, r.arrival, max_iterations=40, tolerance=0.0)
+             for r in reqs]
+    clkmeans.serve_serial(gpu_ctx, sreqs, stream.handle)
+    for a, r in zip(reqs, sreqs):
+        np.testing.assert_array_equal(a.out_centroids.view(np.uint32), r.out_centroids.view(np.uint32))
+        np.testing.assert_array_equal(a.out_labels, r.out_labels)
