@@ -597,9 +597,11 @@ __host__ __device__ inline uint32_t keep_row_width(uint32_t d)
 {
     return d <= 4 ? 4u : d <= 8 ? 8u : d <= 16 ? 16u : d <= 32 ? 32u : 0u;
 }
-// Jobs with fewer clusters scan every centroid: their scan is too short to
-// repay the keep test and the previous label read.
+// FP32-scan jobs with fewer clusters and k*d below kKeepMinKd scan every
+// centroid: their scan (about 2*k*d VALU ops per point) is too short to
+// repay the keep test (2*d ops, a threshold and the previous label read).
 constexpr uint32_t kKeepMinK = 16;
+constexpr uint32_t kKeepMinKd = 96;
 
 template <int DB>
 __device__ __forceinline__ double keep_min_rows(const float *__restrict__ cs, uint32_t a, uint32_t c0, uint32_t c1)
@@ -1178,7 +1180,7 @@ batch_finish_kernel(BJob *__restrict__ jobs, const uint32_t *__restrict__ active
     // hide only a cluster that turned empty (finite -> NaN), whose old
     // centroid in the minimum only makes the other thresholds smaller.
     // MFMA jobs with d <= 32 also get their centroid rows every iteration.
-    const bool fp32_keep = J.kind == 0 && J.k >= kKeepMinK;
+    const bool fp32_keep = J.kind == 0 && (J.k >= kKeepMinK || J.k * J.d >= kKeepMinKd);
     const bool mfma_keep = J.kind == 1 && J.cen_t_w != 0 && (uint64_t)J.k * J.cen_t_w <= kKeepStage;
     if (J.state == CLK_JOB_ACTIVE && (fp32_keep || mfma_keep)) {
         const bool thr = !(J.keep_ok && J.shift == 0.0);
@@ -2372,7 +2374,7 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
             if (lone >= 0) {
                 const BJob &J = b->jobs[lone];  // host record: buffers and sizes
                 const uint32_t it = host_iter[lone];
-                const clk::GridRef g{J.gmax, clk::grid_bits(J.n)};
+                const clk::GridRef g{J.gmax, clk::grid_bits(J.n), 1u};  // computed from J.pts at admission
                 r = clk::lloyd_step_with_grid(b->ctx, &solo_cfg, J.d, J.n, J.k, J.pts, J.cen[it & 1],
                                               J.cen[(it & 1) ^ 1], J.part, J.labels, J.masses, solo_shift, g, st);
                 if (r == CLK_OK) {

@@ -67,11 +67,13 @@ struct clk_context {
     clk::Scratch part_masses;     // per-block uint32 partial masses
     clk::Scratch mfma_tables;     // 16-bit centroid operand table, norms, bounds
     clk::Scratch mfma_queue;      // point ids queued for exact refinement
+    clk::Scratch mfma_hint;       // largest per-wave score offset K of the last MFMA labeling
     uint32_t *mfma_qlen = nullptr;  // device queue length inside mfma_tables
     clk::Scratch grid;            // sum grid: per-feature max |x| bits [d] (clk_grid.hpp)
     bool grid_pinned = false;     // grid holds a caller-pinned grid for grid_d features
     char *noalloc_scratch = nullptr;  // this device's static scratch (clk_lloyd_step_noalloc)
     uint64_t grid_d = 0, grid_n = 0;
+    const void *grid_src = nullptr;  // the points a grid was pinned from (null: from maxima)
     int mfma_subnormals = -1;     // bit 0 FP16, bit 1 BF16: MFMA keeps subnormal inputs (-1 unknown)
     bool profiling = false;
     int32_t iteration = -1;

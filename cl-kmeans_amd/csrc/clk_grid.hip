@@ -77,12 +77,14 @@ int resolve_grid(clk_context *ctx, uint64_t d, uint64_t n, const float *points, 
                     "the sum grid pinned on the context has another number of features");
         out->gmax = static_cast<const uint32_t *>(ctx->grid.ptr);
         out->bits = grid_bits(ctx->grid_n);
+        out->own = ctx->grid_src != nullptr && ctx->grid_src == points && ctx->grid_n == n;
         return CLK_OK;
     }
     CLK_TRY(ensure_scratch(ctx->grid, (d + 1) * sizeof(uint32_t)));
     CLK_TRY(launch_grid_max(ctx, d, n, points, static_cast<uint32_t *>(ctx->grid.ptr), stream));
     out->gmax = static_cast<const uint32_t *>(ctx->grid.ptr);
     out->bits = grid_bits(n);
+    out->own = 1;
     return CLK_OK;
 }
 
@@ -102,6 +104,7 @@ extern "C" int clk_sum_grid_from_points(clk_context *ctx, uint64_t d, uint64_t n
     ctx->grid_pinned = true;
     ctx->grid_d = d;
     ctx->grid_n = n;
+    ctx->grid_src = points;
     return CLK_OK;
 }
 
@@ -125,6 +128,7 @@ extern "C" int clk_sum_grid_from_max(clk_context *ctx, uint64_t d, uint64_t n, c
     ctx->grid_pinned = true;
     ctx->grid_d = d;
     ctx->grid_n = n;
+    ctx->grid_src = nullptr;
     return CLK_OK;
 }
 
@@ -132,6 +136,7 @@ extern "C" int clk_sum_grid_clear(clk_context *ctx)
 {
     CLK_REQUIRE(ctx != nullptr, CLK_ERR_INVALID_ARGUMENT, "null clk_context");
     ctx->grid_pinned = false;
+    ctx->grid_src = nullptr;
     return CLK_OK;
 }
 

@@ -46,9 +46,13 @@ constexpr int kGridMinExp = -100;
 // Kernel argument: per-feature max |x| bit patterns (finite values; the bits
 // of a non-negative float order like its value) and b.  gmax[d] is non-zero
 // when the grid's data hold a NaN or an infinity (or when it is not known).
+// own: the grid was computed from the very points the kernel reads (per
+// call, or pinned from the same buffer and n); then, when gmax[d] is 0,
+// every coordinate is finite and on the grid and needs no range check.
 struct GridRef {
     const uint32_t *gmax;
     uint32_t bits;
+    uint32_t own = 0;
 };
 
 // b = min(25, 53 - ceil(log2(n + 1))).
@@ -141,7 +145,7 @@ int launch_grid_max(clk_context *ctx, uint64_t d, uint64_t n, const float *point
 
 // The grid the sum-accumulating ops of ctx use for (d, n, points): the
 // pinned one (clk_sum_grid_from_*) when set, else computed now into the
-// context's scratch.
+// context's scratch.  out->own: computed now, or pinned from these points.
 int resolve_grid(clk_context *ctx, uint64_t d, uint64_t n, const float *points, hipStream_t stream,
                  GridRef *out);
 

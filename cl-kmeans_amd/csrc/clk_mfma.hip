@@ -157,9 +157,17 @@ mfma_prepare_kernel(const float *__restrict__ cen, uint32_t d, uint32_t k, uint3
         atomicOr(reinterpret_cast<uint32_t *>(stats) + 2, 1u);
 }
 
-__global__ void __launch_bounds__(64) mfma_prepare_finish_kernel(float *__restrict__ stats)
+// stats[5] = the launch-wide score offset K_L for this labeling: 1.02 x the
+// largest per-wave K of the previous labeling on this context (hint[0],
+// reset here for this labeling to record its own), or 0 (none).  A wave
+// whose own K is <= K_L uses K_L with the norms pre-offset in LDS; any K >=
+// ||x||^2 + E is valid, so K_L only changes which waves take that path.
+__global__ void __launch_bounds__(64) mfma_prepare_finish_kernel(float *__restrict__ stats, float *__restrict__ hint)
 {
     if (threadIdx.x == 0) {
+        const float hv = hint[0];
+        stats[5] = (hv > 0.0f && hv < 1e30f) ? hv * 1.02f : 0.0f;
+        hint[0] = 0.0f;
         const unsigned long long *acc = reinterpret_cast<const unsigned long long *>(stats + 12);
         const double n = __longlong_as_double((long long)acc[0]) * (1.0 + 1e-12);
         const double r = __longlong_as_double((long long)acc[1]);
@@ -180,17 +188,26 @@ __global__ void __launch_bounds__(64) mfma_prepare_finish_kernel(float *__restri
 // appended to qidx[] / qthr[] (T, above; count at *qlen).
 constexpr int fast_waves(int nkb) { return nkb <= 2 ? 16 : 4; }
 
+// Score offset: each wave needs K >= ||x||^2 + E over its points (wave_bounds).
+// With a launch-wide K_L (stats[5]; 0 = none) the norms are staged a second
+// time as RN(||c||^2 + K_L), and a wave whose own K is <= K_L starts its
+// accumulators from those (no add per tile); the others add their own K.
+// Every wave's own K goes to hint[0] (atomic max of positive float bits) for
+// the next labeling's K_L.
 template <bool FP16, int NKB, bool FULL>
 __global__ void __launch_bounds__(64 * fast_waves(NKB))
 label_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_t k,
                   const u16x8 *__restrict__ table, const float *__restrict__ norm,
                   const float *__restrict__ stats, uint32_t ntiles, MfmaParams prm,
                   uint32_t *__restrict__ labels, uint32_t *__restrict__ qidx,
-                  float *__restrict__ qthr, uint32_t *__restrict__ qlen)
+                  float *__restrict__ qthr, uint32_t *__restrict__ qlen, float *__restrict__ hint)
 {
     extern __shared__ __align__(16) u16x8 lds_table[];
     float *lds_norm = reinterpret_cast<float *>(lds_table + (uint64_t)ntiles * NKB * 64);
-    stage_table<NKB>(lds_table, lds_norm, table, norm, ntiles);
+    float *lds_normk = lds_norm + (size_t)ntiles * kTile;
+    const float KL = stats[5];
+    stage_table<NKB>(lds_table, lds_norm, table, norm, ntiles, KL > 0.0f ? lds_normk : nullptr, KL);
+    float kmax = 0.0f;  // this wave's largest own K
     const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const uint32_t q = lane % 16, h = lane / 16;
     const bool table_ok = stats[2] == 0.0f;
@@ -208,9 +225,15 @@ label_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
         PointOps<NKB> x;
         float X2[kGroups];
         load_points<FP16, NKB, FULL>(x, X2, pts, n, d, pid, q, h);
-        const WaveBounds w = wave_bounds<NKB, FP16>(X2, stats, prm);
+        WaveBounds w = wave_bounds<NKB, FP16>(X2, stats, prm);
+        kmax = fmaxf(kmax, w.K);  // (a NaN K is skipped)
         uint32_t best[kGroups], second[kGroups];
-        rank_all<FP16, NKB>(lds_table, lds_norm, ntiles, lane, h, w.K, kmask, x, best, second);
+        if (KL > 0.0f && w.K <= KL) {  // wave-uniform
+            w.K = KL;
+            rank_all<FP16, NKB, false>(lds_table, lds_normk, ntiles, lane, h, w.K, kmask, x, best, second);
+        } else {
+            rank_all<FP16, NKB>(lds_table, lds_norm, ntiles, lane, h, w.K, kmask, x, best, second);
+        }
 
         // margin test; lane l owns point bt*64 + l (group h, column q)
         uint32_t mine = 0;
@@ -252,6 +275,8 @@ label_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
             }
         }
     }
+    if (lane == 0 && kmax > 0.0f)
+        atomicMax(reinterpret_cast<uint32_t *>(hint), __float_as_uint(kmax));
 }
 
 constexpr uint32_t kPairCap = 1024;  // candidate pairs per wave and flush
@@ -380,7 +405,7 @@ refine_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32
 
 using LabelKernelM = void (*)(const float *, uint64_t, uint32_t, uint32_t, const u16x8 *,
                               const float *, const float *, uint32_t, MfmaParams, uint32_t *,
-                              uint32_t *, float *, uint32_t *);
+                              uint32_t *, float *, uint32_t *, float *);
 using RefineKernelM = void (*)(const float *, uint64_t, uint32_t, uint32_t, const float *,
                                const u16x8 *, const float *, const float *, uint32_t, MfmaParams,
                                uint32_t *, const uint32_t *, const float *, const uint32_t *);
@@ -447,7 +472,7 @@ bool mfma_labeling_supported(const clk_context *ctx, uint64_t d, uint64_t n, uin
     if (d == 0 || d > (uint64_t)kMaxKb * kKBlock || k == 0 || k > (1u << 20) || n >= (1ull << 32))
         return false;
     const MfmaLayout l = mfma_layout(d, k);
-    return l.table_bytes + l.norm_bytes <= ctx->lds_per_block && l.refine_lds <= ctx->lds_per_block;
+    return l.table_bytes + 2 * l.norm_bytes <= ctx->lds_per_block && l.refine_lds <= ctx->lds_per_block;
 }
 
 int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, uint64_t k,
@@ -464,6 +489,11 @@ int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, ui
     float *stats = norm + (size_t)l.ntiles * kTile;  // 16 floats
     float *cenT = stats + 16;
     uint32_t *qlen = reinterpret_cast<uint32_t *>(stats + 4);
+    if (!ctx->mfma_hint.ptr) {  // no offset hint before the first labeling
+        CLK_TRY(ensure_scratch(ctx->mfma_hint, 256));
+        CLK_TRY(clk_fill_zero(ctx, ctx->mfma_hint.ptr, 256, stream));
+    }
+    float *hint = static_cast<float *>(ctx->mfma_hint.ptr);
     uint32_t *qidx = static_cast<uint32_t *>(ctx->mfma_queue.ptr);
     float *qthr = reinterpret_cast<float *>(qidx + std::max<uint64_t>(n, 1));
     ctx->mfma_qlen = qlen;
@@ -479,7 +509,7 @@ int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, ui
         else
             hipLaunchKernelGGL(mfma_prepare_kernel<false>, dim3(pgrid), dim3(256), 0, stream, centroids,
                                (uint32_t)d, (uint32_t)k, l.ntiles, l.nkb, table, norm, stats, cenT);
-        hipLaunchKernelGGL(mfma_prepare_finish_kernel, dim3(1), dim3(64), 0, stream, stats);
+        hipLaunchKernelGGL(mfma_prepare_finish_kernel, dim3(1), dim3(64), 0, stream, stats, hint);
         CLK_HIP_CHECK(hipGetLastError());
     }
     if (n == 0)
@@ -494,7 +524,7 @@ int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, ui
         full ? pick_mfma<true, true>(l.nkb, lk, rk) : pick_mfma<true, false>(l.nkb, lk, rk);
     else
         full ? pick_mfma<false, true>(l.nkb, lk, rk) : pick_mfma<false, false>(l.nkb, lk, rk);
-    const size_t lds = l.table_bytes + l.norm_bytes;
+    const size_t lds = l.table_bytes + 2 * l.norm_bytes;  // table, norms, norms + K_L
     const size_t lds_r = l.refine_lds;
     CLK_HIP_CHECK(allow_lds((const void *)lk, lds));
     CLK_HIP_CHECK(allow_lds((const void *)rk, lds_r));
@@ -504,7 +534,7 @@ int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, ui
         KernelTimer t(ctx, fp16 ? "LabelingMfmaF16" : "LabelingMfmaBF16", stream);
         hipLaunchKernelGGL(lk, dim3((unsigned)grid), dim3(fast_block), lds, stream, points, n,
                            (uint32_t)d, (uint32_t)k, table, norm, stats, l.ntiles, prm, labels,
-                           qidx, qthr, qlen);
+                           qidx, qthr, qlen, hint);
         CLK_HIP_CHECK(hipGetLastError());
     }
     // a resident grid; blocks past the queue return at once

@@ -145,13 +145,18 @@ struct PointOps {
 
 // Scores of one centroid tile for the wave's 4 point groups: lane (q, h)
 // gets rows 4h..4h+3 (centroids t*16 + 4h + i) of point column q.
-template <bool FP16, int NKB>
+// ADD_K = false: the norms already hold RN(||c||^2 + K).
+template <bool FP16, int NKB, bool ADD_K = true>
 __device__ __forceinline__ void score_tile(f32v4 (&acc)[kGroups], const TileOps<NKB> &o, float K,
                                            const PointOps<NKB> &x)
 {
 #pragma unroll
-    for (int g = 0; g < kGroups; ++g)
-        acc[g] = o.nrm + K;
+    for (int g = 0; g < kGroups; ++g) {
+        if constexpr (ADD_K)
+            acc[g] = o.nrm + K;
+        else
+            acc[g] = o.nrm;
+    }
 #pragma unroll
     for (int kb = 0; kb < NKB; ++kb)
 #pragma unroll
@@ -424,11 +429,12 @@ __device__ __forceinline__ WaveBounds wave_bounds(const float (&X2)[kGroups], co
     return w;
 }
 
-// Best / second keys over all centroid tiles, one tile at a time (the
-// kernels run enough waves per SIMD -- four in the streaming labeling -- that
-// other waves cover the LDS and MFMA latency; software-pipelining two tiles
-// in one wave needed more registers than four waves per SIMD leave).
-template <bool FP16, int NKB>
+// Best / second keys over all centroid tiles, one tile at a time, the next
+// tile's operands read from LDS while this one is scored and ranked (the
+// kernels also run several waves per SIMD -- four in the streaming labeling
+// -- to cover the MFMA latency; holding two tiles' scores in one wave needed
+// more registers than that leaves).
+template <bool FP16, int NKB, bool ADD_K = true>
 __device__ __forceinline__ void rank_all(const u16x8 *lds_table, const float *lds_norm,
                                          uint32_t ntiles, uint32_t lane, uint32_t h, float K,
                                          uint32_t kmask, const PointOps<NKB> &x,
@@ -440,7 +446,7 @@ __device__ __forceinline__ void rank_all(const u16x8 *lds_table, const float *ld
     const uint32_t vkeep = opaque_v(~kmask);
     for (uint32_t t = 0; t < ntiles; ++t) {
         f32v4 acc[kGroups];
-        score_tile<FP16, NKB>(acc, load_ops<NKB>(lds_table, lds_norm, t, lane, h), K, x);
+        score_tile<FP16, NKB, ADD_K>(acc, load_ops<NKB>(lds_table, lds_norm, t, lane, h), K, x);
         rank_tile(acc, t, vkeep, best, second);
     }
 #pragma unroll
@@ -464,17 +470,24 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Stage the 16-bit operand table and the norms in LDS (whole work-group).
+// Stage the 16-bit operand table and the norms in LDS (whole work-group);
+// lds_normk (optional): the norms plus K, RN(||c||^2 + K), the same values
+// the per-wave accumulator initialisation computes.
 template <int NKB>
 __device__ __forceinline__ void stage_table(u16x8 *lds_table, float *lds_norm,
                                             const u16x8 *__restrict__ table,
-                                            const float *__restrict__ norm, uint32_t ntiles)
+                                            const float *__restrict__ norm, uint32_t ntiles,
+                                            float *lds_normk = nullptr, float K = 0.0f)
 {
     const uint64_t entries = (uint64_t)ntiles * NKB * 64;
     for (uint64_t e = threadIdx.x; e < entries; e += blockDim.x)
         lds_table[e] = table[e];
-    for (uint32_t c = threadIdx.x; c < ntiles * kTile; c += blockDim.x)
-        lds_norm[c] = norm[c];
+    for (uint32_t c = threadIdx.x; c < ntiles * kTile; c += blockDim.x) {
+        const float v = norm[c];
+        lds_norm[c] = v;
+        if (lds_normk)
+            lds_normk[c] = v + K;
+    }
     __syncthreads();
 }
 

@@ -149,7 +149,10 @@ int clk_profiling_get(clk_context *ctx, int64_t i, char *name, size_t name_len,
  *   labels[p]  = argmin_c sum_f fma-sequence (x_f - c_f)^2   (overwrite)
  *   masses[c] += #{p : labels[p] == c}                       (accumulate)
  *   sums[f,c] += sum_{p : labels[p] == c} x_f                (accumulate)
- * old_centroids is read only; sums / masses must not alias it.
+ * old_centroids is read only; sums / masses must not alias it.  One pass
+ * over the points when k*d fits the work-group LDS tiles; otherwise the
+ * labeling, mass and centroid stages run one after another (the same
+ * results).
  */
 int clk_fused(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
               uint64_t n, uint64_t k, const float *points,
@@ -299,7 +302,11 @@ int clk_init_first_x(clk_context *ctx, uint64_t d, uint64_t n, uint64_t k,
  * so their chunked (_buffered) pipelines give the unchunked sums.  A point
  * outside a pinned grid's range (|x_f| >= 2^E_f) makes its cell's sum NaN.
  */
-/* Compute the grid of device points[d][n] (on stream) and pin it. */
+/* Compute the grid of device points[d][n] (on stream) and pin it.  Later
+ * ops on this same buffer and n take it as that data's own grid: when the
+ * data hold no NaN or infinity their coordinates are quantized without a
+ * range check, so pin again (or clear) after changing the buffer's
+ * contents.  Ops on other buffers check every coordinate against it. */
 int clk_sum_grid_from_points(clk_context *ctx, uint64_t d, uint64_t n,
                              const float *points, void *stream);
 /* Pin the grid of n points whose per-feature max |x| the caller knows
