@@ -1396,10 +1396,10 @@ constexpr uint32_t kJobStaged = 3;
 // cores (the 16-bit path has a fixed cost per point: conversion, bounds,
 // margin test, and d is padded to 32; the scan's keep test skips most of its
 // work from the second iteration on).  Measured on the 4096-job mix (d in
-// {3, 8, 16, 32}, k up to 256), solves/s by the k*d threshold: 1024: 16.6k,
-// 1536: 18.1k, 2048: 19.0k, 3072: 18.9k, 4096: 16.8k, never: 14.6k.  The
-// labels are the same either way.
-constexpr uint64_t kMfmaMinKD = 2048;
+// {3, 8, 16, 32}, k up to 256), solves/s by the k*d threshold: 1024: 18.6k,
+// 1536: 19.4k, 2048: 20.4k, 3072: 21.3k, 4096: 20.5-21.2k, 6144: 19.7k,
+// 12288: 17.1k.  The labels are the same either way.
+constexpr uint64_t kMfmaMinKD = 3072;
 
 bool use_mfma(const clk_batch *b, uint32_t d, uint32_t k)
 {
