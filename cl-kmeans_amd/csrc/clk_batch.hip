@@ -438,12 +438,20 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
     const bool special = J.gmax[d] != 0;  // the job holds NaN / inf
     float xn[DMAX];
     uint32_t ln = 0;
+    // byte offsets into the points fit 32 bits (n*d <= 2^30, checked at
+    // admission): one base, one 32-bit offset advanced by a row per feature
+    const uint32_t row_bytes = (uint32_t)n * 4u;
     auto fetch = [&](uint64_t b0) {
         const uint64_t p = b0 + tid;
         const uint64_t pc = p < end ? p : first;
+        uint32_t off = (uint32_t)pc * 4u;
 #pragma unroll
-        for (int f = 0; f < DMAX; ++f)
-            xn[f] = pts[(uint64_t)min((uint32_t)f, d - 1) * n + pc];
+        for (int f = 0; f < DMAX; ++f) {
+            xn[f] = *(const gfloat *)((const __attribute__((address_space(1))) char *)pts + off);
+            // next row; past d: feature d-1 again, zeroed at use (the step
+            // is wave-uniform: a scalar select)
+            off += (f + 1 < (int)d) ? row_bytes : 0u;
+        }
         // previous labels only matter to jobs with the keep test; the
         // others read one cached word instead of streaming the labels
         ln = prev_labels[keep_ok ? pc : first];
@@ -880,9 +888,12 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         {
             // unconditional in-bounds loads (past d: feature d-1, zeroed)
             const uint64_t pc = vme ? pme : first;
+            const uint32_t off = (uint32_t)pc * 4u;  // 32-bit byte offsets (n*d <= 2^30)
+            const uint32_t row_bytes = (uint32_t)n * 4u;
 #pragma unroll
             for (int f = 0; f < (int)P; ++f) {
-                const float v = gpts[(uint64_t)min((uint32_t)f, d - 1) * n + pc];
+                const uint32_t row = (uint32_t)min(f, (int)d - 1) * row_bytes;
+                const float v = *(const gfloat *)((const __attribute__((address_space(1))) char *)gpts + (off + row));
                 xc[f] = f < (int)d ? v : 0.0f;
             }
             prev = glabels[keep_ok ? pc : first];  // one cached word without the keep test
@@ -1772,7 +1783,8 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
               const float *host_centroids, uint32_t max_iterations, double tolerance, int32_t *slot,
               hipStream_t st, bool device_points = false, bool zero_outputs = true)
 {
-    CLK_REQUIRE(n >= 1 && n < (1ull << 32), CLK_ERR_SIZE, "clk_batch_add: n must be 1..2^32-1");
+    CLK_REQUIRE(n >= 1 && n <= (1ull << 30) && n * d <= (1ull << 30), CLK_ERR_SIZE,
+                "clk_batch_add: n must be >= 1 with n*d <= 2^30 (32-bit byte offsets into the points)");
     CLK_REQUIRE(d >= 1 && d <= b->cfg.max_d && k >= 1 && k <= b->cfg.max_k, CLK_ERR_SIZE,
                 "clk_batch_add: d or k beyond the batch's max_d / max_k");
     CLK_REQUIRE(host_points && host_centroids, CLK_ERR_INVALID_ARGUMENT, "clk_batch_add: null data");
@@ -2167,7 +2179,9 @@ extern "C" int clk_serve_run(clk_batch *b, clk_serve_job *jobs, uint32_t count, 
         CLK_REQUIRE(b->jobs[s].state == CLK_JOB_FREE, CLK_ERR_INVALID_ARGUMENT, "clk_serve_run: batch not empty");
     for (uint32_t i = 0; i < count; ++i) {  // every job fits an empty batch, so admission cannot stall
         const clk_serve_job &j = jobs[i];
-        CLK_REQUIRE(j.n >= 1 && j.d >= 1 && j.d <= b->cfg.max_d && j.k >= 1 && j.k <= b->cfg.max_k, CLK_ERR_SIZE,
+        CLK_REQUIRE(j.n >= 1 && j.d >= 1 && j.d <= b->cfg.max_d && j.k >= 1 && j.k <= b->cfg.max_k &&
+                        j.n <= (1ull << 30) && j.n * j.d <= (1ull << 30),
+                    CLK_ERR_SIZE,
                     "clk_serve_run: a job's n, d or k is outside the batch's limits");
         CLK_REQUIRE(job_bytes(j.n, j.d, j.k, row_tile_points(b->tile_work, j.d, j.k), j.device_points == 0) <=
                         b->cfg.arena_bytes,

@@ -54,7 +54,10 @@ int clk_batch_destroy(clk_batch *b);
  * to the device.  It runs until max_iterations iterations or, when
  * tolerance > 0, until the max centroid shift (FP64, ||new_c - old_c||_2 over
  * non-empty clusters) falls below tolerance.  CLK_ERR_NOMEM when no slot or
- * arena space is free (the caller retries after taking finished jobs). */
+ * arena space is free (the caller retries after taking finished jobs).
+ * CLK_ERR_SIZE unless n >= 1 and n*d <= 2^30 (the tile kernels address a
+ * job's points with 32-bit byte offsets; larger data sets run through
+ * clk_lloyd_step). */
 int clk_batch_add(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *host_points,
                   const float *host_centroids, uint32_t max_iterations, double tolerance,
                   int32_t *slot, void *stream);

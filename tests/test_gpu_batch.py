@@ -138,6 +138,18 @@ def test_batch_tolerance_stops_jobs(gpu_ctx, stream):
     b.close()
 
 
+def test_batch_rejects_jobs_beyond_32bit_offsets(gpu_ctx, stream):
+    """A job of more than 2^30 coordinates is refused with CLK_ERR_SIZE before
+    anything is read or copied (np.zeros leaves its pages untouched)."""
+    b = clkmeans.Batch(gpu_ctx, max_jobs=2, arena_bytes=64 << 20, max_d=32, max_k=8)
+    pts = np.zeros((32, (1 << 25) + 1), np.float32)
+    with pytest.raises(clkmeans.ClkError) as e:
+        b.add(pts, np.zeros((32, 8), np.float32), 2, stream=stream.handle)
+    assert e.value.code == clkmeans.CLK_ERR_SIZE
+    del pts
+    b.close()
+
+
 def test_batch_admission_by_footprint(gpu_ctx, stream):
     pts, init = make_job(100000, 16, 20, seed=1)
     cfg_bytes = clkmeans.Batch(gpu_ctx, 1, 1 << 20, 16, 20).job_bytes(100000, 16, 20)
