@@ -181,14 +181,12 @@ inline BLayout mfma_layout_for(uint32_t waves, uint32_t kmax, uint32_t kdmax, ui
     return L;
 }
 
-// Floats between the staged centroid rows of the plain kernel: rows of 8 and
-// 16 features get 4 floats of padding, so the lanes of a wave reading
+// Floats between the staged centroid rows of the plain kernel: rows of 8, 16
+// and 32 features get 4 floats of padding, so the lanes of a wave reading
 // different centroids (the keep test: each lane its previous label's row;
 // the wave-wide rescan: each lane another centroid) hit different LDS banks
 // with their 128-bit reads, where unpadded rows start on the same few banks.
-// (Rows of 32 features stay unpadded: the d <= 32 kernel is held to 168
-// VGPRs, and the padded addressing spills.)
-__host__ __device__ constexpr uint32_t cen_stride(uint32_t dm) { return (dm == 8 || dm == 16) ? dm + 4 : dm; }
+__host__ __device__ constexpr uint32_t cen_stride(uint32_t dm) { return (dm == 8 || dm == 16 || dm == 32) ? dm + 4 : dm; }
 
 // LDS of the plain kernel: the exact accumulator (largest k, largest k*d) |
 // centroids [k][cen_stride(DMAX)] | W rescan lists [64]
@@ -441,6 +439,12 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
     // byte offsets into the points fit 32 bits (n*d <= 2^30, checked at
     // admission): one base, one 32-bit offset advanced by a row per feature
     const uint32_t row_bytes = (uint32_t)n * 4u;
+    // The feature count as the loop body sees it.  At DMAX = 32 it is made
+    // opaque at every pass (an empty asm), so the per-feature tests (f < d)
+    // are evaluated where they are used instead of being hoisted out of the
+    // loop as 32 masks that crowd the scalar registers and spill into vector
+    // ones (smaller buckets keep their hoisted masks: they fit).
+    uint32_t dq = d;
     auto fetch = [&](uint64_t b0) {
         const uint64_t p = b0 + tid;
         const uint64_t pc = p < end ? p : first;
@@ -450,7 +454,7 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
             xn[f] = *(const gfloat *)((const __attribute__((address_space(1))) char *)pts + off);
             // next row; past d: feature d-1 again, zeroed at use (the step
             // is wave-uniform: a scalar select)
-            off += (f + 1 < (int)d) ? row_bytes : 0u;
+            off += (f + 1 < (int)dq) ? row_bytes : 0u;
         }
         // previous labels only matter to jobs with the keep test; the
         // others read one cached word instead of streaming the labels
@@ -460,10 +464,12 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
     for (uint64_t base = first; base < end; base += blockDim.x) {
         const uint64_t p = base + tid;
         const bool valid = p < end;
+        if constexpr (DMAX >= 32)
+            asm volatile("" : "+s"(dq));
         float x[DMAX][1];
 #pragma unroll
         for (int f = 0; f < DMAX; ++f)
-            x[f][0] = (f < (int)d) ? xn[f] : 0.0f;
+            x[f][0] = (f < (int)dq) ? xn[f] : 0.0f;
         const uint32_t prev = ln;
         const uint32_t a = prev < k ? prev : 0u;
         // the keep threshold is loaded before the next pass's points, so
@@ -491,7 +497,7 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
             // few failures: the whole wave rescans those points; many: each
             // failing lane scans its own
             if (__builtin_popcountll(fail) <= 16) {
-                const uint32_t l = sparse_scan<DMAX, S>(x, cen_s, d, k, fail, lane, slot);
+                const uint32_t l = sparse_scan<DMAX, S>(x, cen_s, dq, k, fail, lane, slot);
                 if (valid && !keep)
                     lab[0] = l;
             } else if (valid && !keep) {
@@ -507,7 +513,7 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
         bool off = false;
 #pragma unroll
         for (int f = 0; f < DMAX; ++f)
-            if (f < (int)d) {
+            if (f < (int)dq) {
                 int32_t q;
                 if (special) {
                     bool ok;
@@ -527,7 +533,7 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
             if (off) {
 #pragma unroll
                 for (int f = 0; f < DMAX; ++f)
-                    if (f < (int)d && !(__builtin_fabsf(x[f][0] * acc.scl[f]) < kGridLimit))
+                    if (f < (int)dq && !(__builtin_fabsf(x[f][0] * acc.scl[f]) < kGridLimit))
                         atomicOr(&J.flags[(size_t)f * k + lab[0]], special_flag(x[f][0]));
             }
         }
@@ -871,7 +877,12 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
     // Rounds: in round rd wave w labels batch rd*W + w; the run sums go to
     // the work-group's accumulator (exact integer atomics, any order).
     const uint32_t rounds = (uint32_t)((nb + waves - 1) / waves);
+    // d as the rounds see it, opaque at every round (an empty asm): the
+    // per-feature tests are evaluated where they are used, not hoisted out
+    // of the loop as masks that crowd the scalar registers
+    uint32_t dq = d;
     for (uint32_t rd = 0; rd < rounds; ++rd) {
+        asm volatile("" : "+s"(dq));
         const uint64_t bt = (uint64_t)rd * waves + wave;
         WaveRuns r;
         r.last = false;
@@ -892,9 +903,9 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
             const uint32_t row_bytes = (uint32_t)n * 4u;
 #pragma unroll
             for (int f = 0; f < (int)P; ++f) {
-                const uint32_t row = (uint32_t)min(f, (int)d - 1) * row_bytes;
+                const uint32_t row = (uint32_t)min(f, (int)dq - 1) * row_bytes;
                 const float v = *(const gfloat *)((const __attribute__((address_space(1))) char *)gpts + (off + row));
-                xc[f] = f < (int)d ? v : 0.0f;
+                xc[f] = f < (int)dq ? v : 0.0f;
             }
             prev = glabels[keep_ok ? pc : first];  // one cached word without the keep test
         }
@@ -911,7 +922,7 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
                     const f32v4 c4 = rows[(size_t)a * w4 + q];
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        if (4 * q + j < (int)d) {
+                        if (4 * q + j < (int)dq) {
                             const float t = xc[4 * q + j] - c4[j];
                             dist = __builtin_fmaf(t, t, dist);
                         }
@@ -949,7 +960,7 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
                 float xs[P];
 #pragma unroll
                 for (int f = 0; f < (int)P; ++f)
-                    xs[f] = f < (int)d ? __shfl(xc[f], (int)src) : 0.0f;
+                    xs[f] = f < (int)dq ? __shfl(xc[f], (int)src) : 0.0f;
                 float best = FLT_MAX;
                 uint32_t bi = 0xFFFFFFFFu;
                 for (uint32_t c = r; c < k; c += G) {
@@ -960,7 +971,7 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
                             const f32v4 c4 = rows[(size_t)c * w4 + q];
 #pragma unroll
                             for (int j = 0; j < 4; ++j)
-                                if (4 * q + j < (int)d) {
+                                if (4 * q + j < (int)dq) {
                                     const float t = xs[4 * q + j] - c4[j];
                                     dist = __builtin_fmaf(t, t, dist);
                                 }
@@ -1079,7 +1090,7 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         if (special) {
 #pragma unroll
             for (int f = 0; f < (int)P; ++f)
-                if (f < (int)d) {
+                if (f < (int)dq) {
                     bool ok;
                     xc[f] = __int_as_float(grid_q(xc[f], acc.scl[f], ok));
                     off = off || (mp < n && !ok);
@@ -1087,7 +1098,7 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         } else {
 #pragma unroll
             for (int f = 0; f < (int)P; ++f)
-                if (f < (int)d)
+                if (f < (int)dq)
                     xc[f] = __int_as_float(grid_q_on(xc[f], acc.scl[f]));
         }
         if (special && __builtin_expect(__ballot(off) != 0, 0)) {
@@ -1101,13 +1112,13 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         }
 #pragma unroll
         for (int f = 0; f < (int)P; ++f)
-            if (f < (int)d)
+            if (f < (int)dq)
                 xc[f] = __int_as_float(run_sum((int32_t)__shfl(__float_as_int(xc[f]), (int)r.src), r));
         }
         if (r.last) {
 #pragma unroll
             for (int f = 0; f < (int)P; ++f)
-                if (f < (int)d)
+                if (f < (int)dq)
                     lds_add64(acc.A + (uint32_t)f * k + r.label, (long long)__float_as_int(xc[f]));
             lds_add(acc.cnt + r.label, r.count);
         }
