@@ -412,6 +412,9 @@ def dense(args):
     mcfg = ctx.config(clkmeans.STAGE_MASS_UPDATE, "part_local")
     ccfg = ctx.config(clkmeans.STAGE_CENTROID_UPDATE, "cluster_merge")
     ctx.sum_grid_from_points(d, n, dp, stream=s)  # pinned once (setup)
+    pinned = args.labeling.startswith("mfma") and not args.no_pin
+    if pinned:  # the 16-bit operand copy of the points, once (setup)
+        ctx.labeling_pin_points(args.labeling, d, n, dp, stream=s)
     stream.synchronize()
 
     def iteration(lab_cfg):
@@ -462,12 +465,14 @@ def dense(args):
     plain_ms = float(sum(ms for _, _, ms in ctx.records()))
     plain_equal = bool(np.array_equal(labels.to_host(), lab))
     ctx.sum_grid_clear()
+    ctx.labeling_unpin_points()
     ctx.close()
     del dp, cen, sums, masses, labels, c_dev
     out = {
         "workload": "dense Lloyd, N=2^%d d=64 k=1024 float32 feature-major, sigma=0.5 blobs in [-1,1]^64, seed 0, "
-                    "first-x init; three_stage: %s labeling, part_local masses, feature-slice exact sums, FP64 "
-                    "divide" % (log2n, args.labeling),
+                    "first-x init; three_stage: %s labeling%s, part_local masses, feature-slice exact sums, "
+                    "FP64 divide" % (log2n, args.labeling,
+                                     " (points pinned once as its 16-bit operand copy)" if pinned else ""),
         "value": 1000.0 / ms_per_step, "unit": "iter/s", "ms_per_step": ms_per_step, "steps": steps,
         "dtype": "f16-mfma+f32-exact" if args.labeling == "mfma_fp16" else
                  "bf16-mfma+f32-exact" if args.labeling == "mfma_bf16" else "f32",
@@ -800,6 +805,8 @@ def main():
                     help="serve workload: points per cluster in a tile (0: the engine's default)")
     ap.add_argument("--batch-labeling", default="mfma_bf16", help="batch and serve: labeling strategy")
     ap.add_argument("--labeling", default="mfma_fp16", help="dense workload: labeling strategy")
+    ap.add_argument("--no-pin", action="store_true",
+                    help="dense workload: label from the FP32 points instead of the pinned 16-bit operand copy")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scales rocprofv3 byte counters)")
     args = ap.parse_args()

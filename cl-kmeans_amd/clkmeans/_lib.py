@@ -55,6 +55,7 @@ EXPORTS = [
     "clk_matrix_add_u32", "clk_matrix_add_f64", "clk_reduce_vector_parcol_u32",
     "clk_reduce_vector_parcol_f64", "clk_fill_zero", "clk_init_first_x", "clk_generate_blobs",
     "clk_kmeans_run", "clk_conf_describe", "clk_lloyd_step", "clk_labeling_refined_points",
+    "clk_labeling_pin_points", "clk_labeling_unpin_points",
     "clk_batch_job_bytes", "clk_batch_create", "clk_batch_destroy", "clk_batch_add", "clk_batch_step",
     "clk_batch_poll", "clk_batch_take", "clk_batch_job_buffers", "clk_batch_autotune", "clk_batch_run_serial",
     "clk_serve_run", "clk_serve_serial",
@@ -165,6 +166,8 @@ def load(path=None):
                                      ctypes.POINTER(u64), ctypes.POINTER(u64)]),
         "clk_conf_describe": (i32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
         "clk_labeling_refined_points": (i32, [vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "clk_labeling_pin_points": (i32, [vp, i32, u64, u64, vp, vp]),
+        "clk_labeling_unpin_points": (i32, [vp]),
         "clk_batch_job_bytes": (u64, [ctypes.POINTER(BatchConfig), u64, ctypes.c_uint32, ctypes.c_uint32]),
         "clk_batch_create": (i32, [vp, ctypes.POINTER(BatchConfig), ctypes.POINTER(vp)]),
         "clk_batch_destroy": (i32, [vp]),
@@ -336,6 +339,15 @@ class Context:
         out = ctypes.c_uint64(0)
         check(self.L.clk_labeling_refined_points(self.handle, ctypes.byref(out)))
         return out.value
+
+    def labeling_pin_points(self, strategy, d, n, points, stream=None):
+        """Build the 16-bit operand copy of device points [d][n] that later MFMA
+        labelings of `strategy` ("mfma_bf16" / "mfma_fp16" or its id) read."""
+        sid = strategy_id(STAGE_LABELING, strategy) if isinstance(strategy, str) else int(strategy)
+        check(self.L.clk_labeling_pin_points(self.handle, sid, d, n, _p(points), _s(stream)))
+
+    def labeling_unpin_points(self):
+        check(self.L.clk_labeling_unpin_points(self.handle))
 
     def labeling(self, cfg, d, n, k, points, centroids, labels, stream=None):
         fn = self.L.clk_labeling_f64 if _is_f64(points) else self.L.clk_labeling

@@ -69,6 +69,10 @@ struct clk_context {
     clk::Scratch mfma_queue;      // point ids queued for exact refinement
     clk::Scratch mfma_hint;       // largest per-wave score offset K of the last MFMA labeling
     uint32_t *mfma_qlen = nullptr;  // device queue length inside mfma_tables
+    clk::Scratch mfma_pin;        // pinned 16-bit operand copy of points (clk_labeling_pin_points)
+    const void *mfma_pin_src = nullptr;  // the points it was built from (null: none)
+    uint64_t mfma_pin_d = 0, mfma_pin_n = 0;
+    bool mfma_pin_fp16 = false;
     clk::Scratch grid;            // sum grid: per-feature max |x| bits [d] (clk_grid.hpp)
     bool grid_pinned = false;     // grid holds a caller-pinned grid for grid_d features
     char *noalloc_scratch = nullptr;  // this device's static scratch (clk_lloyd_step_noalloc)

@@ -200,7 +200,8 @@ label_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
                   const u16x8 *__restrict__ table, const float *__restrict__ norm,
                   const float *__restrict__ stats, uint32_t ntiles, MfmaParams prm,
                   uint32_t *__restrict__ labels, uint32_t *__restrict__ qidx,
-                  float *__restrict__ qthr, uint32_t *__restrict__ qlen, float *__restrict__ hint)
+                  float *__restrict__ qthr, uint32_t *__restrict__ qlen, float *__restrict__ hint,
+                  const u16x8 *__restrict__ pin_ops, const f32v4 *__restrict__ pin_x2)
 {
     extern __shared__ __align__(16) u16x8 lds_table[];
     float *lds_norm = reinterpret_cast<float *>(lds_table + (uint64_t)ntiles * NKB * 64);
@@ -224,7 +225,19 @@ label_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
             pid[g] = bt * 64 + g * 16 + q;
         PointOps<NKB> x;
         float X2[kGroups];
-        load_points<FP16, NKB, FULL>(x, X2, pts, n, d, pid, q, h);
+        if (pin_ops) {  // the pinned copy: contiguous 128-bit loads
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g)
+#pragma unroll
+                for (int kb = 0; kb < NKB; ++kb)
+                    x.b[g][kb] = pin_ops[((bt * kGroups + g) * NKB + kb) * 64 + lane];
+            const f32v4 x2 = pin_x2[bt * 16 + q];
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g)
+                X2[g] = x2[g];
+        } else {
+            load_points<FP16, NKB, FULL>(x, X2, pts, n, d, pid, q, h);
+        }
         WaveBounds w = wave_bounds<NKB, FP16>(X2, stats, prm);
         kmax = fmaxf(kmax, w.K);  // (a NaN K is skipped)
         uint32_t best[kGroups], second[kGroups];
@@ -277,6 +290,41 @@ label_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
     }
     if (lane == 0 && kmax > 0.0f)
         atomicMax(reinterpret_cast<uint32_t *>(hint), __float_as_uint(kmax));
+}
+
+// The pinned operand copy of points[d][n] (clk_labeling_pin_points): for
+// each 64-point step bt, group g and K-block kb, the 64 lanes' B operands in
+// lane order -- ops[((bt * 4 + g) * NKB + kb) * 64 + lane] -- and each
+// point's X2 at x2[bt * 16 + q][g]: exactly what load_points gives the
+// labeling kernel's lane (q, h).  One wave per step.
+template <bool FP16, int NKB>
+__global__ void __launch_bounds__(256)
+mfma_pin_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, u16x8 *__restrict__ ops,
+                float *__restrict__ x2)
+{
+    const uint32_t lane = threadIdx.x % 64, q = lane % 16, h = lane / 16;
+    const uint64_t bt = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    if (bt >= (n + 63) / 64)
+        return;
+    uint64_t pid[kGroups];
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g)
+        pid[g] = bt * 64 + g * 16 + q;
+    PointOps<NKB> x;
+    float X2[kGroups];
+    float unused[kGroups][NKB * 8];
+    load_points_core<FP16, NKB, false, false>(x, X2, pts, n, d, pid, q, h, nullptr, unused);
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g)
+#pragma unroll
+        for (int kb = 0; kb < NKB; ++kb)
+            ops[((bt * kGroups + g) * NKB + kb) * 64 + lane] = x.b[g][kb];
+    // lane (q, h) writes group h's X2 (the four lanes of a column hold it)
+    float mine = X2[0];
+#pragma unroll
+    for (int g = 1; g < kGroups; ++g)
+        mine = (uint32_t)g == h ? X2[g] : mine;
+    x2[(bt * 16 + q) * 4 + h] = mine;
 }
 
 constexpr uint32_t kPairCap = 1024;  // candidate pairs per wave and flush
@@ -405,7 +453,7 @@ refine_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32
 
 using LabelKernelM = void (*)(const float *, uint64_t, uint32_t, uint32_t, const u16x8 *,
                               const float *, const float *, uint32_t, MfmaParams, uint32_t *,
-                              uint32_t *, float *, uint32_t *, float *);
+                              uint32_t *, float *, uint32_t *, float *, const u16x8 *, const f32v4 *);
 using RefineKernelM = void (*)(const float *, uint64_t, uint32_t, uint32_t, const float *,
                                const u16x8 *, const float *, const float *, uint32_t, MfmaParams,
                                uint32_t *, const uint32_t *, const float *, const uint32_t *);
@@ -437,6 +485,14 @@ MfmaLayout mfma_layout(uint64_t d, uint64_t k)
     l.refine_lds = kRefineWaves * (64 * l.nkb * kKBlock * sizeof(float) + kPairCap * sizeof(uint32_t) +
                                    64 * sizeof(unsigned long long) + sizeof(uint32_t));
     return l;
+}
+
+// Operands (u16x8) of the pinned copy of n points with nkb K-blocks; the
+// X2 array (one f32v4 per 16 points) follows them.
+uint64_t pin_ops_count(uint64_t n, uint32_t nkb) { return div_up(n, (uint64_t)64) * kGroups * nkb * 64; }
+uint64_t pin_bytes(uint64_t n, uint32_t nkb)
+{
+    return pin_ops_count(n, nkb) * sizeof(u16x8) + div_up(n, (uint64_t)64) * 16 * sizeof(f32v4);
 }
 
 hipError_t allow_lds(const void *kern, size_t lds)
@@ -532,9 +588,15 @@ int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, ui
     const uint64_t grid = auto_grid(ctx, (const void *)lk, fast_block, lds, div_up(n, (uint64_t)64) * 64);
     {
         KernelTimer t(ctx, fp16 ? "LabelingMfmaF16" : "LabelingMfmaBF16", stream);
+        // the pinned copy when it was built from these points for this strategy
+        const bool pinned = ctx->mfma_pin_src == (const void *)points && ctx->mfma_pin_n == n &&
+                            ctx->mfma_pin_d == d && ctx->mfma_pin_fp16 == fp16;
+        const u16x8 *pin_ops = pinned ? static_cast<const u16x8 *>(ctx->mfma_pin.ptr) : nullptr;
+        const f32v4 *pin_x2 =
+            pinned ? reinterpret_cast<const f32v4 *>(pin_ops + pin_ops_count(n, l.nkb)) : nullptr;
         hipLaunchKernelGGL(lk, dim3((unsigned)grid), dim3(fast_block), lds, stream, points, n,
                            (uint32_t)d, (uint32_t)k, table, norm, stats, l.ntiles, prm, labels,
-                           qidx, qthr, qlen, hint);
+                           qidx, qthr, qlen, hint, pin_ops, pin_x2);
         CLK_HIP_CHECK(hipGetLastError());
     }
     // a resident grid; blocks past the queue return at once
@@ -549,6 +611,52 @@ int launch_labeling_mfma(clk_context *ctx, bool fp16, uint64_t d, uint64_t n, ui
 }
 
 }  // namespace clk
+
+extern "C" int clk_labeling_pin_points(clk_context *ctx, int32_t strategy, uint64_t d, uint64_t n,
+                                       const float *points, void *stream)
+{
+    using namespace clk;
+    CLK_TRY(enter(ctx));
+    CLK_REQUIRE(strategy == CLK_LABELING_MFMA_BF16 || strategy == CLK_LABELING_MFMA_FP16,
+                CLK_ERR_INVALID_ARGUMENT, "clk_labeling_pin_points: strategy must be mfma_bf16 or mfma_fp16");
+    CLK_REQUIRE(d >= 1 && d <= (uint64_t)kMaxKb * kKBlock && n >= 1 && n < (1ull << 32), CLK_ERR_SIZE,
+                "clk_labeling_pin_points: needs 1 <= d <= 128 and 1 <= n < 2^32");
+    CLK_REQUIRE(points != nullptr, CLK_ERR_INVALID_ARGUMENT, "clk_labeling_pin_points: null points");
+    const bool fp16 = strategy == CLK_LABELING_MFMA_FP16;
+    const uint32_t nkb = (uint32_t)div_up(d, (uint64_t)kKBlock);
+    ctx->mfma_pin_src = nullptr;  // invalid until rebuilt
+    CLK_TRY(ensure_scratch(ctx->mfma_pin, pin_bytes(n, nkb)));
+    u16x8 *ops = static_cast<u16x8 *>(ctx->mfma_pin.ptr);
+    float *x2 = reinterpret_cast<float *>(ops + pin_ops_count(n, nkb));
+    const uint64_t steps = div_up(n, (uint64_t)64);
+    const dim3 grid((unsigned)div_up(steps, (uint64_t)4)), block(256);
+    const hipStream_t st = as_stream(stream);
+    using PinKernel = void (*)(const float *, uint64_t, uint32_t, u16x8 *, float *);
+    PinKernel kern;
+    switch (nkb) {
+    case 1: kern = fp16 ? mfma_pin_kernel<true, 1> : mfma_pin_kernel<false, 1>; break;
+    case 2: kern = fp16 ? mfma_pin_kernel<true, 2> : mfma_pin_kernel<false, 2>; break;
+    case 3: kern = fp16 ? mfma_pin_kernel<true, 3> : mfma_pin_kernel<false, 3>; break;
+    default: kern = fp16 ? mfma_pin_kernel<true, 4> : mfma_pin_kernel<false, 4>; break;
+    }
+    {
+        KernelTimer t(ctx, "MfmaPinPoints", st);
+        hipLaunchKernelGGL(kern, grid, block, 0, st, points, n, (uint32_t)d, ops, x2);
+        CLK_HIP_CHECK(hipGetLastError());
+    }
+    ctx->mfma_pin_src = points;
+    ctx->mfma_pin_d = d;
+    ctx->mfma_pin_n = n;
+    ctx->mfma_pin_fp16 = fp16;
+    return CLK_OK;
+}
+
+extern "C" int clk_labeling_unpin_points(clk_context *ctx)
+{
+    CLK_REQUIRE(ctx != nullptr, CLK_ERR_INVALID_ARGUMENT, "null clk_context");
+    ctx->mfma_pin_src = nullptr;
+    return CLK_OK;
+}
 
 extern "C" int clk_labeling_refined_points(clk_context *ctx, uint64_t *points)
 {

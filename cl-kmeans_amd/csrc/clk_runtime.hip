@@ -454,7 +454,7 @@ extern "C" int clk_context_destroy(clk_context *ctx)
     (void)hipDeviceSynchronize();
     if (ctx->part_sums.ptr)
         (void)hipFree(ctx->part_sums.ptr);
-    for (clk::Scratch *sc : {&ctx->part_masses, &ctx->mfma_tables, &ctx->mfma_queue, &ctx->mfma_hint, &ctx->grid})
+    for (clk::Scratch *sc : {&ctx->part_masses, &ctx->mfma_tables, &ctx->mfma_queue, &ctx->mfma_hint, &ctx->mfma_pin, &ctx->grid})
         if (sc->ptr)
             (void)hipFree(sc->ptr);
     for (auto &r : ctx->records) {

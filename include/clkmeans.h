@@ -210,6 +210,21 @@ int clk_labeling(clk_context *ctx, const clk_kernel_config *cfg, uint64_t d,
  * call).  No reference counterpart. */
 int clk_labeling_refined_points(clk_context *ctx, uint64_t *points);
 
+/* Keep a 16-bit copy of device points[d][n] in the operand order of the MFMA
+ * labeling `strategy` (CLK_LABELING_MFMA_BF16 or CLK_LABELING_MFMA_FP16;
+ * built on stream): 64 * ceil(d / 32) + 4 bytes per point.  Later MFMA
+ * labelings of that strategy on this same buffer, n and d read the copy --
+ * one contiguous stream -- instead of d strided FP32 rows per point (those
+ * touch d pages per 64 points, more than the address-translation cache
+ * holds); the exact refinement still reads the FP32 points, so the labels
+ * are the same.  Pin again after changing the buffer's contents.  A new pin
+ * replaces the old one.  No reference counterpart. */
+int clk_labeling_pin_points(clk_context *ctx, int32_t strategy, uint64_t d, uint64_t n,
+                            const float *points, void *stream);
+/* Drop the copy (its memory is kept for the next pin until the context is
+ * destroyed). */
+int clk_labeling_unpin_points(clk_context *ctx);
+
 /*
  * masses[c] += #{p : labels[p] == c}.  Replaces
  * MassUpdateFactory::MassUpdateFunction (mass_update_factory.hpp:36-48) and

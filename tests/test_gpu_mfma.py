@@ -174,3 +174,88 @@ def test_mfma_many_batches_per_wave_match_fp32_scan(gpu_ctx, strategy, n, d, k):
     ref = mfma_labels(gpu_ctx, "unroll_vector", pts, cen)
     bad = np.flatnonzero(got != ref)
     assert bad.size == 0, "%d labels differ, first at point %d" % (bad.size, bad[0])
+
+
+# ---- the pinned 16-bit operand copy (clk_labeling_pin_points) ------------
+
+PIN_CASES = [
+    # n, d, k, sigma, box
+    ((1 << 14) + 5, 64, 1024, 0.5, 1.0),  # the dense shape, reduced n, partial last step
+    (20000, 40, 64, 1.0, 10.0),           # partial second K-block
+    (7777, 17, 100, 1.0, 10.0),
+    (3001, 128, 300, 0.5, 1.0),           # four K-blocks
+    (63, 3, 17, 1.0, 10.0),               # one partial step
+]
+
+
+def pinned_labels(ctx, strategy, dp, n, d, cen, pin_strategy=None, pin_n=None):
+    k = cen.shape[1]
+    dl = zeros(max(n, 1), np.uint32)
+    ctx.labeling_pin_points(pin_strategy or strategy, d, pin_n or n, dp)
+    ctx.labeling(ctx.config(LAB, strategy), d, n, k, dp, dev(cen), dl)
+    hip.synchronize()
+    refined = ctx.labeling_refined_points()
+    ctx.labeling_unpin_points()
+    return dl.to_host()[:n], refined
+
+
+@pytest.mark.parametrize("strategy", PRECISIONS)
+@pytest.mark.parametrize("n,d,k,sigma,box", PIN_CASES)
+def test_mfma_pinned_points_match_unpinned(gpu_ctx, strategy, n, d, k, sigma, box):
+    # The copy holds exactly the operands and X2 the kernel would build from
+    # the FP32 points, so the scores, the margin tests (the refined count)
+    # and the labels are the same; the labels are the oracle's.  (The score
+    # offset a call starts from is the previous call's largest: the
+    # reference call follows one on the same data, as the pinned call does.)
+    pts, _ = clkmeans.generate_blobs(n, d, k, sigma=sigma, box=box, seed=n + d)
+    cen = np.ascontiguousarray(pts[:, :k]) if n >= k else np.ascontiguousarray(pts[:, np.arange(k) % n])
+    mfma_labels(gpu_ctx, strategy, pts, cen)
+    ref = mfma_labels(gpu_ctx, strategy, pts, cen)
+    ref_refined = gpu_ctx.labeling_refined_points()
+    got, refined = pinned_labels(gpu_ctx, strategy, dev(pts), n, d, cen)
+    np.testing.assert_array_equal(got, ref)
+    assert refined == ref_refined
+    sample = np.arange(0, n, 5)
+    np.testing.assert_array_equal(got[sample], O.label(np.ascontiguousarray(pts[:, sample]), cen))
+
+
+@pytest.mark.parametrize("strategy", PRECISIONS)
+def test_mfma_pinned_special_values(gpu_ctx, strategy):
+    pts, centres = clkmeans.generate_blobs(4096, 6, 12, sigma=1.0, box=10.0, seed=13)
+    pts = pts.copy()
+    pts[:, 100:164] *= np.float32(1e19)     # one step of huge points (exact scan)
+    pts[:, 500] = np.nan
+    pts[:, 501] = np.inf
+    got, _ = pinned_labels(gpu_ctx, strategy, dev(pts), 4096, 6, centres)
+    np.testing.assert_array_equal(got, O.label(pts, centres))
+
+
+def test_mfma_pin_is_used_only_for_its_points(gpu_ctx):
+    # Another buffer, another n, or the other 16-bit format: the labeling
+    # reads the FP32 points, so the labels stay the oracle's.
+    n, d, k = 5000, 24, 40
+    pts, centres = clkmeans.generate_blobs(n, d, k, sigma=1.0, box=10.0, seed=5)
+    other, _ = clkmeans.generate_blobs(n, d, k, sigma=1.0, box=10.0, seed=6)
+    dp, do = dev(pts), dev(other)
+    want_o = O.label(other, centres)
+    dl = zeros(n, np.uint32)
+    gpu_ctx.labeling_pin_points("mfma_fp16", d, n, dp)
+    for strategy, buf, nn, want in [("mfma_fp16", do, n, want_o),            # other buffer
+                                    ("mfma_bf16", dp, n, O.label(pts, centres)),  # other format
+                                    # the same buffer read as [d][n - 100]
+                                    ("mfma_fp16", dp, n - 100, O.label(pts.ravel()[:d * (n - 100)].reshape(d, n - 100),
+                                                                       centres))]:
+        gpu_ctx.labeling(gpu_ctx.config(LAB, strategy), d, nn, k, buf, dev(centres), dl)
+        hip.synchronize()
+        np.testing.assert_array_equal(dl.to_host()[:nn], want)
+    gpu_ctx.labeling_unpin_points()
+
+
+def test_mfma_pin_rejects_bad_arguments(gpu_ctx):
+    pts = dev(np.zeros((4, 100), np.float32))
+    with pytest.raises(clkmeans.InvalidArgument):
+        gpu_ctx.labeling_pin_points("unroll_vector", 4, 100, pts)
+    with pytest.raises(clkmeans.ClkError):
+        gpu_ctx.labeling_pin_points("mfma_fp16", 129, 100, pts)
+    with pytest.raises(clkmeans.ClkError):
+        gpu_ctx.labeling_pin_points("mfma_fp16", 4, 0, pts)
