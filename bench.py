@@ -35,6 +35,7 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 MFMA16_PEAK_TFLOPS = 2500.0  # dense FP16 / BF16
+FP32_PEAK_TFLOPS = 157.3     # FP32 vector
 METRIC = "Lloyd iter/s+%HBM (4f/10c 2GiB); served/batched vs plain solves/s @tol; p99"
 
 
@@ -551,6 +552,11 @@ def batch(args):
     points = sum(n for n, _, _, _ in mix)
     bytes_per_iter = sum(n * (4 * d + 4) for n, d, _, _ in mix)
     dist_flops = sum(2.0 * n * k * d for n, d, k, _ in mix)
+    # the engine labels jobs with k >= 16 and k*d >= 3072 on the matrix cores
+    # (MFMA labelings), the others with the FP32 scan
+    on_mfma = lambda d, k: args.batch_labeling.startswith("mfma") and k >= 16 and k * d >= 3072
+    mfma_flops = sum(2.0 * n * k * d for n, d, k, _ in mix if on_mfma(d, k))
+    fp32_flops = dist_flops - mfma_flops
 
     # tile sizes by the autotuner (dry passes over these jobs), then the
     # batch engine: graph capture on the first step, replays timed
@@ -599,6 +605,12 @@ def batch(args):
     it_ms = float(np.mean(step_ms)) if step_ms else batch_ms / iters
     ach_hbm = bytes_per_iter / (it_ms * 1e-3) / 1e9
     ach_flops = dist_flops / (it_ms * 1e-3) / 1e12
+    hbm_floor_ms = bytes_per_iter / (HBM_PEAK_GBS * 1e9) * 1e3
+    fp32_floor_ms = fp32_flops / (FP32_PEAK_TFLOPS * 1e12) * 1e3
+    mfma_floor_ms = mfma_flops / (MFMA16_PEAK_TFLOPS * 1e12) * 1e3
+    compute_floor_ms = fp32_floor_ms + mfma_floor_ms
+    bound = "hbm" if hbm_floor_ms >= compute_floor_ms else "compute (fp32 vector + mfma)"
+    floor_ms = max(hbm_floor_ms, compute_floor_ms)
     out = {
         "workload": "%d jobs: N log-uniform [2^10,2^18], d in {3,8,16,32}, k log-uniform [4,256] capped at N/16, "
                     "sigma=1 blobs in [-10,10]^d, per-job seeds, first-x init, %d iterations; batch engine (size-"
@@ -611,14 +623,19 @@ def batch(args):
         "serial_per_job_loop": {"solves_per_s": len(mix) / (serial_ms * 1e-3), "total_ms": serial_ms}
                                if serial_ms else None,
         "speedup_vs_serial": serial_ms / batch_ms if serial_ms else None,
-        "roofline": {"bound": "mfma", "achieved": ach_flops, "peak": MFMA16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": ach_flops / MFMA16_PEAK_TFLOPS, "traffic": None,
+        "roofline": {"bound": bound, "achieved": ach_hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": floor_ms / it_ms, "traffic": None,
                      "kernel": "one batch iteration (size-class tile kernels, merge, finish)",
-                     "flops_per_iteration": dist_flops, "iteration_ms": it_ms,
-                     "note": "2*sum(n*k*d) distance flops per iteration over its time; the FP32-scan classes run "
-                             "on the VALU (157.3 TFLOP/s peak) and skip most scans via the keep test",
-                     "hbm_GBps": ach_hbm, "hbm_frac": ach_hbm / HBM_PEAK_GBS,
-                     "bytes_per_iteration": bytes_per_iter},
+                     "bytes_per_iteration": bytes_per_iter, "iteration_ms": it_ms,
+                     "floor_ms": {"hbm": hbm_floor_ms, "fp32_classes_on_vector_peak": fp32_floor_ms,
+                                  "mfma_classes_on_matrix_peak": mfma_floor_ms},
+                     "flops_per_iteration": {"fp32_classes": fp32_flops, "mfma_classes": mfma_flops},
+                     "achieved_TFLOPs": ach_flops,
+                     "note": "floors: points + labels bytes at 8 TB/s; 2*n*k*d distance flops of the FP32-scan "
+                             "jobs at the 157.3 TFLOP/s vector peak plus those of the MFMA jobs at 2.5 PFLOP/s "
+                             "(compute and memory overlap: the larger floor is the bound); frac = floor / "
+                             "iteration time. The per-point labeling, run sort and LDS accumulation issue "
+                             "rates are the limit (PMC in profiles/batch_step_c4mix_4096jobs.md)"},
         "parity": {"jobs_labels_equal_serial": lab_equal, "jobs_masses_equal_serial": mass_equal,
                    "jobs_centroids_bit_equal_serial": cen_equal, "jobs_iterations_equal_serial": iters_equal,
                    "jobs_done": int((states == clkmeans.JOB_DONE).sum()), "still_active": int(active)},
