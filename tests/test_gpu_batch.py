@@ -67,6 +67,35 @@ def test_batch_matches_oracle(gpu_ctx, stream, labeling):
     b.close()
 
 
+ORDER_JOBS = [  # n, d, k
+    (20000, 3, 4), (15000, 16, 40), (9000, 32, 120), (30000, 8, 200), (4096, 32, 16), (7000, 2, 70),
+]
+
+
+@pytest.mark.parametrize("labeling", ["unroll_vector", "mfma_bf16"])
+@pytest.mark.parametrize("order", ["interleaved", "grouped", "shuffled"])
+def test_batch_point_orders(gpu_ctx, stream, labeling, order):
+    # The tiles add a wave's grid integers lane by lane when its labels are
+    # spread and along sorted runs when they repeat: points interleaved over
+    # the blobs (k = 4: 16 lanes per label, the most same-cell atomics the
+    # spread form takes), grouped by blob (runs of one label) and shuffled.
+    b = clkmeans.Batch(gpu_ctx, max_jobs=8, arena_bytes=128 << 20, max_d=32, max_k=256, labeling=labeling)
+    jobs = []
+    for i, (n, d, k) in enumerate(ORDER_JOBS):
+        pts, _ = clkmeans.generate_blobs(n, d, k, sigma=1.0, box=10.0, seed=300 + i)
+        init = np.ascontiguousarray(pts[:, :k])  # one point of each blob
+        if order == "grouped":
+            pts = np.ascontiguousarray(pts[:, np.argsort(np.arange(n) % k, kind="stable")])
+        elif order == "shuffled":
+            pts = np.ascontiguousarray(pts[:, np.random.default_rng(i).permutation(n)])
+        jobs.append((b.add(pts, init, max_iterations=5, stream=stream.handle), pts, init))
+    for _ in range(5):
+        b.step(stream.handle)
+    for s, pts, init in jobs:
+        check_job(pts, init, 5, b.take(s, stream.handle))
+    b.close()
+
+
 @pytest.mark.parametrize("labeling", ["unroll_vector", "mfma_bf16"])
 def test_batch_keep_radius_paths(gpu_ctx, stream, labeling):
     # From the second iteration on: points inside their centroid's keep
