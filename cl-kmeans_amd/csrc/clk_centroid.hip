@@ -110,34 +110,95 @@ cu_cluster_merge_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, u
             c = (c + 1 == k) ? 0u : c + 1;
         }
     };
+    // grid scales in registers for small buckets, read from LDS otherwise
+    constexpr bool kRegScale = DMAX <= 16;
+    float sc_reg[kRegScale ? DMAX : 1];
+    if constexpr (kRegScale) {
+#pragma unroll
+        for (int f = 0; f < DMAX; ++f)
+            sc_reg[f] = h.scl[f];
+    }
+    auto scale_of = [&](int f) -> float {
+        if constexpr (kRegScale)
+            return sc_reg[f];
+        else
+            return h.scl[f];
+    };
+    // the points' own grid with no NaN / inf in them: every coordinate is on
+    // the grid (no range checks, no flags)
+    const bool on_grid = grid.own != 0 && grid.gmax[d] == 0;
     // V points p .. p+V-1 (V = VEC, or 1 for the n % VEC tail points)
-    auto add_points = [&](auto vtag, uint64_t p) {
-        constexpr int V = decltype(vtag)::value;
-        float x[DMAX][V];
-        load_points<DMAX, V>(pts, n, d, p, x);
-        uint32_t lab[V];
-        load_labels<V>(labels, p, lab);
+    auto add_points = [&](const auto &x, const auto &lab) {
+        constexpr int V = (int)(sizeof(lab) / sizeof(uint32_t));
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             if (lab[v] >= k)
                 continue;
             uint32_t *t = mine + __umul24(lab[v], row_stride);
+            if (on_grid) {
 #pragma unroll
-            for (int f = 0; f < DMAX; ++f)
-                if (f < (int)d)
-                    add_private(t + f * B, x[f][v], h.scl[f], h, (uint32_t)f, k, lab[v]);
+                for (int f = 0; f < DMAX; ++f)
+                    if (f < (int)d)
+                        lds_add(t + f * B, (uint32_t)grid_q_on(x[f][v], scale_of(f)));
+            } else {
+#pragma unroll
+                for (int f = 0; f < DMAX; ++f)
+                    if (f < (int)d)
+                        add_private(t + f * B, x[f][v], scale_of(f), h, (uint32_t)f, k, lab[v]);
+            }
         }
         if (++since == kFlushSteps) {
             flush();
             since = 0;
         }
     };
-    const uint64_t id = (uint64_t)blockIdx.x * B + tid;
-    for (uint64_t g = id; g < groups; g += stride)
-        add_points(std::integral_constant<int, VEC>{}, g * VEC);
-    if (VEC > 1 && id == groups % stride)
-        for (uint64_t p = groups * VEC; p < n; ++p)
-            add_points(std::integral_constant<int, 1>{}, p);
+    auto load = [&](uint64_t gg, float (&x)[DMAX][VEC], uint32_t (&l)[VEC]) {
+        load_points<DMAX, VEC>(pts, n, d, gg * VEC, x);
+        load_labels<VEC>(labels, gg * VEC, l);
+    };
+    uint64_t g = (uint64_t)blockIdx.x * B + tid;
+    if constexpr (DMAX * VEC <= 32) {
+        // Software pipeline, two register buffers: one step's points and
+        // labels are in flight while the other step's are accumulated.
+        // Loads are unconditional (index clamped to the last group, always
+        // in bounds).  (Wider steps would not fit the registers twice.)
+        const uint64_t last = groups ? groups - 1 : 0;
+        float xa[DMAX][VEC], xb[DMAX][VEC];
+        uint32_t la[VEC], lb[VEC];
+        if (groups)
+            load(min(g, last), xa, la);
+        while (g < groups) {
+            const uint64_t g1 = g + stride;
+            load(min(g1, last), xb, lb);
+            add_points(xa, la);
+            if (g1 >= groups)
+                break;
+            const uint64_t g2 = g1 + stride;
+            load(min(g2, last), xa, la);
+            add_points(xb, lb);
+            g = g2;
+        }
+    } else {
+        for (; g < groups; g += stride) {
+            float x[DMAX][VEC];
+            uint32_t l[VEC];
+            load(g, x, l);
+            add_points(x, l);
+        }
+    }
+    if constexpr (VEC > 1) {
+        // the n % VEC points after the last group, one at a time, by the
+        // work-item that would take group `groups` next
+        if ((uint64_t)blockIdx.x * B + tid == groups % stride) {
+            for (uint64_t p = groups * VEC; p < n; ++p) {
+                float x1[DMAX][1];
+                uint32_t l1[1];
+                load_points<DMAX, 1>(pts, n, d, p, x1);
+                load_labels<1>(labels, p, l1);
+                add_points(x1, l1);
+            }
+        }
+    }
     if (since != 0)
         flush();
     __syncthreads();
