@@ -195,7 +195,7 @@ __host__ __device__ inline size_t batch_lds_bytes(uint32_t waves, uint32_t k, ui
     return acc_bytes(k, kd, dm) + (size_t)k * cen_stride(dm) * 4 + (size_t)waves * 64 * 4;
 }
 
-// Labels of the lanes in `fail` (at most 16 of the wave's lanes: points that
+// Labels of the lanes in `fail` (at most 32 of the wave's lanes: points that
 // failed the keep test), with the wave's lanes split over them: G = 64 /
 // next_pow2(m) lanes per point, lane r of a group scanning centroids r, r+G,
 // r+2G, ... with the reference arithmetic (strict '<' from FLT_MAX), then the
@@ -209,7 +209,7 @@ __device__ __forceinline__ uint32_t sparse_scan(const float (&x)[DMAX][1], const
                                                 uint32_t *slot)
 {
     const uint32_t m = (uint32_t)__builtin_popcountll(fail);
-    const uint32_t G = m <= 1 ? 64u : m <= 2 ? 32u : m <= 4 ? 16u : m <= 8 ? 8u : 4u;
+    const uint32_t G = m <= 1 ? 64u : m <= 2 ? 32u : m <= 4 ? 16u : m <= 8 ? 8u : m <= 16 ? 4u : 2u;
     const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(fail >> 32),
                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)fail, 0u));
     if ((fail >> lane) & 1ull)
@@ -503,7 +503,7 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
         if (fail != 0) {
             // few failures: the whole wave rescans those points; many: each
             // failing lane scans its own
-            if (__builtin_popcountll(fail) <= 16) {
+            if (__builtin_popcountll(fail) <= 32) {
                 const uint32_t l = sparse_scan<DMAX, S>(x, cen_s, dq, k, fail, lane, slot);
                 if (valid && !keep)
                     lab[0] = l;
