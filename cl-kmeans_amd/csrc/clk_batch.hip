@@ -1196,12 +1196,13 @@ constexpr uint32_t kMfmaClass0 = 12, kMfmaNkb2Class = 16;
 constexpr uint32_t kFp32Dmax[] = {3, 4, 8, 16, 32, 64};
 
 // The merge of every active job's tile rows: one work-group per element
-// slice of a job (large jobs get many).
+// slice of a job (large jobs get many).  It runs in the autotuner's dry
+// passes too (its cost depends on the tiling); the merged row 0 it leaves
+// is rewritten by the next iteration's tiles.
 __global__ void __launch_bounds__(kMergeThreads)
 batch_merge_kernel(const BJob *__restrict__ jobs, const BMergeUnit *__restrict__ units, const BCtl *__restrict__ ctl)
 {
-    if (ctl[0].dry != 0)
-        return;
+    (void)ctl;
     const BMergeUnit u = units[blockIdx.x];
     const BJob &J = jobs[u.job];
     if (J.state != CLK_JOB_ACTIVE)
@@ -1374,7 +1375,7 @@ struct clk_batch {
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
     uint32_t stable_steps = 0;  // steps since the job set last changed
-    bool dry = false;           // autotuning: iterations label and accumulate only
+    bool dry = false;           // autotuning: iterations label, accumulate and merge only
     BClassState cls[clk::kClasses];
     // the size classes' tile kernels run concurrently: forked from the
     // step's stream onto these (default priority) and joined before the merge
@@ -2714,14 +2715,13 @@ extern "C" int clk_batch_autotune(clk_batch *b, uint32_t *chosen, void *stream)
     CLK_HIP_CHECK(hipStreamSynchronize(st));
     std::vector<BJob> dev(b->jobs.size());
     CLK_HIP_CHECK(hipMemcpy(dev.data(), b->d_jobs, dev.size() * sizeof(BJob), hipMemcpyDeviceToHost));
+    // (the sums are exact: re-tiling changes no result, before or after a
+    // job's first iteration; the dry passes' labels are those the next
+    // iteration computes anyway)
     uint32_t active = 0;
-    for (size_t s = 0; s < dev.size(); ++s) {
-        if (b->jobs[s].state != CLK_JOB_ACTIVE)
-            continue;
-        CLK_REQUIRE(dev[s].iter == 0, CLK_ERR_INVALID_ARGUMENT,
-                    "clk_batch_autotune: tune before the first iteration (the tiling fixes the summation order)");
-        ++active;
-    }
+    for (size_t s = 0; s < dev.size(); ++s)
+        if (b->jobs[s].state == CLK_JOB_ACTIVE)
+            ++active;
     CLK_REQUIRE(active > 0, CLK_ERR_INVALID_ARGUMENT, "clk_batch_autotune: no active jobs to time");
     // re-tile every active job (records on the host and the device)
     auto retile = [&](uint32_t tw) -> int {

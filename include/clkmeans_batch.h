@@ -88,14 +88,15 @@ int clk_batch_job_buffers(clk_batch *b, int32_t slot, const float **points, floa
  * iteration.  Jobs end in the same states as with clk_batch_step. */
 int clk_batch_run_serial(clk_batch *b, void *stream);
 
-/* Autotune tile_work on the current jobs, before their first iteration:
- * times label + accumulate passes (no job advances) for tile_work in
- * {4, 8, 16, 32, 64, 128, 256}, re-tiles the jobs with the fastest and keeps it for
- * jobs admitted later.  The tiling fixes each job's FP32 summation order, so
- * a job's bits depend on tile_work: pass the chosen value as
- * clk_batch_config.tile_work to reproduce them (the tuned value takes the
- * place of a hand-edited .conf entry).  chosen may be NULL.
- * CLK_ERR_INVALID_ARGUMENT if no job is active or one has iterated. */
+/* Autotune tile_work on the current jobs: times label + accumulate passes
+ * (no job advances) for tile_work in {4, 8, 16, 32, 64, 128, 256}, re-tiles
+ * the jobs with the fastest and keeps it for jobs admitted later (the tuned
+ * value takes the place of a hand-edited .conf entry).  Centroid sums are
+ * exact, so the tiling changes no job's result and the call may come at any
+ * point; after the jobs' first iteration the timed passes include the keep
+ * test that every later iteration runs (the first iteration's full scans
+ * favour smaller tiles).  chosen may be NULL.  CLK_ERR_INVALID_ARGUMENT if
+ * no job is active. */
 int clk_batch_autotune(clk_batch *b, uint32_t *chosen, void *stream);
 
 /* ---- serving: jobs arriving over time --------------------------------- */

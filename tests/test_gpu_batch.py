@@ -263,11 +263,24 @@ def test_batch_autotune_picks_a_tiling_and_reproduces(gpu_ctx, stream):
     for _ in range(5):
         b.step(stream.handle)
     tuned = [b.take(s, stream.handle) for s in slots]
-    with pytest.raises(clkmeans.InvalidArgument):  # only before the first iteration
-        s2 = b.add(*jobs[0], 5, stream=stream.handle)
-        b.step(stream.handle)
+    with pytest.raises(clkmeans.InvalidArgument):  # no active job to time
         b.autotune(stream.handle)
     b.close()
+    # tuning between iterations (the dry passes then include the keep test)
+    # changes no bit: the sums are exact
+    mid = clkmeans.Batch(gpu_ctx, max_jobs=16, arena_bytes=256 << 20, max_d=64, max_k=256, labeling="mfma_fp16")
+    slots = [mid.add(p, c, 5, stream=stream.handle) for p, c in jobs]
+    for _ in range(2):
+        mid.step(stream.handle)
+    assert mid.autotune(stream.handle) in (4, 8, 16, 32, 64, 128, 256)
+    for _ in range(3):
+        mid.step(stream.handle)
+    for s, r in zip(slots, tuned):
+        r2 = mid.take(s, stream.handle)
+        for a, b_ in zip(r[:3], r2[:3]):
+            np.testing.assert_array_equal(a.view(np.uint32), b_.view(np.uint32))
+        assert r2[3] == 5
+    mid.close()
     # the same tile_work in a fresh batch: the same bits; the iteration is the reference's
     again = clkmeans.Batch(gpu_ctx, max_jobs=16, arena_bytes=256 << 20, max_d=64, max_k=256, tile_work=chosen,
                            labeling="mfma_fp16")
