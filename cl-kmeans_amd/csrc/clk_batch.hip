@@ -88,7 +88,8 @@ struct BJob {
     // (zero-padded to 4/8/16/32 floats), refreshed by the finish kernel, for
     // the keep test and the exact rescan of the tile kernel
     float *cen_t;
-    uint32_t cen_t_w, pad2;
+    uint32_t cen_t_w;
+    uint32_t tile_base;  // (host) the tile size of the batch's tile_work; rebuild may use smaller tiles
     // exact sums (clk_grid.hpp): the per-feature max |x| bits of the job's
     // points (its grid), and per cell [d][k] the flags of off-grid values
     // (set by the tiles, consumed and cleared by the finish kernel)
@@ -1195,6 +1196,18 @@ constexpr uint32_t kMfmaClass0 = 12, kMfmaNkb2Class = 16;
 // fma per centroid, so d = 3 gets its own bucket).
 constexpr uint32_t kFp32Dmax[] = {3, 4, 8, 16, 32, 64};
 
+// New tilings of jobs (slot, tile_pts, ntiles triples) into their records.
+__global__ void __launch_bounds__(256)
+batch_retile_kernel(BJob *__restrict__ jobs, const uint32_t *__restrict__ upd, uint32_t count)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
+        return;
+    BJob &J = jobs[upd[3 * i]];
+    J.tile_pts = upd[3 * i + 1];
+    J.ntiles = upd[3 * i + 2];
+}
+
 // The merge of every active job's tile rows: one work-group per element
 // slice of a job (large jobs get many).  It runs in the autotuner's dry
 // passes too (its cost depends on the tiling); the merged row 0 it leaves
@@ -1363,6 +1376,8 @@ struct clk_batch {
     uint32_t nactive = 0;
     clk::BMergeUnit *d_units = nullptr;  // merge work-groups
     uint32_t nunits = 0, unit_cap = 0;
+    uint32_t *d_retile = nullptr;  // (slot, tile_pts, ntiles) of jobs rebuild re-tiled
+    uint32_t retile_cap = 0;
     char *arena = nullptr;
     // host mirror of the job table (device-side fields iter/state/shift are
     // read back by poll / take)
@@ -1425,6 +1440,9 @@ uint32_t tile_points(uint32_t tile_work, uint32_t d, uint32_t k)
 // The autotuner's tile_work candidates; a job's partial rows are allocated for
 // the smallest, so the tuner can re-tile admitted jobs.
 constexpr uint32_t kTileWorkCandidates[] = {4, 8, 16, 32, 64, 128, 256};
+// rebuild halves a class's tiles while it has fewer tiles than this many per
+// resident work-group
+constexpr uint64_t kMinClassTiles = 4;
 // tile_work when the configuration leaves it 0: small tiles keep the
 // persistent work-groups busy when few jobs are resident (serving).
 constexpr uint32_t kDefaultTileWork = 8;
@@ -1551,28 +1569,24 @@ bool class_fits_with(const clk_batch *b, uint32_t c, uint32_t k, uint32_t d)
 int rebuild(clk_batch *b, hipStream_t stream)
 {
     std::vector<BTile> tiles;
-    std::vector<uint32_t> active;
+    std::vector<uint32_t> active, retile;
     std::vector<BCtl> ctl(clk::kClasses, BCtl{0, 0, b->dry ? 1u : 0u, 0});
     for (uint32_t c = 0; c < clk::kClasses; ++c) {
         BClassState &C = b->cls[c];
         C = BClassState{};
         C.tile0 = (uint32_t)tiles.size();
+        std::vector<uint32_t> mem;
         uint32_t kmax = 0, kdmax = 0, dmax = 0;
         for (uint32_t s = 0; s < b->jobs.size(); ++s) {
-            BJob &J = b->jobs[s];
+            const BJob &J = b->jobs[s];
             if (J.state != CLK_JOB_ACTIVE || class_of(J.kind, J.d, J.k) != c)
                 continue;
-            active.push_back(s);
+            mem.push_back(s);
             kmax = std::max(kmax, J.k);
             kdmax = std::max(kdmax, J.k * J.d);
             dmax = std::max(dmax, J.d);
-            J.tile0 = (uint32_t)tiles.size();
-            for (uint32_t t = 0; t < J.ntiles; ++t)
-                tiles.push_back({s, t});
         }
-        C.ntiles = (uint32_t)tiles.size() - C.tile0;
-        ctl[c].ntiles = C.ntiles;
-        if (C.ntiles == 0)
+        if (mem.empty())
             continue;
         // the most waves whose LDS fits the class's jobs (the sums are
         // exact, so the wave count does not change any result)
@@ -1585,8 +1599,67 @@ int rebuild(clk_batch *b, hipStream_t stream)
         const void *kern = class_kernel(b, c);
         if (C.lds > 65536)
             CLK_HIP_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)C.lds));
+        // Tile sizes: each job's base size (the batch's tile_work), halved
+        // -- not below the smallest candidate's, which its partial rows are
+        // sized for -- while the class has fewer than kMinClassTiles tiles
+        // per resident work-group: a class of few, large tiles leaves work-
+        // groups idle behind its last ones.  (The sums are exact: the tiling
+        // changes no result.)
+        const uint64_t resident = clk::auto_grid(b->ctx, kern, (int)(64 * C.waves), C.lds, ~0ull >> 8);
+        std::vector<uint32_t> tp(mem.size());
+        uint64_t total = 0;
+        for (size_t i = 0; i < mem.size(); ++i) {
+            const BJob &J = b->jobs[mem[i]];
+            tp[i] = J.tile_base;
+            total += clk::div_up(J.n, (uint64_t)tp[i]);
+        }
+        while (total < kMinClassTiles * resident) {
+            bool shrunk = false;
+            total = 0;
+            for (size_t i = 0; i < mem.size(); ++i) {
+                const BJob &J = b->jobs[mem[i]];
+                const uint32_t floor_pts = tile_points(kTileWorkCandidates[0], J.d, J.k);
+                const uint32_t half = std::max(floor_pts, tp[i] / 2 / 256 * 256);
+                if (half < tp[i]) {
+                    tp[i] = half;
+                    shrunk = true;
+                }
+                total += clk::div_up(J.n, (uint64_t)tp[i]);
+            }
+            if (!shrunk)
+                break;
+        }
+        for (size_t i = 0; i < mem.size(); ++i) {
+            const uint32_t s = mem[i];
+            BJob &J = b->jobs[s];
+            if (J.tile_pts != tp[i]) {
+                J.tile_pts = tp[i];
+                J.ntiles = (uint32_t)clk::div_up(J.n, (uint64_t)tp[i]);
+                retile.insert(retile.end(), {s, J.tile_pts, J.ntiles});
+            }
+            active.push_back(s);
+            J.tile0 = (uint32_t)tiles.size();
+            for (uint32_t t = 0; t < J.ntiles; ++t)
+                tiles.push_back({s, t});
+        }
+        C.ntiles = (uint32_t)tiles.size() - C.tile0;
+        ctl[c].ntiles = C.ntiles;
         C.grid = (uint32_t)clk::auto_grid(b->ctx, kern, (int)(64 * C.waves), C.lds,
                                           (uint64_t)C.ntiles * 64 * C.waves);
+    }
+    if (!retile.empty()) {  // the device records' tiling
+        const uint32_t count = (uint32_t)(retile.size() / 3);
+        if (retile.size() > b->retile_cap) {
+            if (b->d_retile)
+                CLK_HIP_CHECK(hipFree(b->d_retile));
+            b->retile_cap = (uint32_t)std::max<size_t>(retile.size(), 3 * 1024) * 2;
+            CLK_HIP_CHECK(hipMalloc(&b->d_retile, b->retile_cap * sizeof(uint32_t)));
+        }
+        CLK_HIP_CHECK(hipMemcpyAsync(b->d_retile, retile.data(), retile.size() * sizeof(uint32_t),
+                                     hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(clk::batch_retile_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, b->d_jobs,
+                           b->d_retile, count);
+        CLK_HIP_CHECK(hipGetLastError());
     }
     if (tiles.size() > b->tile_cap) {
         if (b->d_tiles)
@@ -1809,6 +1882,8 @@ extern "C" int clk_batch_destroy(clk_batch *b)
         (void)hipFree(b->d_active);
     if (b->d_units)
         (void)hipFree(b->d_units);
+    if (b->d_retile)
+        (void)hipFree(b->d_retile);
     if (b->h_rec)
         (void)hipHostFree(b->h_rec);
     if (b->h_status)
@@ -1914,6 +1989,7 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
     J.d = d;
     J.k = k;
     J.tile_pts = tp;
+    J.tile_base = tp;
     J.ntiles = (uint32_t)clk::div_up(n, tp);
     J.max_iter = max_iterations;
     J.iter = 0;
@@ -2730,8 +2806,10 @@ extern "C" int clk_batch_autotune(clk_batch *b, uint32_t *chosen, void *stream)
             if (J.state != CLK_JOB_ACTIVE)
                 continue;
             J.tile_pts = tile_points(tw, J.d, J.k);
+            J.tile_base = J.tile_pts;
             J.ntiles = (uint32_t)clk::div_up(J.n, J.tile_pts);
             dev[s].tile_pts = J.tile_pts;
+            dev[s].tile_base = J.tile_base;
             dev[s].ntiles = J.ntiles;
         }
         CLK_HIP_CHECK(hipMemcpy(b->d_jobs, dev.data(), dev.size() * sizeof(BJob), hipMemcpyHostToDevice));
