@@ -351,8 +351,8 @@ def c1_leg(args):
     pts, _ = clkmeans.generate_blobs(n, d, k, sigma=1.0, box=50.0, seed=0, threads=threads)
     init = np.ascontiguousarray(pts[:, :k])
     out = {"workload": "N=2^20 d=4 k=10, sigma=1 blobs in [-50,50]^4, seed 0, first-x init, 30 iterations",
-           "note": "20 MiB per iteration sits in the 256 MB Infinity Cache: the HBM fraction is reported, "
-                   "not judged"}
+           "note": "20 MiB per iteration sits in the 256 MB Infinity Cache: the HBM fraction is reported "
+                   "for reference only"}
     gc, gl, gm = O.lloyd(pts, init, iters, mode=O.MODE_GRID)
     for pipeline, conf_extra in (("single_stage", "[kmeans.fused]\nstrategy = cluster_merge_bounded\n"),
                                  ("three_stage", "[kmeans.labeling]\nstrategy = unroll_vector_bounded\n")):

@@ -118,7 +118,7 @@ def load(path=None):
         return _lib
     path = path or LIB_PATH
     if not os.path.exists(path):
-        raise ImportError("%s is not built; run __graft_entry__.build() (make -C cl-kmeans_amd)" % path)
+        raise ImportError("%s is not built; run make -C cl-kmeans_amd" % path)
     hip.lib()  # the HIP runtime first, so both resolve to one instance
     L = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
     vp, u64, i32, dp = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_double

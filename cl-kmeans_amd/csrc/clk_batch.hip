@@ -1033,7 +1033,7 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
         const WaveBounds w = wave_bounds<NKB, FP16>(X2, J.stats, J.prm);
         uint32_t best[kGroups], second[kGroups];
         rank_all<FP16, NKB>(ltab, lnorm, nt16, lane, h, w.K, kmask, x, best, second);
-        Verdict v[kGroups];
+        MarginDecision v[kGroups];
         uint32_t lab[kGroups];
         bool need = false;
 #pragma unroll

@@ -516,13 +516,13 @@ inline MfmaParams make_params(bool fp16, uint64_t d, uint64_t k, bool subnormals
 // the best key's centroid is the exact argmin; otherwise every centroid that
 // can be the argmin has trunc(score) <= thr (full: bounds unusable, every
 // centroid is a candidate).
-struct Verdict {
+struct MarginDecision {
     uint32_t label;
     bool unique, full;
     float thr;
 };
 
-__device__ __forceinline__ Verdict decide(uint32_t best, uint32_t second, const WaveBounds &w,
+__device__ __forceinline__ MarginDecision decide(uint32_t best, uint32_t second, const WaveBounds &w,
                                           int g, uint32_t k, uint32_t kmask, bool table_ok,
                                           const MfmaParams &prm)
 {
@@ -533,7 +533,7 @@ __device__ __forceinline__ Verdict decide(uint32_t best, uint32_t second, const 
     const float vb = __uint_as_float(b & ~kmask), vs = __uint_as_float(s & ~kmask);
     const float E = w.E0[g] + w.beta * w.K;
     const float ta = 1.01f * (2.0f * E + tr1 * vb + 2.0f * ep1 * (vb * (1.0f + tr1) + E)) + 1e-30f;
-    Verdict v;
+    MarginDecision v;
     // written so that NaN fails every "finite" test
     v.full = !(table_ok && vb >= 0.0f && vb + ta < 1e37f && w.K < 1e37f && w.Xn[g] < 1e18f);
     v.unique = !v.full && (b & kmask) < k && vs - vb > ta;
