@@ -1443,9 +1443,11 @@ constexpr uint32_t kTileWorkCandidates[] = {4, 8, 16, 32, 64, 128, 256};
 // rebuild halves a class's tiles while it has fewer tiles than this many per
 // resident work-group
 constexpr uint64_t kMinClassTiles = 4;
-// tile_work when the configuration leaves it 0: small tiles keep the
-// persistent work-groups busy when few jobs are resident (serving).
-constexpr uint32_t kDefaultTileWork = 8;
+// tile_work when the configuration leaves it 0.  Large tiles write fewer
+// partial rows; rebuild halves the tiles of a class that has too few of them
+// for its work-groups (few jobs resident, as when serving), so a large
+// default costs the sparse classes nothing.
+constexpr uint32_t kDefaultTileWork = 128;
 
 uint32_t row_tile_points(uint32_t tile_work, uint32_t d, uint32_t k)
 {

@@ -35,7 +35,7 @@ typedef struct clk_batch clk_batch;
 typedef struct clk_batch_config {
     uint32_t max_jobs;        /* slots */
     uint64_t arena_bytes;     /* device memory for job data (points, labels, centroids, partials) */
-    uint32_t tile_work;       /* points per tile per cluster (tiles of 1024..16384 points); 0 = 8;
+    uint32_t tile_work;       /* points per tile per cluster (tiles of 1024..16384 points); 0 = 128;
                                  clk_batch_autotune picks one */
     uint32_t max_d;           /* largest d admitted (<= 64) */
     uint32_t max_k;           /* largest k admitted */
