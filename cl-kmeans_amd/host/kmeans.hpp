@@ -12,6 +12,7 @@
 #include <cmath>
 #include <functional>
 #include <memory>
+#include <string>
 #include <thread>
 #include <type_traits>
 #include <utility>
@@ -32,6 +33,24 @@ void pin_sum_grid(const ContextHandle &ctx, const Queue &q, size_t d, size_t n, 
 {
     if constexpr (std::is_same<PointT, float>::value)
         check_status(clk_sum_grid_from_points(ctx.get(), d, n, device_points, q.get()), "clk_sum_grid_from_points");
+}
+
+// The MFMA labelings read a pinned 16-bit copy of the run's device points
+// (clk_labeling_pin_points: contiguous loads in place of d strided rows)
+// when there is device memory for it; without it they read the FP32 points.
+// The labels are the same either way.  Returns whether a copy was pinned.
+template <typename PointT>
+bool pin_mfma_points(const ContextHandle &ctx, const Queue &q, const std::string &labeling, size_t d,
+                     size_t n, const PointT *device_points)
+{
+    if constexpr (!std::is_same<PointT, float>::value) {
+        return false;
+    } else {
+        const int id = clk_strategy_id(CLK_STAGE_LABELING, labeling.c_str());
+        if ((id != CLK_LABELING_MFMA_BF16 && id != CLK_LABELING_MFMA_FP16) || n == 0)
+            return false;
+        return clk_labeling_pin_points(ctx.get(), id, d, n, device_points, q.get()) == CLK_OK;
+    }
 }
 
 // The same from host-resident points [d][n]: per-feature max |x| over the
@@ -253,6 +272,7 @@ public:
     {
         LabelingFactory<PointT, LabelT, ColMajor> f;
         f_labeling = f.create(context, config, *this->measurement);
+        labeling_strategy = config.strategy;
     }
     void set_mass_updater(MassUpdateConfiguration config)
     {
@@ -274,6 +294,7 @@ public:
         if (this->centroids_initializer)
             this->centroids_initializer(queue, d, n, k, buf.points.data(), buf.centroids.data());
         pin_sum_grid(context, queue, d, n, buf.points.data());
+        const bool pinned = pin_mfma_points(context, queue, labeling_strategy, d, n, buf.points.data());
         queue.finish();
 
         auto t0 = std::chrono::steady_clock::now();
@@ -304,10 +325,13 @@ public:
         auto ns = std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0);
         m.add_datapoint().set_name("TotalTime").add_value() = (uint64_t)ns.count();
         buf.download(queue, d, n, k, *this->host_centroids, *this->host_labels, *this->host_masses);
+        if (pinned)
+            check_status(clk_labeling_unpin_points(context.get()), "clk_labeling_unpin_points");
         check_status(clk_sum_grid_clear(context.get()), "clk_sum_grid_clear");
     }
 
 private:
+    std::string labeling_strategy = "unroll_vector";
     ContextHandle context;
     Queue queue;
     LabelingFunction f_labeling;
