@@ -594,6 +594,8 @@ def batch(args):
     ctx.profiling(False)
     batch_ms = e0.elapsed_ms(e1)
     step_ms = [ms for name, _, ms in ctx.records() if name == "BatchStep"]
+    if step_ms:
+        log("batch step ms: " + " ".join("%.2f" % x for x in step_ms))
     states, active = eng.poll(s)
     # the plain per-job loop on the same GPU
     stream.synchronize()
