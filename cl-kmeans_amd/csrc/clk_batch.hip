@@ -83,7 +83,17 @@ struct BJob {
     // at admission).  (The MFMA tiles do not use them: a whole wave of 64
     // points seldom passes there, and the test measured slower.)
     float *keep;
-    uint32_t keep_ok, pad1;
+    uint32_t keep_ok;
+    // FP32-scan jobs with d <= 8: lb[n], per point a lower bound on its
+    // distance to every centroid but its label's (second_bound), rewritten
+    // by every tile pass; valid when bound_ok (set by the finish kernel after
+    // a batch iteration, clear at admission and after an iteration outside
+    // the batch).  drift: an upper bound on how far any centroid that can
+    // still be chosen moved in the last iteration (batch_finalize).
+    uint32_t bound_ok;
+    float *lb;
+    float drift;
+    uint32_t pad1;
     // MFMA jobs with d <= 32: the current centroids as rows [k][cen_t_w]
     // (zero-padded to 4/8/16/32 floats), refreshed by the finish kernel, for
     // the keep test and the exact rescan of the tile kernel
@@ -196,6 +206,44 @@ __host__ __device__ inline size_t batch_lds_bytes(uint32_t waves, uint32_t k, ui
     return acc_bytes(k, kd, dm) + (size_t)k * cen_stride(dm) * 4 + (size_t)waves * 64 * 4;
 }
 
+// ---- second-distance bound of the d <= 8 tiles -----------------------------
+//
+// A scan that returns label a also gives S, the smallest FP32 distance D(x, c)
+// over the other centroids (capped at FLT_MAX; NaN distances, which never win
+// the strict '<' scan, skipped).  D is the reference's fma sequence: within a
+// relative (d+3)*2^-24 of the exact ||x - c||^2, plus at most d*2^-126 from
+// underflow.  So with m = 2^-10, B = sqrt(S (1-m)) (0 below 1e-20) is at most
+// the exact distance from x to every centroid but c_a.  When every centroid
+// that can still be chosen then moves by at most `drift` (exact Euclidean,
+// rounded up), L = (B - drift) (1 - 2^-20) (rounded values; 0 if negative)
+// bounds the new distances the same way, and so on each iteration.  A point
+// whose FP32 distance to its label's new centroid is below L^2 (1-m) keeps
+// the label: every other D(x, c) >= L^2 (1 - (d+3) 2^-24) - d 2^-126 is
+// larger (m dwarfs the rounding terms; thresholds below 1e-20 keep nothing),
+// so c_a is the strict minimum, below FLT_MAX, and the scan returns it
+// (kmeans_naive.cpp:49-63).  A centroid with a NaN or inf coordinate has
+// D(x, c) NaN or >= FLT_MAX and is never chosen over such a point.
+constexpr float kBoundMargin = 1.0f - 0x1p-10f;
+constexpr float kBoundShrink = 1.0f - 0x1p-20f;
+constexpr float kBoundMin = 1e-20f;
+
+__device__ __forceinline__ float second_bound(float s)
+{
+    const float t = s * kBoundMargin;
+    return t >= kBoundMin ? __builtin_sqrtf(t) : 0.0f;
+}
+
+// The running (minimum, second minimum) of the scan: strict '<' replaces the
+// minimum, whose old value (or the new distance) goes into the second one;
+// fminf skips a NaN distance.
+__device__ __forceinline__ void scan_step2(float dist, uint32_t c, float &best, uint32_t &bi, float &sec)
+{
+    const bool lt = dist < best;
+    sec = fminf(sec, lt ? best : dist);
+    best = lt ? dist : best;
+    bi = lt ? c : bi;
+}
+
 // Labels of the lanes in `fail` (at most 32 of the wave's lanes: points that
 // failed the keep test), with the wave's lanes split over them: G = 64 /
 // next_pow2(m) lanes per point, lane r of a group scanning centroids r, r+G,
@@ -204,10 +252,12 @@ __host__ __device__ inline size_t batch_lds_bytes(uint32_t waves, uint32_t k, ui
 // distances -- the label the sequential scan returns (0 when no distance is
 // below FLT_MAX).  Returns the label of the calling lane's own point (lanes
 // outside `fail` get a value to ignore).  slot: 64 words of LDS of the wave.
-template <int DMAX, int S>
+// SEC: also the second minimum of the point's distances (scan_step2; the
+// pairs combine with the loser's minimum joining the second ones) into sec.
+template <int DMAX, int S, bool SEC = false>
 __device__ __forceinline__ uint32_t sparse_scan(const float (&x)[DMAX][1], const float *__restrict__ cen_s,
                                                 uint32_t d, uint32_t k, unsigned long long fail, uint32_t lane,
-                                                uint32_t *slot)
+                                                uint32_t *slot, float *sec_out = nullptr)
 {
     const uint32_t m = (uint32_t)__builtin_popcountll(fail);
     const uint32_t G = m <= 1 ? 64u : m <= 2 ? 32u : m <= 4 ? 16u : m <= 8 ? 8u : m <= 16 ? 4u : 2u;
@@ -222,7 +272,7 @@ __device__ __forceinline__ uint32_t sparse_scan(const float (&x)[DMAX][1], const
 #pragma unroll
     for (int f = 0; f < DMAX; ++f)
         xs[f] = f < (int)d ? __shfl(x[f][0], (int)src) : 0.0f;
-    float best = FLT_MAX;
+    float best = FLT_MAX, sec = FLT_MAX;
     uint32_t bi = 0xFFFFFFFFu;
     for (uint32_t c = r; c < k; c += G) {
         const float *cc = cen_s + (size_t)c * S;
@@ -233,7 +283,9 @@ __device__ __forceinline__ uint32_t sparse_scan(const float (&x)[DMAX][1], const
             t = xs[f] - cc[f];
             dist = __builtin_fmaf(t, t, dist);
         }
-        if (dist < best) {
+        if constexpr (SEC) {
+            scan_step2(dist, c, best, bi, sec);
+        } else if (dist < best) {
             best = dist;
             bi = c;
         }
@@ -241,14 +293,42 @@ __device__ __forceinline__ uint32_t sparse_scan(const float (&x)[DMAX][1], const
     for (uint32_t off = 1; off < G; off <<= 1) {
         const float ob = __shfl_xor(best, (int)off);
         const uint32_t oi = (uint32_t)__shfl_xor((int)bi, (int)off);
-        if (ob < best || (ob == best && oi < bi)) {
+        const bool take = ob < best || (ob == best && oi < bi);
+        if constexpr (SEC)  // symmetric: both lanes of the pair get the same
+            sec = fminf(fminf(sec, __shfl_xor(sec, (int)off)), take ? best : ob);
+        if (take) {
             best = ob;
             bi = oi;
         }
     }
     const uint32_t lab = bi == 0xFFFFFFFFu ? 0u : bi;
     mfma::wave_lds_order();  // slot is written again by the next call
+    if constexpr (SEC)
+        *sec_out = __shfl(sec, (int)(rank * G));
     return (uint32_t)__shfl((int)lab, (int)(rank * G));
+}
+
+// The sequential scan of argmin_lds (one point per lane) with the second
+// minimum of scan_step2.
+template <int DMAX, int S>
+__device__ __forceinline__ void argmin2_lds(const float (&x)[DMAX][1], const float *__restrict__ cen,
+                                            uint32_t k, uint32_t &lab, float &sec)
+{
+    float best = FLT_MAX;
+    uint32_t bi = 0;
+    sec = FLT_MAX;
+    for (uint32_t c = 0; c < k; ++c) {
+        const float *cc = cen + (size_t)c * S;
+        float t = x[0][0] - cc[0];
+        float dist = t * t;
+#pragma unroll
+        for (int f = 1; f < DMAX; ++f) {
+            t = x[f][0] - cc[f];
+            dist = __builtin_fmaf(t, t, dist);
+        }
+        scan_step2(dist, c, best, bi, sec);
+    }
+    lab = bi;
 }
 
 // ---- runs of equal labels in a wave -------------------------------------
@@ -442,8 +522,16 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
     // separates the loads from their use.
     const bool keep_ok = J.keep_ok != 0;
     const bool special = J.gmax[d] != 0;  // the job holds NaN / inf
+    // d <= 8: the second-distance bound (second_bound), kept for every point
+    constexpr bool kBound = DMAX <= 8;
+    const bool bound_ok = kBound && J.bound_ok != 0;
+    const gfloat *lb_in = (const gfloat *)J.lb;
+    float *lb_out = J.lb;
+    const float drift = J.drift;
+    const float shrink = drift == 0.0f ? 1.0f : kBoundShrink;  // B - 0 is exact
     float xn[DMAX];
     uint32_t ln = 0;
+    float bn = 0.0f;
     // byte offsets into the points fit 32 bits (n*d <= 2^30, checked at
     // admission): one base, one 32-bit offset advanced by a row per feature
     const uint32_t row_bytes = (uint32_t)n * 4u;
@@ -466,7 +554,9 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
         }
         // previous labels only matter to jobs with the keep test; the
         // others read one cached word instead of streaming the labels
-        ln = prev_labels[keep_ok ? pc : first];
+        ln = prev_labels[keep_ok || bound_ok ? pc : first];
+        if constexpr (kBound)
+            bn = lb_in[bound_ok ? pc : first];
     };
     fetch(first);
     for (uint64_t base = first; base < end; base += blockDim.x) {
@@ -480,15 +570,19 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
             x[f][0] = (f < (int)dq) ? xn[f] : 0.0f;
         const uint32_t prev = ln;
         const uint32_t a = prev < k ? prev : 0u;
+        const float bprev = bn;
         // the keep threshold is loaded before the next pass's points, so
         // waiting for it does not wait for them
-        const float thr = keep_ok ? keep_thr[a] : 0.0f;
+        const float thr = keep_ok ? keep_thr[a] : -1.0f;
         fetch(base + blockDim.x);
         // a point inside its previous centroid's keep radius keeps the label
-        // (the scan would return it, keep_threshold); the others scan
+        // (the scan would return it, keep_threshold), and so does one closer
+        // to it than its bound allows any other centroid (second_bound); the
+        // others scan
         uint32_t lab[1] = {0u};
         bool keep = false;
-        if (keep_ok) {
+        float bkeep = 0.0f;  // a kept point's bound (0 when none is valid)
+        if (keep_ok || bound_ok) {
             const float *ca = cen_s + (size_t)a * S;
             float t = x[0][0] - ca[0];
             float dist = t * t;
@@ -497,22 +591,38 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
                 t = x[f][0] - ca[f];
                 dist = __builtin_fmaf(t, t, dist);
             }
-            keep = prev < k && dist <= thr;
+            float lsq = 0.0f;
+            if (bound_ok) {
+                bkeep = fmaxf((bprev - drift) * shrink, 0.0f);
+                lsq = bkeep * bkeep * kBoundMargin;
+                lsq = lsq >= kBoundMin ? lsq : 0.0f;
+            }
+            keep = prev < k && (dist <= thr || dist < lsq);
             lab[0] = a;
         }
         const unsigned long long fail = __ballot(valid && !keep);
+        float sec = FLT_MAX;
         if (fail != 0) {
             // few failures: the whole wave rescans those points; many: each
             // failing lane scans its own
             if (__builtin_popcountll(fail) <= 32) {
-                const uint32_t l = sparse_scan<DMAX, S>(x, cen_s, dq, k, fail, lane, slot);
+                const uint32_t l = sparse_scan<DMAX, S, kBound>(x, cen_s, dq, k, fail, lane, slot, &sec);
                 if (valid && !keep)
                     lab[0] = l;
             } else if (valid && !keep) {
-                argmin_lds<DMAX, 1, S>(x, cen_s, k, lab);
+                if constexpr (kBound)
+                    argmin2_lds<DMAX, S>(x, cen_s, k, lab[0], sec);
+                else
+                    argmin_lds<DMAX, 1, S>(x, cen_s, k, lab);
             }
             if (valid && !keep)
                 J.labels[p] = lab[0];
+        }
+        if constexpr (kBound) {
+            // every point's bound for the next iteration (a kept one's is
+            // unchanged when nothing moved)
+            if (valid && !(keep && bound_ok && drift == 0.0f))
+                lb_out[p] = keep ? bkeep : second_bound(sec);
         }
         // grid integers into the work-group's accumulator (a job without
         // NaN / inf is on its grid: no range checks)
@@ -752,11 +862,12 @@ __device__ void batch_finalize(BJob &J, double *red)
     const double *tot = J.part;
     const float *old = J.cen[J.iter & 1];
     float *nxt = J.cen[(J.iter & 1) ^ 1];
-    double local = 0.0;
+    double local = 0.0, moved = 0.0;
     const bool special = J.gmax[d] != 0;  // the job holds NaN / inf: flags may be set
     for (uint32_t c = tid; c < k; c += blockDim.x) {
         const double m = tot[(size_t)k * d + c];
         double sq = 0.0;
+        bool fin_new = true, fin_old = true;
         for (uint32_t f = 0; f < d; ++f) {
             // exact sums, or NaN / inf where off-grid values were flagged
             // (the flags are cleared for the next iteration)
@@ -765,13 +876,20 @@ __device__ void batch_finalize(BJob &J, double *red)
             if (fl)
                 J.flags[i] = 0u;
             const float v = (float)((fl ? special_value(fl) : tot[i]) / m);
+            const float o = old[(size_t)f * k + c];
             nxt[(size_t)f * k + c] = v;
-            const double diff = (double)v - (double)old[(size_t)f * k + c];
+            const double diff = (double)v - (double)o;
             sq += diff * diff;
+            fin_new = fin_new && __builtin_isfinite(v);
+            fin_old = fin_old && __builtin_isfinite(o);
         }
         J.masses[c] = (uint32_t)m;
         if (m != 0.0)
             local = fmax(local, sqrt(sq));
+        // the drift bound covers the centroids that can be chosen next
+        // (finite ones); one that was not finite before is unbounded
+        if (fin_new)
+            moved = fmax(moved, fin_old ? sqrt(sq) : __builtin_inf());
     }
     red[tid] = local;
     __syncthreads();
@@ -792,6 +910,25 @@ __device__ void batch_finalize(BJob &J, double *red)
         J.iter += 1;
         if (J.iter >= J.max_iter || (J.tol > 0.0 && red[0] < J.tol))
             J.state = CLK_JOB_DONE;
+    }
+    if (J.lb == nullptr)
+        return;
+    // drift: the largest move, FP64 from the FP32 coordinates (a few 2^-53
+    // relative), widened by 2^-40 and rounded up to FP32
+    __syncthreads();
+    red[tid] = moved;
+    __syncthreads();
+    if (tid < blockDim.x - p2)
+        red[tid] = fmax(red[tid], red[tid + p2]);
+    __syncthreads();
+    for (uint32_t s = p2 / 2; s > 0; s /= 2) {
+        if (tid < s)
+            red[tid] = fmax(red[tid], red[tid + s]);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        J.drift = __double2float_ru(red[0] * (1.0 + 0x1p-40));
+        J.bound_ok = 1;  // the tiles wrote every point's bound in this iteration
     }
 }
 
@@ -1333,6 +1470,7 @@ solo_finish_kernel(BJob *__restrict__ jobs, uint32_t slot, const double *__restr
     if (J.iter >= J.max_iter || (J.tol > 0.0 && J.shift < J.tol))
         J.state = CLK_JOB_DONE;
     J.keep_ok = 0;
+    J.bound_ok = 0;  // the step wrote no bounds
     status[slot] = BStatus{J.iter, J.state, J.shift};
 }
 
@@ -1423,6 +1561,7 @@ uint64_t job_bytes(uint64_t n, uint32_t d, uint32_t k, uint32_t tile_pts, bool p
     b += clk::align_up(nt * (uint64_t)k * (d + 1) * 8, 256);  // partial rows
     b += clk::align_up((uint64_t)(d + 1) * 4, 256);            // grid (max |x| bits, NaN/inf seen)
     b += clk::align_up((uint64_t)d * k * 4, 256);              // off-grid flags
+    b += d <= 8 ? clk::align_up(n * 4, 256) : 0;               // second-distance bounds
     return b;
 }
 
@@ -1987,6 +2126,12 @@ int stage_job(clk_batch *b, uint64_t n, uint32_t d, uint32_t k, const float *hos
     J.gmax = reinterpret_cast<uint32_t *>(p);
     p += clk::align_up((uint64_t)(d + 1) * 4, 256);
     J.flags = reinterpret_cast<uint32_t *>(p);
+    p += clk::align_up((uint64_t)d * k * 4, 256);
+    // the second-distance bounds of the d <= 8 FP32 tiles (reserved for every
+    // d <= 8 job, used by the FP32-scan ones)
+    J.lb = d <= 8 && !use_mfma(b, d, k) ? reinterpret_cast<float *>(p) : nullptr;
+    J.bound_ok = 0;  // written by the first iteration's tiles
+    J.drift = 0.0f;
     J.n = n;
     J.d = d;
     J.k = k;
@@ -2230,6 +2375,7 @@ extern "C" int clk_batch_run_serial(clk_batch *b, void *stream)
                 break;
             J.iter += 1;
             J.keep_ok = 0;  // the keep thresholds are of older centroids
+            J.bound_ok = 0;  // and the step wrote no bounds
             if (J.tol > 0.0) {
                 hipError_t e = hipMemcpyAsync(&J.shift, shift_d, sizeof(double), hipMemcpyDeviceToHost, st);
                 if (e == hipSuccess)

@@ -125,6 +125,59 @@ def test_batch_keep_radius_paths(gpu_ctx, stream, labeling):
     b.close()
 
 
+def test_batch_second_distance_bound(gpu_ctx, stream):
+    # d <= 8 FP32 tiles also keep a point's label while its distance to its
+    # centroid stays below its lower bound on every other centroid's distance
+    # (shrunk each iteration by the largest centroid move).  Long runs on
+    # overlapping blobs (the bound decides most points), a small k without
+    # keep radii (the bound alone), lattice ties, a duplicated and an empty
+    # (NaN) centroid, NaN points and points whose distances overflow.
+    rng = np.random.default_rng(21)
+    jobs = []
+    for i, (n, d, k, sigma, box, iters) in enumerate([
+            (60000, 3, 256, 1.0, 10.0, 25), (30000, 3, 12, 1.5, 5.0, 25), (40000, 8, 60, 2.0, 10.0, 20),
+            (20000, 4, 30, 1.0, 10.0, 15), (25000, 6, 90, 1.0, 8.0, 20)]):
+        pts, init = make_job(n, d, k, seed=500 + i, sigma=sigma, box=box)
+        if i == 3:
+            init[:, 5] = init[:, 3]  # duplicated centroid
+            init[:, 7] = 1e6         # far away: empty, then NaN
+        jobs.append((pts, init, iters))
+    p = np.arange(-2, 9, dtype=np.float32)
+    lat = np.stack(np.meshgrid(p, p, p, indexing="ij"), 0).reshape(3, -1)
+    lat = np.ascontiguousarray(np.repeat(lat, 3, axis=1))
+    g = np.arange(0, 7, 2, dtype=np.float32)
+    cen = np.ascontiguousarray(np.stack(np.meshgrid(g, g, g, indexing="ij"), 0).reshape(3, -1))
+    jobs.append((lat, cen, 10))  # every odd lattice coordinate ties two centroids
+    pts, init = make_job(12000, 5, 20, seed=520)
+    pts = pts.copy()
+    pts[:, 41] = np.nan
+    pts[:, 900:964] *= np.float32(1e19)
+    pts[:, 2000] = np.inf
+    jobs.append((pts, np.ascontiguousarray(pts[:, rng.choice(np.arange(3000, 12000), 20, replace=False)]), 12))
+    res = []
+    for serial in (False, True):  # the batch, and the plain loop on its buffers
+        b = clkmeans.Batch(gpu_ctx, max_jobs=8, arena_bytes=256 << 20, max_d=8, max_k=256)
+        try:
+            slots = [b.add(p_, c_, it, stream=stream.handle) for p_, c_, it in jobs]
+            if serial:
+                b.run_serial(stream=stream.handle)
+            else:
+                for _ in range(25):
+                    b.step(stream.handle)
+            res.append([b.take(s, stream.handle) for s in slots])
+        finally:
+            b.close()
+    for i, ((p_, c_, it), got, ser) in enumerate(zip(jobs, *res)):
+        assert got[3] == it
+        for x, y in zip(got[:4], ser[:4]):
+            np.testing.assert_array_equal(x, y)
+        if i < len(jobs) - 1:  # the oracle's grid loop (the last job holds NaN / inf)
+            rc, rl, rm = O.lloyd(p_, c_, it, mode=O.MODE_GRID)
+            np.testing.assert_array_equal(got[1], rl)
+            np.testing.assert_array_equal(got[2], rm)
+            np.testing.assert_array_equal(got[0], rc)
+
+
 @pytest.mark.parametrize("labeling", LABELINGS)
 def test_batch_result_independent_of_mix(gpu_ctx, stream, labeling):
     # the same job alone and among 20 others: identical bits
