@@ -493,7 +493,7 @@ __device__ __forceinline__ void write_partial_row(const BAcc &a, uint32_t d, uin
         prow[i] = i < d * k ? (double)a.A[i] * grid_unit(a.scl[i / k]) : (double)a.cnt[i - d * k];
 }
 
-template <int DMAX>
+template <int DMAX, bool FULL>
 __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float *lds, uint32_t waves)
 {
     (void)waves;
@@ -539,8 +539,9 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
     // opaque at every pass (an empty asm), so the per-feature tests (f < d)
     // are evaluated where they are used instead of being hoisted out of the
     // loop as 32 masks that crowd the scalar registers and spill into vector
-    // ones (smaller buckets keep their hoisted masks: they fit).
-    uint32_t dq = d;
+    // ones (smaller buckets keep their hoisted masks: they fit).  FULL
+    // (d == DMAX): every feature is real and the tests vanish.
+    uint32_t dq = FULL ? (uint32_t)DMAX : d;
     auto fetch = [&](uint64_t b0) {
         const uint64_t p = b0 + tid;
         const uint64_t pc = p < end ? p : first;
@@ -562,7 +563,7 @@ __device__ __forceinline__ void batch_tile(const BJob &J, uint32_t tindex, float
     for (uint64_t base = first; base < end; base += blockDim.x) {
         const uint64_t p = base + tid;
         const bool valid = p < end;
-        if constexpr (DMAX >= 32)
+        if constexpr (DMAX >= 32 && !FULL)
             asm volatile("" : "+s"(dq));
         float x[DMAX][1];
 #pragma unroll
@@ -971,7 +972,10 @@ batch_tiles_kernel(const BJob *__restrict__ jobs, const BTile *__restrict__ tile
     extern __shared__ __align__(16) float lds[];
     const uint32_t waves = blockDim.x / 64;
     for_each_tile(jobs, tiles, ctl, [&](const BJob &J, uint32_t index) {
-        batch_tile<DMAX>(J, index, lds, waves);
+        if (J.d == DMAX)
+            batch_tile<DMAX, true>(J, index, lds, waves);
+        else
+            batch_tile<DMAX, false>(J, index, lds, waves);
         __syncthreads();  // the tables are cleared for the next tile
     });
 }
