@@ -983,7 +983,7 @@ batch_tiles_kernel(const BJob *__restrict__ jobs, const BTile *__restrict__ tile
 // Read in place of a missing point or feature (the loads need no masks).
 __device__ const float kZeroWord = 0.0f;
 
-template <bool FP16, int NKB>
+template <bool FP16, int NKB, bool FULL>
 __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, char *lds, const BLayout &Lo)
 {
     using namespace mfma;
@@ -1039,7 +1039,7 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
     auto fetch = [&](uint64_t b0) {
         uint64_t pid[kGroups];
         batch_ids(b0, pid);
-        fetch_points<NKB>(raw, J.pts, n, d, pid, h, &kZeroWord);
+        fetch_points<NKB>(raw, J.pts, n, FULL ? P : d, pid, h, &kZeroWord);
     };
     const bool keep_ok = J.keep_ok != 0;
     bool pre = !keep_ok && wave < nb;  // raw holds this batch's points
@@ -1050,10 +1050,12 @@ __device__ __forceinline__ void batch_tile_mfma(const BJob &J, uint32_t tindex, 
     const uint32_t rounds = (uint32_t)((nb + waves - 1) / waves);
     // d as the rounds see it, opaque at every round (an empty asm): the
     // per-feature tests are evaluated where they are used, not hoisted out
-    // of the loop as masks that crowd the scalar registers
-    uint32_t dq = d;
+    // of the loop as masks that crowd the scalar registers.  FULL (d equal
+    // to the K-blocks' 32 * NKB features): the tests vanish.
+    uint32_t dq = FULL ? (uint32_t)(32 * NKB) : d;
     for (uint32_t rd = 0; rd < rounds; ++rd) {
-        asm volatile("" : "+s"(dq));
+        if constexpr (!FULL)
+            asm volatile("" : "+s"(dq));
         const uint64_t bt = (uint64_t)rd * waves + wave;
         WaveRuns r;
         r.last = false;
@@ -1323,7 +1325,10 @@ batch_tiles_mfma_kernel(const BJob *__restrict__ jobs, const BTile *__restrict__
 {
     extern __shared__ __align__(16) char lds_b[];
     for_each_tile(jobs, tiles, ctl, [&](const BJob &J, uint32_t index) {
-        batch_tile_mfma<FP16, NKB>(J, index, lds_b, Lo);
+        if (J.d == 32 * NKB)
+            batch_tile_mfma<FP16, NKB, true>(J, index, lds_b, Lo);
+        else
+            batch_tile_mfma<FP16, NKB, false>(J, index, lds_b, Lo);
         __syncthreads();
     });
 }
