@@ -154,9 +154,9 @@ class C2:
         for _ in range(warmup):
             iteration()
         self.stream.synchronize()
+        # The step time: HIP events around the K iterations only (per-kernel
+        # events between the launches cost 3-7% of a 0.45 ms step).
         restart()
-        ctx.clear_records()
-        ctx.profiling(True)
         e0, e1 = hip.Event(), hip.Event()
         w0 = time.perf_counter()
         e0.record(self.stream)
@@ -167,13 +167,27 @@ class C2:
         e1.synchronize()
         hip.synchronize()
         wall = time.perf_counter() - w0
+        ms_step = e0.elapsed_ms(e1) / steps
+        # The kernels' durations: the same K iterations from the same start,
+        # every kernel bracketed by HIP events on its stream.
+        restart()
+        ctx.clear_records()
+        ctx.profiling(True)
+        p0, p1 = hip.Event(), hip.Event()
+        p0.record(self.stream)
+        for it in range(steps):
+            ctx.set_iteration(it)
+            iteration()
+        p1.record(self.stream)
+        p1.synchronize()
+        hip.synchronize()
         ctx.profiling(False)
         by = kernel_times(ctx)
         c_in = cen[0].to_host()
         iteration()
         self.stream.synchronize()
-        return {"ms_per_step": e0.elapsed_ms(e1) / steps, "wall": wall, "by_name": by, "c_in": c_in,
-                "labels": self.labels.to_host(), "masses": self.masses.to_host(),
+        return {"ms_per_step": ms_step, "ms_per_step_instrumented": p0.elapsed_ms(p1) / steps, "wall": wall,
+                "by_name": by, "c_in": c_in, "labels": self.labels.to_host(), "masses": self.masses.to_host(),
                 "centroids": cen[0].to_host()}
 
     def full_parity(self, r, O):
@@ -254,7 +268,10 @@ def fused_headline(args, c2):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": hot,
                      "kernel_ms": hot_ms, "bytes_per_launch": bytes_per_launch,
                      "kernel_ms_iterations_1_to_3": [round(x, 4) for x in hot_each[:3]],
-                     "kernel_ms_last": round(hot_each[-1], 4) if hot_each else None},
+                     "kernel_ms_last": round(hot_each[-1], 4) if hot_each else None,
+                     "kernel_ms_source": "a second pass over the same %d iterations from the same start, every "
+                                         "kernel bracketed by HIP events (%.4f ms per step with those events)"
+                                         % (args.steps, run["ms_per_step_instrumented"])},
         "exhaustive": summary(exhaustive, "cluster_merge") if exhaustive else None,
         "cpu_baseline": cpu,
         "wall_s": run["wall"],
@@ -431,8 +448,8 @@ def dense(args):
     for _ in range(args.warmup):
         iteration(lcfg)
     stream.synchronize()
-    ctx.clear_records()
-    ctx.profiling(True)
+    # the step time (events around the K iterations only), then K more
+    # iterations with every kernel bracketed by events for the roofline
     e0, e1 = hip.Event(), hip.Event()
     e0.record(stream)
     for it in range(steps):
@@ -440,8 +457,18 @@ def dense(args):
         iteration(lcfg)
     e1.record(stream)
     e1.synchronize()
-    ctx.profiling(False)
     ms_per_step = e0.elapsed_ms(e1) / steps
+    ctx.clear_records()
+    ctx.profiling(True)
+    p0, p1 = hip.Event(), hip.Event()
+    p0.record(stream)
+    for it in range(steps):
+        ctx.set_iteration(steps + it)
+        iteration(lcfg)
+    p1.record(stream)
+    p1.synchronize()
+    ctx.profiling(False)
+    ms_instrumented = p0.elapsed_ms(p1) / steps
     by = kernel_times(ctx)
     log("dense kernels (avg ms): " + ", ".join("%s=%.4f x%d" % (kk, np.mean(v), len(v)) for kk, v in by.items()))
     hot, hot_ms, _ = hot_kernel(by, ["LabelingMfmaF16" if args.labeling == "mfma_fp16" else "LabelingMfmaBF16",
@@ -480,7 +507,9 @@ def dense(args):
         "kernels_ms": {kk: round(float(np.mean(v)), 4) for kk, v in by.items()},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / MFMA16_PEAK_TFLOPS, "traffic": None, "kernel": hot,
-                     "kernel_ms": hot_ms, "flops_per_launch": flops},
+                     "kernel_ms": hot_ms, "flops_per_launch": flops,
+                     "kernel_ms_source": "the %d iterations after the timed ones, every kernel bracketed by HIP "
+                                         "events (%.4f ms per step with those events)" % (steps, ms_instrumented)},
         "refined_points_last_call": refined,
         "plain_fp32_labeling_ms": plain_ms,
         "parity": {"plain_fp32_labels_equal": plain_equal},
