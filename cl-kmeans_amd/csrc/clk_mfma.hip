@@ -180,13 +180,15 @@ __global__ void __launch_bounds__(64) mfma_prepare_finish_kernel(float *__restri
 
 
 
-// Streaming kernel: work-group of fast_waves(NKB) waves (up to d = 64 four per
-// SIMD sharing one LDS table, so each wave's point loads and LDS reads hide
-// behind the others' MFMA and ranking work -- 8% faster than two per SIMD at
-// d = 64, k = 1024; one per SIMD above, for the registers), each wave labels
-// 64 consecutive points per step.  Points whose margin test fails are
+// Streaming kernel: work-group of fast_waves(NKB) waves (up to d = 64 three
+// per SIMD sharing one LDS table, so each wave's point loads and LDS reads
+// hide behind the others' MFMA and ranking work, and each wave holds two
+// tiles' scores so that its next MFMAs run under its ranking -- at d = 64,
+// k = 1024 2-4% faster than four per SIMD with one tile's scores, which was 8%
+// faster than two per SIMD; one per SIMD above, for the registers), each
+// wave labels 64 consecutive points per step.  Points whose margin test fails are
 // appended to qidx[] / qthr[] (T, above; count at *qlen).
-constexpr int fast_waves(int nkb) { return nkb <= 2 ? 16 : 4; }
+constexpr int fast_waves(int nkb) { return nkb <= 2 ? 12 : 4; }
 
 // Score offset: each wave needs K >= ||x||^2 + E over its points (wave_bounds).
 // With a launch-wide K_L (stats[5]; 0 = none) the norms are staged a second
@@ -243,9 +245,9 @@ label_mfma_kernel(const float *__restrict__ pts, uint64_t n, uint32_t d, uint32_
         uint32_t best[kGroups], second[kGroups];
         if (KL > 0.0f && w.K <= KL) {  // wave-uniform
             w.K = KL;
-            rank_all<FP16, NKB, false>(lds_table, lds_normk, ntiles, lane, h, w.K, kmask, x, best, second);
+            rank_all<FP16, NKB, false, true>(lds_table, lds_normk, ntiles, lane, h, w.K, kmask, x, best, second);
         } else {
-            rank_all<FP16, NKB>(lds_table, lds_norm, ntiles, lane, h, w.K, kmask, x, best, second);
+            rank_all<FP16, NKB, true, true>(lds_table, lds_norm, ntiles, lane, h, w.K, kmask, x, best, second);
         }
 
         // margin test; lane l owns point bt*64 + l (group h, column q)

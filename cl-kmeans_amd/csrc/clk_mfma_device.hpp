@@ -430,11 +430,13 @@ __device__ __forceinline__ WaveBounds wave_bounds(const float (&X2)[kGroups], co
 }
 
 // Best / second keys over all centroid tiles, one tile at a time, the next
-// tile's operands read from LDS while this one is scored and ranked (the
-// kernels also run several waves per SIMD -- four in the streaming labeling
-// -- to cover the MFMA latency; holding two tiles' scores in one wave needed
-// more registers than that leaves).
-template <bool FP16, int NKB, bool ADD_K = true>
+// tile's operands read from LDS while this one is scored and ranked; the
+// kernels run several waves per SIMD to cover the MFMA latency.  PIPE: two
+// score sets in turn, the next tile's MFMAs issued before this tile's
+// ranking, so a wave's own VALU work runs under its matrix work (16 more
+// registers: the streaming labeling takes it with three waves per SIMD; the
+// batch tiles, already at two per SIMD, do not).
+template <bool FP16, int NKB, bool ADD_K = true, bool PIPE = false>
 __device__ __forceinline__ void rank_all(const u16x8 *lds_table, const float *lds_norm,
                                          uint32_t ntiles, uint32_t lane, uint32_t h, float K,
                                          uint32_t kmask, const PointOps<NKB> &x,
@@ -444,10 +446,31 @@ __device__ __forceinline__ void rank_all(const u16x8 *lds_table, const float *ld
     for (int g = 0; g < kGroups; ++g)
         best[g] = second[g] = 0xFFFFFFFFu;
     const uint32_t vkeep = opaque_v(~kmask);
-    for (uint32_t t = 0; t < ntiles; ++t) {
-        f32v4 acc[kGroups];
-        score_tile<FP16, NKB, ADD_K>(acc, load_ops<NKB>(lds_table, lds_norm, t, lane, h), K, x);
-        rank_tile(acc, t, vkeep, best, second);
+    if constexpr (PIPE) {
+        // two score sets in turn: the next tile's MFMAs are issued before
+        // this tile's keys are ranked
+        f32v4 A[kGroups], B[kGroups];
+        score_tile<FP16, NKB, ADD_K>(A, load_ops<NKB>(lds_table, lds_norm, 0, lane, h), K, x);
+        uint32_t t = 0;
+        for (; t + 2 < ntiles; t += 2) {
+            score_tile<FP16, NKB, ADD_K>(B, load_ops<NKB>(lds_table, lds_norm, t + 1, lane, h), K, x);
+            rank_tile(A, t, vkeep, best, second);
+            score_tile<FP16, NKB, ADD_K>(A, load_ops<NKB>(lds_table, lds_norm, t + 2, lane, h), K, x);
+            rank_tile(B, t + 1, vkeep, best, second);
+        }
+        if (t + 1 < ntiles) {
+            score_tile<FP16, NKB, ADD_K>(B, load_ops<NKB>(lds_table, lds_norm, t + 1, lane, h), K, x);
+            rank_tile(A, t, vkeep, best, second);
+            rank_tile(B, t + 1, vkeep, best, second);
+        } else {
+            rank_tile(A, t, vkeep, best, second);
+        }
+    } else {
+        for (uint32_t t = 0; t < ntiles; ++t) {
+            f32v4 acc[kGroups];
+            score_tile<FP16, NKB, ADD_K>(acc, load_ops<NKB>(lds_table, lds_norm, t, lane, h), K, x);
+            rank_tile(acc, t, vkeep, best, second);
+        }
     }
 #pragma unroll
     for (int g = 0; g < kGroups; ++g) {
