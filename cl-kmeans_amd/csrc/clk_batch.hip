@@ -1827,6 +1827,11 @@ int rebuild(clk_batch *b, hipStream_t stream)
     std::vector<BMergeUnit> units;
     for (uint32_t s : active) {
         const BJob &J = b->jobs[s];
+        // a job of one tile has its sums in row 0 already (its values are
+        // counts and unit multiples, never -0, so 0 + row 0 would change no
+        // bit)
+        if (J.ntiles <= 1)
+            continue;
         const uint32_t rl = J.k * (J.d + 1);
         const uint32_t per = clk::kMergeThreads / clk::merge_split(rl, J.ntiles);
         for (uint32_t e0 = 0; e0 < rl; e0 += per)
