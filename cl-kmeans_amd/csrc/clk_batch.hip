@@ -1634,7 +1634,10 @@ constexpr uint32_t kJobStaged = 3;
 // work from the second iteration on).  Measured on the 4096-job mix (d in
 // {3, 8, 16, 32}, k up to 256), solves/s by the k*d threshold: 1024: 18.6k,
 // 1536: 19.4k, 2048: 20.4k, 3072: 21.3k, 4096: 20.5-21.2k, 6144: 19.7k,
-// 12288: 17.1k.  The labels are the same either way.
+// 12288: 17.1k.  With the exact sums, the second-distance bound and the
+// d-equals-bucket tiles: 1536: 31.2k, 2048: 36.9-37.3k, 3072: 38.1k (a lower
+// threshold shortens the first iteration, 10.5 -> 9.9 ms, but lengthens the
+// later ones).  The labels are the same either way.
 constexpr uint64_t kMfmaMinKD = 3072;
 
 bool use_mfma(const clk_batch *b, uint32_t d, uint32_t k)
